@@ -1,0 +1,161 @@
+/*
+ * pdcs.h -- C ABI of libpdcs_hip.so, an MI355X-native (gfx950) PDHG solver for
+ * conic programs with the PDCS algorithm.
+ *
+ *     min c'x  s.t.  G x - h in K,   bl <= x[0:nb] <= bu
+ *     K = {0}^mzero x R+^mnonneg x SOC(soc[0]) x ... x Kexp^nexp x Kexp*^ndexp
+ *
+ * Rows of G follow exactly that cone order (the reference's
+ * recover_soc_cone_rotated_exp_cone_indices, src/pdcs_gpu/rpdhg_alg_gpu_gen.jl:60-118).
+ * G is CSR, 0-based, int32 indices.  Plain pointers and sizes only; no C++ or
+ * framework types cross the boundary.
+ *
+ * Error convention: every function returns PDCS_OK (0) or a negative code;
+ * pdcs_last_error(ctx) returns a message for the calling thread.  One context
+ * per host thread; contexts share nothing.
+ */
+#ifndef PDCS_H
+#define PDCS_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PDCS_OK 0
+#define PDCS_ERR_ARG -1
+#define PDCS_ERR_HIP -2
+#define PDCS_ERR_UNSUPPORTED -3
+#define PDCS_ERR_STATE -4
+
+#define PDCS_FP64 0
+#define PDCS_MIXED 1 /* FP32 iterates + SpMV, FP64 reductions/tests, FP64 polish */
+
+typedef struct pdcs_ctx pdcs_ctx;
+typedef struct pdcs_problem pdcs_problem;
+
+/* Cone layout of the constraint rows (the reference's mGzero, mGnonnegative,
+ * socG, rsocG, expG, dual_expG arguments, rpdhg_alg_gpu_gen.jl:1219-1234).
+ * Rotated SOC rows (nrsoc > 0) are rejected with PDCS_ERR_UNSUPPORTED. */
+typedef struct {
+    int64_t mzero;
+    int64_t mnonneg;
+    int64_t nsoc;
+    const int64_t *soc;
+    int64_t nrsoc;
+    const int64_t *rsoc;
+    int64_t nexp;
+    int64_t ndualexp;
+} pdcs_cones;
+
+/* Solve parameters: the reference's keyword arguments and defaults
+ * (rpdhg_alg_gpu_gen.jl:387-434); pdcs_params_default() fills them. */
+typedef struct {
+    int64_t max_outer_iter;           /* 10000 */
+    int64_t max_inner_iter;           /* 500000 */
+    double abs_tol;                   /* 1e-6 */
+    double rel_tol;                   /* 1e-6 */
+    double time_limit;                /* seconds, +inf */
+    int64_t check_terminate_freq;     /* 2000 */
+    int64_t duality_gap_restart_freq; /* 2000 */
+    int64_t kkt_restart_freq;         /* 2000 */
+    int32_t use_kkt_restart;          /* 0 */
+    int32_t use_duality_gap_restart;  /* 1 */
+    int32_t use_preconditioner;       /* 1: Ruiz(10) + Pock-Chambolle(alpha=1) */
+    int32_t verbose;                  /* 0 */
+    int32_t warm_start;               /* use x0/y0 of pdcs_solve */
+    int32_t precision;                /* PDCS_FP64 or PDCS_MIXED */
+    int32_t polish;                   /* MIXED: FP64 polish from the FP32 iterate (1) */
+    int32_t bisect_intent;            /* 0: duality-gap bisection as the reference parses it */
+    int32_t code_primal_denominator;  /* 0: README primal residual denominator */
+    int32_t use_graph;                /* 1: replay attempt chunks as hipGraphs */
+    int64_t mixed_check_freq;         /* MIXED: FP32 phase check interval (0 = check_terminate_freq) */
+    int64_t polish_check_freq;        /* FP64 polish check interval (0 = check_terminate_freq) */
+    double mixed_switch_tol;          /* MIXED: FP32 phase stops below this README KKT (1e-5) */
+} pdcs_params;
+
+/* Result of a solve; exit codes as the reference's check_termination_criteria
+ * (src/pdcs_gpu/termination.jl:181-260): 0 optimal, 1 iteration limit,
+ * 6 time limit, 8 NaN, 9 Inf. */
+typedef struct {
+    int32_t exit_code;
+    int32_t sequence;        /* 0 = current iterate met the test, 1 = averaged iterate */
+    int64_t iter;            /* PDHG attempts, accepted + rejected (all phases) */
+    int64_t iter_fp32;       /* attempts run in FP32 (MIXED) */
+    int64_t restarts;
+    double pobj, dobj;
+    double rel_primal_res, rel_dual_res, rel_gap; /* README l_inf residuals */
+    double abs_primal_res, abs_dual_res, abs_gap;
+    double l2_rel_primal_res, l2_rel_dual_res;
+    double rel_primal_res_code;                   /* code-variant primal denominator */
+    double omega;
+    double time_setup;       /* upload + transpose + scaling, seconds */
+    double time_solve;       /* main loop (+ polish), seconds: the time-to-tolerance */
+    double time_fp32;        /* MIXED: FP32 phase seconds */
+    double time_kkt;         /* seconds spent in residual checks */
+    double time_restart;     /* seconds spent in restart checks */
+} pdcs_result;
+
+void pdcs_params_default(pdcs_params *p);
+
+/* Context: owns a HIP stream on `device`; replaces the reference's global
+ * cuBLAS handle (create_cublas_handle_inner, src/pdcs_gpu/cuda/few_block_proj.cu:535). */
+int pdcs_ctx_create(int device, pdcs_ctx **out);
+int pdcs_ctx_destroy(pdcs_ctx *ctx);
+const char *pdcs_last_error(pdcs_ctx *ctx);
+
+/* Problem from host arrays (copied to the device; the caller keeps ownership).
+ * Replaces the coeffUnion / probData / Diagonal_preconditioner construction of
+ * rpdhg_gpu_solve (rpdhg_alg_gpu_gen.jl:1219-1400, def_struct.jl:863-946). */
+int pdcs_problem_create_host(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double *c,
+                             const int32_t *rowptr, const int32_t *colidx, const double *vals,
+                             const double *h, const double *bl, const double *bu,
+                             const pdcs_cones *cones, pdcs_problem **out);
+/* Same from device pointers (copied on the device; replaces
+ * rpdhg_gpu_solve_input_gpu_data, rpdhg_alg_gpu_gen.jl:387). */
+int pdcs_problem_create_device(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double *c,
+                               const int32_t *rowptr, const int32_t *colidx, const double *vals,
+                               const double *h, const double *bl, const double *bu,
+                               const pdcs_cones *cones, pdcs_problem **out);
+int pdcs_problem_destroy(pdcs_problem *p);
+
+/* Whole solve (main loop pdhg_main_iter_..._aggressive!,
+ * src/pdcs_gpu/rpdhg_alg_gpu_gen_scaling.jl:770-896).  x0 (n) / y0 (m) are
+ * unscaled host warm starts used when params->warm_start (may be NULL). */
+int pdcs_solve(pdcs_problem *p, const pdcs_params *params, const double *x0, const double *y0,
+               pdcs_result *res);
+
+/* Unscaled solution of the last solve (recover_solution!, postprocess.jl:10):
+ * x (n), y (m), and the dual slack s = c - G'y (n).  Host pointers; any may be NULL. */
+int pdcs_get_solution(pdcs_problem *p, double *x, double *y, double *s);
+
+/* Scaling vectors of the problem (Dr_product n, Dl_product m; preprocess.jl:347-575). */
+int pdcs_get_scaling(pdcs_problem *p, double *Dr, double *Dl);
+
+/* README residuals of host (x, y) against the problem (converge_info_calculation_diagonal!,
+ * rpdhg_alg_gpu_gen_scaling.jl:126-183), computed on the device. */
+int pdcs_evaluate(pdcs_problem *p, const double *x, const double *y, pdcs_result *res);
+
+/* ---- kernel-level entry points (device pointers, run on the context's stream,
+ * synchronous on return) ---- */
+
+/* y = G x for a CSR matrix; precision selects FP64 or FP32 values/vectors
+ * (replaces the cuSPARSE mv! of unionPrimalMV!, def_rpdhg_gen.jl:856-861). */
+int pdcs_spmv(pdcs_ctx *ctx, int32_t precision, int64_t m, int64_t n, const int32_t *rowptr,
+              const int32_t *colidx, const void *vals, const void *x, void *y);
+
+/* In-place projection of y (m) onto the dual cone of K in scaled space with
+ * per-row scale d (the reference's dual proj_diagonal! codes 1/4/6/12/15 of
+ * massive_block_proj / moderate_block_proj / sufficient_block_proj / few_block_proj,
+ * src/pdcs_gpu/cuda/{massive,moderate,sufficient,few}_block_proj.cu).  d: for SOC rows d_i = Dl[start]/Dl[i]; for exp
+ * rows d_i = 1/Dl[i]; elsewhere ignored. */
+int pdcs_project_dual(pdcs_ctx *ctx, int32_t precision, const pdcs_cones *cones, int64_t m,
+                      void *y, const double *d);
+
+/* In-place projection of r (m) onto K itself (con_proj!, codes 2/3/5/13/16). */
+int pdcs_project_primal_cone(pdcs_ctx *ctx, const pdcs_cones *cones, int64_t m, double *r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1137 @@
+/*
+ * pdcs_oracle.c -- CPU FP64 restatement of the PDCS PDHG hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see pdcs_oracle.h).  Every function cites the
+ * reference file:line it restates.  The algorithm is the one the reference GPU
+ * solver runs (src/pdcs_gpu/rpdhg_alg_gpu_gen_scaling.jl:770-896, aggressive
+ * restarted adaptive PDHG with Ruiz + Pock-Chambolle scaling); the cone
+ * projections restate the CPU FP64 versions (src/pdcs_cpu/soc_rsoc_proj.jl,
+ * src/pdcs_cpu/exp_proj.jl).
+ *
+ * Deliberate deviations (documented, all deterministic):
+ *   - the wall-clock `% 3179` extra checks and the rand()-fed infeasibility
+ *     trend buffers (rpdhg_alg_gpu_gen_scaling.jl:241,598,717) are dropped;
+ *   - infeasibility certificates and the pessimistic re-solve
+ *     (rpdhg_alg_gpu_gen_scaling.jl:511-638) are not restated: a trigger is
+ *     reported in oracle_result.pessimistic_flag instead.
+ */
+#include "pdcs_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#define POS_ZERO 1e-20 /* positive_zero, src/pdcs_gpu/PDCS_GPU.jl:27 */
+
+static double now_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+static double *dalloc(int64_t n) { return (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double)); }
+static double dmax(double a, double b) { return a > b ? a : b; }
+static double dmin(double a, double b) { return a < b ? a : b; }
+
+static double dot(const double *a, const double *b, int64_t n) {
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+static double nrm2(const double *a, int64_t n) { return sqrt(dot(a, a, n)); }
+static double nrm1(const double *a, int64_t n) {
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) s += fabs(a[i]);
+    return s;
+}
+static double nrminf(const double *a, int64_t n) {
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) s = dmax(s, fabs(a[i]));
+    return s;
+}
+
+void oracle_default_params(oracle_params *p) {
+    /* defaults of rpdhg_gpu_solve_input_gpu_data, rpdhg_alg_gpu_gen.jl:387-434 */
+    memset(p, 0, sizeof(*p));
+    p->max_outer_iter = 10000;
+    p->max_inner_iter = 500000;
+    p->abs_tol = 1e-6;
+    p->rel_tol = 1e-6;
+    p->time_limit = INFINITY;
+    p->check_terminate_freq = 2000;
+    p->duality_gap_restart_freq = 2000;
+    p->kkt_restart_freq = 2000;
+    p->use_kkt_restart = 0;
+    p->use_duality_gap_restart = 1;
+    p->use_preconditioner = 1;
+    p->ruiz_iters = 10;
+    p->pock_chambolle = 1;
+    p->warm_start = 0;
+    p->verbose = 0;
+    p->bisect_intent = 0;
+    p->code_primal_denominator = 0;
+}
+
+/* ---------------------------------------------------------------- SpMV -- */
+/* unionPrimalMV! / union_adjoint!, src/pdcs_gpu/def_rpdhg_gen.jl:856-869 */
+void oracle_spmv(int64_t m, const int32_t *rowptr, const int32_t *colidx,
+                 const double *vals, const double *x, double *y) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < m; ++i) {
+        double s = 0.0;
+        for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) s += vals[k] * x[colidx[k]];
+        y[i] = s;
+    }
+}
+
+typedef struct {
+    int64_t m, n, nnz;
+    int32_t *rowptr, *colidx;
+    double *vals;
+    /* transpose (CSR of G') with map from transposed slot to original slot */
+    int32_t *trowptr, *tcolidx, *tmap;
+    double *tvals;
+} csr_pair;
+
+static void csr_pair_build(csr_pair *G, int64_t m, int64_t n, const int32_t *rowptr,
+                           const int32_t *colidx, const double *vals) {
+    G->m = m;
+    G->n = n;
+    G->nnz = rowptr[m];
+    G->rowptr = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m + 1));
+    G->colidx = (int32_t *)malloc(sizeof(int32_t) * (size_t)(G->nnz + 1));
+    G->vals = dalloc(G->nnz);
+    memcpy(G->rowptr, rowptr, sizeof(int32_t) * (size_t)(m + 1));
+    memcpy(G->colidx, colidx, sizeof(int32_t) * (size_t)G->nnz);
+    memcpy(G->vals, vals, sizeof(double) * (size_t)G->nnz);
+    G->trowptr = (int32_t *)calloc((size_t)(n + 2), sizeof(int32_t));
+    G->tcolidx = (int32_t *)malloc(sizeof(int32_t) * (size_t)(G->nnz + 1));
+    G->tmap = (int32_t *)malloc(sizeof(int32_t) * (size_t)(G->nnz + 1));
+    G->tvals = dalloc(G->nnz);
+    for (int64_t k = 0; k < G->nnz; ++k) G->trowptr[colidx[k] + 1]++;
+    for (int64_t j = 0; j < n; ++j) G->trowptr[j + 1] += G->trowptr[j];
+    int32_t *fill = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
+    memcpy(fill, G->trowptr, sizeof(int32_t) * (size_t)(n + 1));
+    for (int64_t i = 0; i < m; ++i)
+        for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+            int32_t slot = fill[colidx[k]]++;
+            G->tcolidx[slot] = (int32_t)i;
+            G->tmap[slot] = k;
+        }
+    free(fill);
+    for (int64_t s = 0; s < G->nnz; ++s) G->tvals[s] = G->vals[G->tmap[s]];
+}
+static void csr_pair_free(csr_pair *G) {
+    free(G->rowptr); free(G->colidx); free(G->vals);
+    free(G->trowptr); free(G->tcolidx); free(G->tmap); free(G->tvals);
+}
+static void Gx(const csr_pair *G, const double *x, double *y) {
+    oracle_spmv(G->m, G->rowptr, G->colidx, G->vals, x, y);
+}
+static void Gty(const csr_pair *G, const double *y, double *x) {
+    oracle_spmv(G->n, G->trowptr, G->tcolidx, G->tvals, y, x);
+}
+
+/* --------------------------------------------------------- projections -- */
+/* soc_proj!, src/pdcs_cpu/soc_rsoc_proj.jl:4-18 */
+void oracle_proj_soc(double *v, int64_t len) {
+    double t = v[0];
+    double nx = nrm2(v + 1, len - 1);
+    if (nx <= -t) {
+        for (int64_t i = 0; i < len; ++i) v[i] = 0.0;
+    } else if (nx <= t) {
+        return;
+    } else {
+        double c = (1.0 + t / nx) / 2.0;
+        v[0] = nx * c;
+        for (int64_t i = 1; i < len; ++i) v[i] *= c;
+    }
+}
+
+/* oracle_soc_f_sqrt, soc_rsoc_proj.jl:208-214 */
+static double soc_f_sqrt(double xi, double t, const double *dx, const double *d2, int64_t k) {
+    double s = 0.0;
+    for (int64_t i = 0; i < k; ++i) {
+        double q = dx[i] / (1.0 + 2.0 * xi * d2[i]);
+        s += q * q;
+    }
+    return sqrt(s) - t / (1.0 - 2.0 * xi);
+}
+/* oracle_soc_h, soc_rsoc_proj.jl:216-226 */
+static void soc_h(double xi, double t, const double *dx, const double *d2, int64_t k,
+                  double *f, double *h) {
+    double left = 0.0, left2 = 0.0;
+    for (int64_t i = 0; i < k; ++i) {
+        double q = dx[i] / (1.0 + 2.0 * xi * d2[i]);
+        left += q * q;
+        double q2 = q / sqrt(2.0 * xi + d2[i]);
+        left2 += q2 * q2;
+    }
+    double right = t / (1.0 - 2.0 * xi);
+    right = right * right;
+    *f = left - right;
+    right /= (1.0 - 2.0 * xi);
+    *h = -4.0 * (left2 + right);
+}
+
+/* soc_proj_diagonal!, soc_rsoc_proj.jl:257-373 (with newton_soc_rootsearch :228-255).
+ * Projects onto {(t, z): ||d .* z|| <= t}; d = D_scaled (d[0] is the head, == 1). */
+void oracle_proj_soc_scaled(double *v, const double *d, const double *d2, int64_t len,
+                            double *xi_ws, double abs_tol, double rel_tol, double *tmp) {
+    int64_t k = len - 1;
+    const double *dd = d + 1, *dd2 = d2 + 1;
+    double *x = v + 1;
+    double *dx = tmp; /* D_scaled_part_mul_x_part */
+    /* minVal normalisation :277-284 */
+    double minval = INFINITY;
+    for (int64_t i = 0; i < len; ++i) {
+        double a = fabs(v[i]);
+        if (a > 0 && a < minval) minval = a;
+    }
+    minval = isinf(minval) ? 1.0 : dmax(minval, 1e-3);
+    for (int64_t i = 0; i < len; ++i) v[i] /= minval;
+    double t = v[0];
+    double nxd = 0.0;
+    for (int64_t i = 0; i < k; ++i) nxd += (x[i] / dd[i]) * (x[i] / dd[i]);
+    if (sqrt(nxd) <= -v[0] && v[0] <= 0) { /* :287-293 */
+        for (int64_t i = 0; i < len; ++i) v[i] = 0.0;
+        return;
+    }
+    for (int64_t i = 0; i < k; ++i) dx[i] = dd[i] * x[i];
+    if (nrm2(dx, k) <= v[0]) { /* :295-305 */
+        v[0] = dmax(v[0], 0.0);
+        for (int64_t i = 0; i < len; ++i) v[i] *= minval;
+        return;
+    }
+    double ximid;
+    if (t > rel_tol) { /* :306-351 */
+        double xiR = 0.5, xiL = 0.0, oracleval = 1.0;
+        if (*xi_ws > xiL && *xi_ws < xiR) {
+            oracleval = soc_f_sqrt(*xi_ws, v[0], dx, dd2, k);
+            if (fabs(oracleval) < abs_tol) {
+                ximid = *xi_ws;
+                v[0] = v[0] / (1.0 - 2.0 * ximid);
+                for (int64_t i = 0; i < k; ++i) x[i] /= (1.0 + 2.0 * ximid * dd2[i]);
+                for (int64_t i = 0; i < len; ++i) v[i] *= minval;
+                return;
+            }
+            if (oracleval < 0) xiR = *xi_ws; else xiL = *xi_ws;
+        }
+        /* newton_soc_rootsearch :228-255: narrows the bracket; its root is not used */
+        double xi = 0.5 * (xiR + xiL);
+        for (int it = 0; it < 20; ++it) {
+            double f, df;
+            soc_h(xi, v[0], dx, dd2, k, &f, &df);
+            if (f < 0) xiR = xi; else xiL = xi;
+            if (xiR <= xiL) break;
+            if (isfinite(f) && df < -rel_tol) xi = xi - f / df; else break;
+            if (fabs(f) <= abs_tol) break;
+            xi = dmin(dmax(xi, xiL + rel_tol), xiR - rel_tol);
+        }
+        ximid = xi;
+        while ((xiR - xiL) / (1.0 + xiR + xiL) > rel_tol && fabs(oracleval) > abs_tol) {
+            ximid = 0.5 * (xiR + xiL);
+            oracleval = soc_f_sqrt(ximid, v[0], dx, dd2, k);
+            if (oracleval < 0) xiR = ximid; else xiL = ximid;
+        }
+        *xi_ws = ximid;
+    } else if (t < -rel_tol) { /* :352-372 */
+        double xiR = 1.0, xiL = 0.5;
+        while (soc_f_sqrt(xiR, v[0], dx, dd2, k) < 0) { xiL = xiR; xiR *= 2.0; }
+        double oracleval = 1.0;
+        ximid = 0.5 * (xiR + xiL);
+        while ((xiR - xiL) / (1.0 + xiR + xiL) > rel_tol && fabs(oracleval) > abs_tol) {
+            ximid = 0.5 * (xiR + xiL);
+            oracleval = soc_f_sqrt(ximid, v[0], dx, dd2, k);
+            if (oracleval < 0) xiL = ximid; else xiR = ximid;
+        }
+    } else { /* t == 0, :373-384 */
+        double s = 0.0;
+        for (int64_t i = 0; i < k; ++i) {
+            x[i] /= (1.0 + dd2[i]);
+            s += (dd[i] * x[i]) * (dd[i] * x[i]);
+        }
+        v[0] = sqrt(s);
+        for (int64_t i = 0; i < len; ++i) v[i] *= minval;
+        return;
+    }
+    v[0] = v[0] / (1.0 - 2.0 * ximid);
+    for (int64_t i = 0; i < k; ++i) x[i] /= (1.0 + 2.0 * ximid * dd2[i]);
+    for (int64_t i = 0; i < len; ++i) v[i] *= minval;
+}
+
+/* ---------------------------------------------- exponential cone (Friberg) */
+/* oracle_h / oracle_f, src/pdcs_cpu/exp_proj.jl:1-31 */
+static void exp_h(double r, double s, double t, double rho, double *f, double *df) {
+    double e = exp(rho), en = exp(-rho);
+    *f = ((rho - 1) * r + s) * e - (r - rho * s) * en - (rho * (rho - 1) + 1) * t;
+    *df = (rho * r + s) * e + (r - (rho - 1) * s) * en - (2 * rho - 1) * t;
+}
+static double exp_f(double r, double s, double t, double rho) {
+    double f, df;
+    exp_h(r, s, t, rho, &f, &df);
+    return f;
+}
+/* primal_heuristic / dual_heuristic, exp_proj.jl:33-95 */
+static void exp_primal_heur(double r0, double s0, double t0, double *v, double *dist) {
+    v[0] = dmin(r0, 0); v[1] = 0.0; v[2] = dmax(t0, 0);
+    *dist = sqrt((v[2] - t0) * (v[2] - t0) + (v[1] - s0) * (v[1] - s0) + (v[0] - r0) * (v[0] - r0));
+    if (s0 > 0.0) {
+        double tp = dmax(t0, s0 * exp(r0 / s0));
+        double nd = tp - t0;
+        if (nd < *dist) { v[0] = r0; v[1] = s0; v[2] = tp; *dist = nd; }
+    }
+}
+static void exp_dual_heur(double r0, double s0, double t0, double *v, double *dist) {
+    v[0] = 0.0; v[1] = dmin(s0, 0); v[2] = dmin(t0, 0);
+    *dist = sqrt((v[0] - r0) * (v[0] - r0) + (v[1] - s0) * (v[1] - s0) + (v[2] - t0) * (v[2] - t0));
+    if (r0 > 0.0) {
+        double td = dmin(t0, -r0 * exp(s0 / r0 - 1));
+        double nd = t0 - td;
+        if (nd < *dist) { v[0] = r0; v[1] = s0; v[2] = td; *dist = nd; }
+    }
+}
+/* ppsi / dpsi / pomega / domega, exp_proj.jl:128-191 */
+static double exp_ppsi(double r0, double s0) {
+    double psi;
+    if (r0 > s0) psi = (r0 - s0 + sqrt(r0 * r0 + s0 * s0 - r0 * s0)) / r0;
+    else psi = -s0 / (r0 - s0 - sqrt(r0 * r0 + s0 * s0 - r0 * s0));
+    return ((psi - 1) * r0 + s0) / (psi * (psi - 1) + 1);
+}
+static double exp_dpsi(double r0, double s0) {
+    double psi;
+    if (s0 > r0) psi = (r0 - sqrt(r0 * r0 + s0 * s0 - r0 * s0)) / s0;
+    else psi = (r0 - s0) / (r0 + sqrt(r0 * r0 + s0 * s0 - r0 * s0));
+    return (r0 - psi * s0) / (psi * (psi - 1) + 1);
+}
+static double exp_pomega(double rho) {
+    double val = exp(rho) / (rho * (rho - 1) + 1);
+    if (rho < 2.0) val = dmin(val, exp(2.0) / 3);
+    return val;
+}
+static double exp_domega(double rho) {
+    double val = -exp(-rho) / (rho * (rho - 1) + 1);
+    if (rho > -1.0) val = dmax(val, -exp(1.0) / 3);
+    return val;
+}
+/* rho_bound, exp_proj.jl:204-266 */
+static void exp_rho_bound(double r0, double s0, double t0, double pdist, double ddist,
+                          double *lo, double *hi) {
+    double baselow = -INFINITY, baseupr = INFINITY, low = -INFINITY, upr = INFINITY;
+    double Dp = sqrt(dmax(pdist * pdist - dmin(s0, 0) * dmin(s0, 0), 0.0));
+    double Dd = sqrt(dmax(ddist * ddist - dmin(r0, 0) * dmin(r0, 0), 0.0));
+    if (t0 > 0) low = dmax(low, log(t0 / exp_ppsi(r0, s0)));
+    if (t0 < 0) upr = dmin(upr, -log(-t0 / exp_dpsi(r0, s0)));
+    if (r0 > 0) {
+        baselow = 1 - s0 / r0;
+        low = dmax(low, baselow);
+        double tpu = dmax(1e-12, dmin(Dd, Dp + t0));
+        double palpha = low;
+        double curbnd = dmax(palpha, baselow + tpu / r0 / exp_pomega(palpha));
+        upr = dmin(upr, curbnd);
+    }
+    if (s0 > 0) {
+        baseupr = r0 / s0;
+        upr = dmin(upr, baseupr);
+        double tdl = -dmax(1e-12, dmin(Dp, Dd - t0));
+        double dalpha = upr;
+        double curbnd = dmin(dalpha, baseupr - tdl / s0 / exp_domega(dalpha));
+        low = dmax(low, curbnd);
+    }
+    double a = dmin(low, upr), b = dmax(low, upr);
+    low = dmin(dmax(a, baselow), baseupr);
+    upr = dmin(dmax(b, baselow), baseupr);
+    if (low != upr) {
+        double fl = exp_f(r0, s0, t0, low), fu = exp_f(r0, s0, t0, upr);
+        if (!(fl * fu < 0)) {
+            if (fabs(fl) < fabs(fu) || isnan(fl)) upr = low; else low = upr;
+        }
+    }
+    *lo = low;
+    *hi = upr;
+}
+/* rootsearch_bn, exp_proj.jl:468-489 */
+static double exp_rootsearch_bn(double r0, double s0, double t0, double rl, double rh, double rho0) {
+    double rho = 0.0;
+    for (;;) {
+        double f = exp_f(r0, s0, t0, rho0);
+        if (f < 0.0) rl = rho0; else rh = rho0;
+        rho = 0.5 * (rl + rh);
+        if (fabs(rho - rho0) <= POS_ZERO * dmax(1., fabs(rho)) || rho == rl || rho == rh) break;
+        rho0 = rho;
+    }
+    return rho;
+}
+/* newton_rootsearch, exp_proj.jl:491-532 */
+static double exp_newton_rootsearch(double r0, double s0, double t0, double rl, double rh, double rho0) {
+    int converged = 0;
+    double rho = rho0;
+    const double LODAMP = 0.05, HIDAMP = 0.95, tol = 1e-10;
+    for (int i = 0; i < 20; ++i) {
+        double f, df;
+        exp_h(r0, s0, t0, rho0, &f, &df);
+        if (f < 0.0) rl = rho0; else rh = rho0;
+        if (rh <= rl) { converged = 1; break; }
+        if (isfinite(f) && df > tol) rho = rho0 - f / df; else break;
+        if (fabs(rho - rho0) <= POS_ZERO * dmax(1., fabs(rho))) { converged = 1; break; }
+        if (rho >= rh) rho0 = dmin(LODAMP * rho0 + HIDAMP * rh, rh);
+        else if (rho <= rl) rho0 = dmax(LODAMP * rho0 + HIDAMP * rl, rl);
+        else rho0 = rho;
+    }
+    if (converged) return dmax(rl, dmin(rh, rho));
+    return exp_rootsearch_bn(r0, s0, t0, rl, rh, rho0);
+}
+/* exponent_proj!, exp_proj.jl:635-664 (with projsol_primalexpcone :587-604) */
+void oracle_proj_exp(double *v) {
+    const double tol = 1e-10;
+    double r0 = v[0], s0 = v[1], t0 = v[2];
+    double vp[3], vd[3], pdist, ddist;
+    exp_primal_heur(r0, s0, t0, vp, &pdist);
+    exp_dual_heur(r0, s0, t0, vd, &ddist);
+    double min_dist = dmin(pdist, ddist);
+    double infn = -1;
+    infn = dmax(fabs(vp[0] + vd[0] - r0), infn);
+    infn = dmax(fabs(vp[1] + vd[1] - s0), infn);
+    infn = dmax(fabs(vp[2] + vd[2] - t0), infn);
+    double dvv = vp[0] * vd[0] + vp[1] * vd[1] + vp[2] * vd[2];
+    if (!((s0 <= 0 && r0 <= 0) || min_dist <= tol || (infn <= tol && dvv <= tol))) {
+        double rl, rh;
+        exp_rho_bound(r0, s0, t0, pdist, ddist, &rl, &rh);
+        double rho = exp_newton_rootsearch(r0, s0, t0, rl, rh, 0.5 * (rl + rh));
+        double linrho = (rho - 1) * r0 + s0, er = exp(rho);
+        if (linrho > 0 && isfinite(er)) {
+            double q = rho * (rho - 1) + 1, tmp = linrho / q;
+            double pr = rho * tmp, ps = tmp, pt = er * tmp;
+            double d1 = sqrt((pt - t0) * (pt - t0) + (ps - s0) * (ps - s0) + (pr - r0) * (pr - r0));
+            if (d1 <= pdist) { vp[0] = pr; vp[1] = ps; vp[2] = pt; }
+        }
+    }
+    v[0] = vp[0]; v[1] = vp[1]; v[2] = vp[2];
+}
+/* dualExponent_proj!, exp_proj.jl:741-750 (Moreau: v + P_Kexp(-v)) */
+void oracle_proj_dual_exp(double *v) {
+    double w[3] = {-v[0], -v[1], -v[2]};
+    oracle_proj_exp(w);
+    v[0] += w[0]; v[1] += w[1]; v[2] += w[2];
+}
+/* Scaled exponential cone {v: D v in Kexp}.  The reference's
+ * exponent_proj_diagonal! (exp_proj.jl:666-720) is restated through the
+ * boundary parametrisation v(rho, s) = s (rho/dr, 1/ds, e^rho/dt): the
+ * projection is the best of the heuristic points and the boundary point whose
+ * rho solves <v0,u'>||u||^2 = <v0,u><u,u'> (u = dv/ds), found here by
+ * bracketed bisection over rho.  Parity unpinned against the reference's own
+ * case analysis (rho_bound_diagonal, exp_proj.jl:268-466); pinned instead by
+ * the projection optimality conditions in tests. */
+static double sexp_F(const double *v0, const double *d, double rho) {
+    double dr = d[0], ds = d[1], dt = d[2], e = exp(rho);
+    double u0 = rho / dr, u1 = 1.0 / ds, u2 = e / dt;
+    double p0 = 1.0 / dr, p2 = e / dt;
+    double uu = u0 * u0 + u1 * u1 + u2 * u2, up = u0 * p0 + u2 * p2;
+    double vu = v0[0] * u0 + v0[1] * u1 + v0[2] * u2, vp = v0[0] * p0 + v0[2] * p2;
+    return (vp * uu - vu * up) / (e * uu);
+}
+static int in_sexp(const double *v, const double *d, double tol) {
+    double r = d[0] * v[0], s = d[1] * v[1], t = d[2] * v[2];
+    if (s > 0) return s * exp(r / s) <= t + tol * (1 + fabs(t));
+    return (s >= -tol) && r <= tol && t >= -tol;
+}
+void oracle_proj_exp_scaled(double *v, const double *d) {
+    double v0[3] = {v[0], v[1], v[2]};
+    double best[3], bestd;
+    if (in_sexp(v0, d, 0.0)) return;
+    /* candidate 1: origin / r<=0,s=0 face: (min(r,0), 0, max(t,0)) is in the cone */
+    best[0] = dmin(v0[0], 0); best[1] = 0; best[2] = dmax(v0[2], 0);
+    bestd = sqrt((best[0] - v0[0]) * (best[0] - v0[0]) + v0[1] * v0[1] + (best[2] - v0[2]) * (best[2] - v0[2]));
+    /* candidate 2: boundary ray point; scan rho for sign changes of F */
+    double prev = NAN, prho = 0;
+    for (int k = 0; k <= 800; ++k) {
+        double rho = -40.0 + 0.1 * k;
+        double f = sexp_F(v0, d, rho);
+        if (k > 0 && isfinite(f) && isfinite(prev) && (f == 0 || (f > 0) != (prev > 0))) {
+            double lo = prho, hi = rho, flo = prev;
+            for (int it = 0; it < 200; ++it) {
+                double mid = 0.5 * (lo + hi), fm = sexp_F(v0, d, mid);
+                if ((fm > 0) == (flo > 0)) { lo = mid; flo = fm; } else hi = mid;
+            }
+            double r = 0.5 * (lo + hi), e = exp(r);
+            double u[3] = {r / d[0], 1.0 / d[1], e / d[2]};
+            double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+            double s = (v0[0] * u[0] + v0[1] * u[1] + v0[2] * u[2]) / uu;
+            if (s > 0) {
+                double c[3] = {s * u[0], s * u[1], s * u[2]};
+                double dd = sqrt((c[0] - v0[0]) * (c[0] - v0[0]) + (c[1] - v0[1]) * (c[1] - v0[1]) + (c[2] - v0[2]) * (c[2] - v0[2]));
+                if (dd < bestd) { bestd = dd; best[0] = c[0]; best[1] = c[1]; best[2] = c[2]; }
+            }
+        }
+        prev = f;
+        prho = rho;
+    }
+    v[0] = best[0]; v[1] = best[1]; v[2] = best[2];
+}
+/* dualExponent_proj_diagonal!, exp_proj.jl:722-730.  With C = {w: D w in Kexp},
+ * C* = {y: D^-1 y in Kexp*} and Moreau gives P_{C*}(v) = v + P_C(-v). */
+void oracle_proj_dual_exp_scaled(double *v, const double *d) {
+    double w[3] = {-v[0], -v[1], -v[2]};
+    oracle_proj_exp_scaled(w, d);
+    v[0] += w[0]; v[1] += w[1]; v[2] += w[2];
+}
+
+/* dual-side projection in scaled space: codes 1 (free), 4 (nonneg), 6 (scaled SOC),
+ * 12 (Kexp* scaled), 15 (Kexp scaled); massive_block_proj.cu:1081-1194 dispatch */
+void oracle_proj_dual(const oracle_problem *P, double *y, const double *dsc,
+                      const double *dsc2, double *warm, double abs_tol, double rel_tol) {
+    int64_t off = P->mzero;
+    for (int64_t i = 0; i < P->mnonneg; ++i) y[off + i] = dmax(y[off + i], 0.0);
+    off += P->mnonneg;
+    double *tmp = NULL;
+    int64_t maxq = 0;
+    for (int64_t b = 0; b < P->nsoc; ++b) maxq = P->soc[b] > maxq ? P->soc[b] : maxq;
+    tmp = dalloc(maxq);
+    for (int64_t b = 0; b < P->nsoc; ++b) {
+        oracle_proj_soc_scaled(y + off, dsc + off, dsc2 + off, P->soc[b], &warm[b], abs_tol, rel_tol, tmp);
+        off += P->soc[b];
+    }
+    free(tmp);
+    for (int64_t b = 0; b < P->nexp; ++b) { /* code 12: y in (scaled) Kexp* */
+        double dd[3] = {1.0 / dsc[off], 1.0 / dsc[off + 1], 1.0 / dsc[off + 2]};
+        oracle_proj_dual_exp_scaled(y + off, dd);
+        off += 3;
+    }
+    for (int64_t b = 0; b < P->ndexp; ++b) { /* code 15: y/Dl in Kexp, i.e. dsc .* y in Kexp */
+        oracle_proj_exp_scaled(y + off, dsc + off);
+        off += 3;
+    }
+}
+
+/* con_proj!: projection onto K in the original space (codes 2/3/5/13/16) */
+void oracle_proj_primal_cone(const oracle_problem *P, double *r) {
+    for (int64_t i = 0; i < P->mzero; ++i) r[i] = 0.0;
+    int64_t off = P->mzero;
+    for (int64_t i = 0; i < P->mnonneg; ++i) r[off + i] = dmax(r[off + i], 0.0);
+    off += P->mnonneg;
+    for (int64_t b = 0; b < P->nsoc; ++b) { oracle_proj_soc(r + off, P->soc[b]); off += P->soc[b]; }
+    for (int64_t b = 0; b < P->nexp; ++b) { oracle_proj_exp(r + off); off += 3; }
+    for (int64_t b = 0; b < P->ndexp; ++b) { oracle_proj_dual_exp(r + off); off += 3; }
+}
+
+/* ------------------------------------------------------------- scaling -- */
+/* ruiz_rescaling! (preprocess.jl:521-575) + pock_chambolle_rescaling!
+ * (:424-470, alpha = 1) + scale_data! (:300-345), applied to the CSR values. */
+static void scale_pass(const oracle_problem *P, double *vals, const double *Dr,
+                       const double *Dl, double *Drp, double *Dlp) {
+    for (int64_t i = 0; i < P->m; ++i)
+        for (int32_t k = P->rowptr[i]; k < P->rowptr[i + 1]; ++k)
+            vals[k] /= (Dl[i] * Dr[P->colidx[k]]);
+    for (int64_t j = 0; j < P->n; ++j) Drp[j] *= Dr[j];
+    for (int64_t i = 0; i < P->m; ++i) Dlp[i] *= Dl[i];
+}
+void oracle_scale(const oracle_problem *P, int ruiz_iters, int pock, double *Drp,
+                  double *Dlp, double *vals) {
+    int64_t nnz = P->rowptr[P->m];
+    memcpy(vals, P->vals, sizeof(double) * (size_t)nnz);
+    for (int64_t j = 0; j < P->n; ++j) Drp[j] = 1.0;
+    for (int64_t i = 0; i < P->m; ++i) Dlp[i] = 1.0;
+    double *Dr = dalloc(P->n), *Dl = dalloc(P->m);
+    for (int it = 0; it < ruiz_iters; ++it) {
+        for (int64_t j = 0; j < P->n; ++j) Dr[j] = 0.0;
+        for (int64_t i = 0; i < P->m; ++i) {
+            double rm = 0.0;
+            for (int32_t k = P->rowptr[i]; k < P->rowptr[i + 1]; ++k) {
+                double a = fabs(vals[k]);
+                rm = dmax(rm, a);
+                Dr[P->colidx[k]] = dmax(Dr[P->colidx[k]], a);
+            }
+            Dl[i] = sqrt(rm);
+        }
+        for (int64_t j = 0; j < P->n; ++j) Dr[j] = sqrt(Dr[j]);
+        for (int64_t j = 0; j < P->n; ++j) if (Dr[j] == 0.0) Dr[j] = 1.0;
+        for (int64_t i = 0; i < P->m; ++i) if (Dl[i] == 0.0) Dl[i] = 1.0;
+        scale_pass(P, vals, Dr, Dl, Drp, Dlp);
+    }
+    if (pock) {
+        for (int64_t j = 0; j < P->n; ++j) Dr[j] = 0.0;
+        for (int64_t i = 0; i < P->m; ++i) {
+            double rs = 0.0;
+            for (int32_t k = P->rowptr[i]; k < P->rowptr[i + 1]; ++k) {
+                double a = fabs(vals[k]);
+                rs += a;
+                Dr[P->colidx[k]] += a;
+            }
+            Dl[i] = sqrt(rs);
+        }
+        for (int64_t j = 0; j < P->n; ++j) Dr[j] = sqrt(Dr[j]);
+        for (int64_t j = 0; j < P->n; ++j) if (Dr[j] == 0.0) Dr[j] = 1.0;
+        for (int64_t i = 0; i < P->m; ++i) if (Dl[i] == 0.0) Dl[i] = 1.0;
+        scale_pass(P, vals, Dr, Dl, Drp, Dlp);
+    }
+    free(Dr);
+    free(Dl);
+}
+
+/* ----------------------------------------------------------------- KKT -- */
+typedef struct {
+    double pobj, dobj, abs_gap, rel_gap;
+    double l2_abs_p, l2_rel_p, linf_abs_p, linf_rel_p, linf_rel_p_code;
+    double l2_abs_d, l2_rel_d, linf_abs_d, linf_rel_d;
+} conv_info;
+
+typedef struct {
+    const oracle_problem *P;
+    double hN1, hNinf, cN1, cNinf;
+    double *blf, *buf, *laml, *lamu;
+    double *tmp_n, *tmp_n2, *tmp_m, *tmp_m2;
+    csr_pair *G; /* raw data */
+} kkt_ws;
+
+/* converge_info_calculation_diagonal!, rpdhg_alg_gpu_gen_scaling.jl:126-183,
+ * evaluated on the unscaled data (raw_data) */
+static void converge_info(kkt_ws *W, const double *x, const double *y, conv_info *ci) {
+    const oracle_problem *P = W->P;
+    int64_t n = P->n, m = P->m, nb = P->nb;
+    double pobj = dot(x, P->c, n);
+    double *aty = W->tmp_n, *lam = W->tmp_n2;
+    Gty(W->G, y, aty);
+    double atyinf = nrminf(aty, n), atyn1 = nrm1(aty, n);
+    for (int64_t j = 0; j < n; ++j) lam[j] = P->c[j] - aty[j];
+    double dobj = dot(P->h, y, m);
+    for (int64_t j = 0; j < nb; ++j) dobj += W->blf[j] * dmax(lam[j], 0.0) + W->buf[j] * dmin(lam[j], 0.0);
+    double abs_gap = fabs(pobj - dobj);
+    double rel_gap = abs_gap / (1 + dmax(fabs(pobj), fabs(dobj)));
+    double *ax = W->tmp_m, *pr = W->tmp_m2;
+    Gx(W->G, x, ax);
+    double axinf = nrminf(ax, m), axn1 = nrm1(ax, m);
+    for (int64_t i = 0; i < m; ++i) pr[i] = ax[i] - P->h[i];
+    for (int64_t i = 0; i < m; ++i) ax[i] = pr[i]; /* keep Gx - h */
+    oracle_proj_primal_cone(P, pr);
+    double projinf = nrminf(pr, m);
+    for (int64_t i = 0; i < m; ++i) pr[i] = pr[i] - ax[i];
+    /* slack_proj!: clamp lambda to [lambd_l, lambd_u] (gen_lambd, def_rpdhg_gen.jl:5-17);
+     * entries past nb are free variables whose reduced cost must vanish */
+    double sinf = 0.0, sn1 = 0.0, d2 = 0.0, dinf = 0.0;
+    for (int64_t j = 0; j < n; ++j) {
+        double s = (j < nb) ? dmin(dmax(lam[j], W->laml[j]), W->lamu[j]) : 0.0;
+        sinf = dmax(sinf, fabs(s));
+        sn1 += fabs(s);
+        double r = s - lam[j];
+        d2 += r * r;
+        dinf = dmax(dinf, fabs(r));
+    }
+    ci->pobj = pobj;
+    ci->dobj = dobj;
+    ci->abs_gap = abs_gap;
+    ci->rel_gap = rel_gap;
+    ci->l2_abs_p = nrm2(pr, m);
+    ci->l2_rel_p = ci->l2_abs_p / (1 + dmax(sn1, dmax(W->hN1, axn1)));
+    ci->linf_abs_p = nrminf(pr, m);
+    ci->linf_rel_p = ci->linf_abs_p / (1 + dmax(W->hNinf, dmax(axinf, projinf)));
+    ci->linf_rel_p_code = ci->linf_abs_p / (1 + dmax(sinf, dmax(W->hNinf, axinf)));
+    ci->l2_abs_d = sqrt(d2);
+    ci->l2_rel_d = ci->l2_abs_d / (1 + dmax(W->cN1, atyn1));
+    ci->linf_abs_d = dinf;
+    ci->linf_rel_d = dinf / (1 + dmax(W->cNinf, atyinf));
+}
+
+static void kkt_ws_init(kkt_ws *W, const oracle_problem *P, csr_pair *Graw) {
+    W->P = P;
+    W->G = Graw;
+    W->hN1 = nrm1(P->h, P->m);
+    W->hNinf = nrminf(P->h, P->m);
+    W->cN1 = nrm1(P->c, P->n);
+    W->cNinf = nrminf(P->c, P->n);
+    W->blf = dalloc(P->nb); W->buf = dalloc(P->nb);
+    W->laml = dalloc(P->nb); W->lamu = dalloc(P->nb);
+    for (int64_t j = 0; j < P->nb; ++j) {
+        W->blf[j] = isinf(P->bl[j]) ? 0.0 : P->bl[j];
+        W->buf[j] = isinf(P->bu[j]) ? 0.0 : P->bu[j];
+        W->laml[j] = (P->bu[j] == INFINITY) ? 0.0 : -INFINITY;
+        W->lamu[j] = (P->bl[j] == -INFINITY) ? 0.0 : INFINITY;
+    }
+    W->tmp_n = dalloc(P->n); W->tmp_n2 = dalloc(P->n);
+    W->tmp_m = dalloc(P->m); W->tmp_m2 = dalloc(P->m);
+}
+static void kkt_ws_free(kkt_ws *W) {
+    free(W->blf); free(W->buf); free(W->laml); free(W->lamu);
+    free(W->tmp_n); free(W->tmp_n2); free(W->tmp_m); free(W->tmp_m2);
+}
+
+void oracle_kkt(const oracle_problem *P, const double *x, const double *y, oracle_result *res) {
+    csr_pair G;
+    csr_pair_build(&G, P->m, P->n, P->rowptr, P->colidx, P->vals);
+    kkt_ws W;
+    kkt_ws_init(&W, P, &G);
+    conv_info ci;
+    converge_info(&W, x, y, &ci);
+    memset(res, 0, sizeof(*res));
+    res->pobj = ci.pobj; res->dobj = ci.dobj;
+    res->rel_primal_res = ci.linf_rel_p; res->rel_dual_res = ci.linf_rel_d; res->rel_gap = ci.rel_gap;
+    res->abs_primal_res = ci.linf_abs_p; res->abs_dual_res = ci.linf_abs_d; res->abs_gap = ci.abs_gap;
+    res->l2_rel_primal_res = ci.l2_rel_p; res->l2_rel_dual_res = ci.l2_rel_d;
+    res->rel_primal_res_code = ci.linf_rel_p_code;
+    kkt_ws_free(&W);
+    csr_pair_free(&G);
+}
+
+/* -------------------------------------------------------------- solver -- */
+typedef struct {
+    const oracle_problem *P;
+    oracle_params prm;
+    int64_t n, m, nb;
+    csr_pair G;      /* scaled */
+    csr_pair Graw;   /* original */
+    double *c, *h, *bl, *bu;       /* scaled data */
+    double *Drp, *Dlp, *dsc, *dsc2;
+    double *x, *xl, *xm, *y, *yl, *ym;   /* primal_sol / _lag / _mean (scaled) */
+    double *x0, *y0, *x0l, *y0l;         /* restart anchors */
+    double *tn1, *tn2, *tn3, *tm1, *tm2, *tm3, *tm4; /* scratch */
+    double *h1, *h2;
+    double *warm_dual;                   /* per-SOC-block xi warm start */
+    double *xr, *yr, *xmr, *ymr;         /* recovered */
+    kkt_ws W;
+    /* state */
+    double tau, sigma, omega, proj_abs_tol, proj_rel_tol;
+    int64_t iter, iter_stepsize, restart_used;
+    double max_kkt, min_kkt, bs_t0;
+    int restart_dg_flag;
+    double dg_threshold, kkt_threshold, kkt_err[2];
+    conv_info ci[2];
+    int exit_code, exit_seq;
+    double t_start, t_loop;
+    int64_t attempt_limit; /* >0: stop after this many attempts (CPU baseline sample) */
+} solver;
+
+static void proj_primal(solver *S, double *x) { /* box_proj_diagonal! code 18 */
+    for (int64_t j = 0; j < S->nb; ++j) x[j] = dmin(dmax(x[j], S->bl[j]), S->bu[j]);
+}
+static void proj_dual(solver *S, double *y) {
+    oracle_proj_dual(S->P, y, S->dsc, S->dsc2, S->warm_dual, S->proj_abs_tol, S->proj_rel_tol);
+}
+
+/* Mnorm, rpdhg_alg_gpu_gen.jl:133-138 */
+static double Mnorm(solver *S, const double *x, const double *y, double tau, double sigma) {
+    Gx(&S->G, x, S->tm4);
+    double yax = dot(y, S->tm4, S->m);
+    return dot(x, x, S->n) / tau - 2 * yax + dot(y, y, S->m) / sigma;
+}
+
+/* approximate_cal_diagonal!, rpdhg_alg_gpu_gen_scaling.jl:1-13 */
+static void approx_cal(solver *S, double t, const double *primal, const double *dual,
+                       double tau, double sigma, double *xs, double *ys) {
+    for (int64_t j = 0; j < S->n; ++j) xs[j] = primal[j] + (t * tau / 2) * S->h1[j];
+    proj_primal(S, xs);
+    for (int64_t i = 0; i < S->m; ++i) ys[i] = dual[i] + (t * sigma / 2) * S->h2[i];
+    proj_dual(S, ys);
+}
+static double bs_eval(solver *S, double t, const double *primal, const double *dual,
+                      double tau, double sigma, double *xs, double *ys) {
+    approx_cal(S, t, primal, dual, tau, sigma, xs, ys);
+    for (int64_t j = 0; j < S->n; ++j) S->tn3[j] = xs[j] - primal[j];
+    for (int64_t i = 0; i < S->m; ++i) S->tm3[i] = ys[i] - dual[i];
+    return Mnorm(S, S->tn3, S->tm3, tau, sigma);
+}
+
+/* binary_search_duality_gap_diagonal!, rpdhg_alg_gpu_gen_scaling.jl:19-117 */
+static double binary_search_dg(solver *S, double r, const double *primal, const double *dual,
+                               double tau, double sigma) {
+    double t = S->bs_t0, tR = t, tL = 0.0;
+    double *xs = S->tn1, *ys = S->tm1;
+    Gty(&S->G, dual, S->h1);
+    for (int64_t j = 0; j < S->n; ++j) S->h1[j] -= S->c[j];
+    Gx(&S->G, primal, S->h2);
+    for (int64_t i = 0; i < S->m; ++i) S->h2[i] = S->h[i] - S->h2[i];
+    for (int k = 1; k <= 1000; ++k) {
+        double v = bs_eval(S, t, primal, dual, tau, sigma, xs, ys);
+        if (v > r && v < r + 1e-6) {
+            tR = t;
+            tL = dmax(t - 0.01, 0.0);
+            v = bs_eval(S, tL, primal, dual, tau, sigma, xs, ys);
+            if (v < r) break;
+        }
+        if (v < r && v > r - 1e-6) {
+            tL = t;
+            tR = t + 0.01;
+            v = bs_eval(S, tR, primal, dual, tau, sigma, xs, ys);
+            if (v > r) break;
+        }
+        if (v > r) {
+            if (k == 1) break;
+            if (S->prm.bisect_intent) { tR = t; tL = t / 2; }
+            break;
+        }
+        t *= 2;
+    }
+    double tM = (tR + tL) / 2;
+    int64_t count = 0;
+    while (tR - tL > 1e-6) {
+        tM = (tR + tL) / 2;
+        double v = bs_eval(S, tM, primal, dual, tau, sigma, xs, ys);
+        if (v < r) tL = tM; else tR = tM;
+        if (++count > 1000000) break;
+    }
+    S->bs_t0 = tM;
+    for (int64_t i = 0; i < S->m; ++i) S->tm3[i] = ys[i] - dual[i];
+    for (int64_t j = 0; j < S->n; ++j) S->tn3[j] = xs[j] - primal[j];
+    double v1 = dot(S->h1, S->tn3, S->n), v2 = dot(S->h2, S->tm3, S->m);
+    return (v1 + v2) / r;
+}
+
+/* kkt_error_diagonal!, rpdhg_alg_gpu_gen_scaling.jl:119-122 */
+static double kkt_error(double omega, const conv_info *c) {
+    return sqrt(omega * omega * c->l2_abs_p * c->l2_abs_p + c->l2_abs_d * c->l2_abs_d / (omega * omega) + c->abs_gap * c->abs_gap);
+}
+
+/* recover_solution!, postprocess.jl:10-22 */
+static void recover(solver *S) {
+    for (int64_t j = 0; j < S->n; ++j) { S->xr[j] = S->x[j] / S->Drp[j]; S->xmr[j] = S->xm[j] / S->Drp[j]; }
+    for (int64_t i = 0; i < S->m; ++i) { S->yr[i] = S->y[i] / S->Dlp[i]; S->ymr[i] = S->ym[i] / S->Dlp[i]; }
+}
+
+/* optimality_criteria_met + check_termination_criteria, termination.jl:74-91,181-260 */
+static int check_termination(solver *S) {
+    for (int q = 0; q < 2; ++q) {
+        const conv_info *c = &S->ci[q];
+        double rp = S->prm.code_primal_denominator ? c->linf_rel_p_code : c->linf_rel_p;
+        int opt = 0;
+        if (dmax(rp, dmax(c->linf_rel_d, c->rel_gap)) < S->prm.rel_tol) opt = 1;
+        if (dmax(c->linf_abs_p, dmax(c->linf_abs_d, c->abs_gap)) < S->prm.abs_tol) opt = 1;
+        if (isnan(c->rel_gap)) { S->exit_code = 8; S->exit_seq = q; return 1; }
+        if (isinf(c->rel_gap)) { S->exit_code = 9; S->exit_seq = q; return 1; }
+        if (opt) { S->exit_code = 0; S->exit_seq = q; return 1; }
+    }
+    if (S->iter >= S->prm.max_outer_iter * S->prm.max_inner_iter) { S->exit_code = 1; S->exit_seq = 0; return 1; }
+    if (now_s() - S->t_start > S->prm.time_limit) { S->exit_code = 6; S->exit_seq = 0; return 1; }
+    return 0;
+}
+
+/* exit_condition_check_diagonal!, rpdhg_alg_gpu_gen_scaling.jl:309-352 */
+static int exit_check(solver *S) {
+    recover(S);
+    converge_info(&S->W, S->xr, S->yr, &S->ci[0]);
+    converge_info(&S->W, S->xmr, S->ymr, &S->ci[1]);
+    return check_termination(S);
+}
+
+/* restart_threshold_calculation_diagonal!, rpdhg_alg_gpu_gen_scaling.jl:354-393 */
+static void restart_threshold(solver *S, int dg) {
+    if (dg) {
+        for (int64_t j = 0; j < S->n; ++j) S->tn2[j] = S->x0[j] - S->x0l[j];
+        for (int64_t i = 0; i < S->m; ++i) S->tm2[i] = S->y0[i] - S->y0l[i];
+        double r = Mnorm(S, S->tn2, S->tm2, S->tau, S->sigma);
+        S->dg_threshold = binary_search_dg(S, r, S->x0, S->y0, S->tau, S->sigma);
+        if (S->dg_threshold < 0) S->restart_dg_flag = 0;
+    } else {
+        recover(S);
+        converge_info(&S->W, S->xr, S->yr, &S->ci[0]);
+        S->kkt_threshold = kkt_error(S->omega, &S->ci[0]);
+    }
+}
+
+/* restart_condition_check_diagonal!, rpdhg_alg_gpu_gen_scaling.jl:395-509 */
+static int restart_check(solver *S, int64_t inner_iter, int dg) {
+    if (dg) {
+        for (int64_t j = 0; j < S->n; ++j) S->tn2[j] = S->x[j] - S->x0[j];
+        for (int64_t i = 0; i < S->m; ++i) S->tm2[i] = S->y[i] - S->y0[i];
+        double r = Mnorm(S, S->tn2, S->tm2, S->tau, S->sigma);
+        double rho = binary_search_dg(S, r, S->x, S->y, S->tau, S->sigma);
+        double rho_m = binary_search_dg(S, r, S->xm, S->ym, S->tau, S->sigma);
+        for (int cond = 0; cond < 2; ++cond) {
+            double beta = cond == 0 ? 0.4 : 0.8;
+            if (rho < beta * S->dg_threshold || rho_m < beta * S->dg_threshold) {
+                S->restart_used++;
+                if (rho < rho_m && S->restart_dg_flag) {
+                    memcpy(S->xm, S->x, sizeof(double) * (size_t)S->n);
+                    memcpy(S->ym, S->y, sizeof(double) * (size_t)S->m);
+                }
+                return 1;
+            }
+        }
+    } else {
+        recover(S);
+        converge_info(&S->W, S->xr, S->yr, &S->ci[0]);
+        converge_info(&S->W, S->xmr, S->ymr, &S->ci[1]);
+        double lag0 = S->kkt_err[0], lag1 = S->kkt_err[1];
+        double e0 = kkt_error(S->omega, &S->ci[0]), e1 = kkt_error(S->omega, &S->ci[1]);
+        S->kkt_err[0] = e0;
+        S->kkt_err[1] = e1;
+        int c1 = e0 < 0.4 * S->kkt_threshold || e1 < 0.4 * S->kkt_threshold;
+        int c2 = (e0 < 0.8 * S->kkt_threshold && e0 > lag0) || (e1 < 0.8 * S->kkt_threshold && e1 > lag1);
+        if (c1 || c2) {
+            S->restart_used++;
+            if (e0 < e1) {
+                memcpy(S->xm, S->x, sizeof(double) * (size_t)S->n);
+                memcpy(S->ym, S->y, sizeof(double) * (size_t)S->m);
+            }
+            return 1;
+        }
+    }
+    if (inner_iter >= S->iter * 0.223 && S->iter > 100000) { S->restart_used++; return 1; }
+    return 0;
+}
+
+/* initialize_primal_weight, rpdhg_alg_gpu_gen_scaling.jl:271-282 */
+static double init_primal_weight(solver *S) {
+    double hn = nrm2(S->h, S->m), cn = nrm2(S->c, S->n);
+    return (hn > 1e-10 && cn > 1e-10) ? cn / hn : 1.0;
+}
+/* dynamic_primal_weight_update, rpdhg_alg_gpu_gen_scaling.jl:284-307 */
+static double primal_weight_update(solver *S, const double *dx, const double *dy, double omega) {
+    double xn = nrm2(dx, S->n), yn = nrm2(dy, S->m);
+    int64_t ru = S->restart_used;
+    if (omega > 10000 || omega < 1.0 / 10000)
+        if (ru % 20 == 0 && ru != 0) return init_primal_weight(S);
+    if (omega > 30000 || omega < 1.0 / 30000)
+        if (ru % 10 == 0 && ru != 0) return init_primal_weight(S);
+    if (xn > 1e-10 && yn > 1e-10) return exp(0.5 * log(yn / xn) + 0.5 * log(omega));
+    return omega;
+}
+
+static double max_l2_kkt(const conv_info *c) { return dmax(dmax(c->l2_rel_p, c->l2_rel_d), c->rel_gap); }
+
+/* pdhg_one_iter_diagonal_rescaling_adaptive_step_size!, rpdhg_alg_gpu_gen_scaling.jl:656-767.
+ * Returns 1 if the solve must exit. */
+static int adaptive_step(solver *S, double *eta_io, double *eta_prime_out, int *force) {
+    double eta = dmax(*eta_io, 1e-6);
+    int64_t n = S->n, m = S->m;
+    memcpy(S->xl, S->x, sizeof(double) * (size_t)n);
+    memcpy(S->yl, S->y, sizeof(double) * (size_t)m);
+    double dec = dmin(0.9 + 0.1 * log10(S->max_kkt * 1e5), 1.0);
+    double *aty = S->tn1, *xbar = S->tn2, *gx = S->tm1, *dy = S->tm2, *atdy = S->tn3;
+    int64_t count = 0;
+    *force = 0;
+    for (;;) {
+        S->iter++;
+        S->iter_stepsize++;
+        if (S->attempt_limit > 0 && S->iter > S->attempt_limit) { S->iter--; S->exit_code = 1; return 1; }
+        Gty(&S->G, S->yl, aty);
+        S->tau = eta / S->omega;
+        double ps = S->tau * dec;
+        for (int64_t j = 0; j < n; ++j) S->x[j] = S->xl[j] + ps * (aty[j] - S->c[j]);
+        proj_primal(S, S->x);
+        for (int64_t j = 0; j < n; ++j) xbar[j] = 2.0 * S->x[j] - S->xl[j];
+        Gx(&S->G, xbar, gx);
+        S->sigma = eta * S->omega;
+        double ds = S->sigma * dec;
+        for (int64_t i = 0; i < m; ++i) S->y[i] = S->yl[i] - ds * (gx[i] - S->h[i]);
+        proj_dual(S, S->y);
+        for (int64_t i = 0; i < m; ++i) dy[i] = S->y[i] - S->yl[i];
+        for (int64_t j = 0; j < n; ++j) xbar[j] = S->x[j] - S->xl[j]; /* dx */
+        double num = S->omega * dot(xbar, xbar, n) + dot(dy, dy, m) / S->omega;
+        Gty(&S->G, dy, atdy);
+        double den = 2 * fabs(dot(xbar, atdy, n));
+        double eta_bar = num / (den + POS_ZERO);
+        double k1 = (double)(S->iter_stepsize + 1);
+        double eta_prime = dmin((1 - pow(k1, -0.3)) * eta_bar, (1 + pow(k1, -0.6)) * eta);
+        if (S->iter % S->prm.check_terminate_freq == 0) {
+            int ex = exit_check(S);
+            S->max_kkt = max_l2_kkt(&S->ci[0]);
+            double avg = (S->ci[0].l2_rel_p + S->ci[0].l2_rel_d + S->ci[0].rel_gap) / 3;
+            S->proj_abs_tol = dmax(dmin(S->proj_abs_tol, dmin(avg * 1e-9, 1e-7)), 5e-16);
+            S->proj_rel_tol = dmax(dmin(S->proj_rel_tol, dmin(avg * 1e-9, 1e-7)), 5e-16);
+            if (S->prm.verbose)
+                fprintf(stderr, "[oracle] iter %ld relp %.3e reld %.3e gap %.3e pobj %.8e dobj %.8e restarts %ld\n",
+                        (long)S->iter, S->ci[0].linf_rel_p, S->ci[0].linf_rel_d, S->ci[0].rel_gap,
+                        S->ci[0].pobj, S->ci[0].dobj, (long)S->restart_used);
+            if (ex) { *eta_io = eta; *eta_prime_out = eta_prime; return 1; }
+        }
+        if (eta <= eta_bar) { *eta_io = eta; *eta_prime_out = eta_prime; return 0; }
+        eta = eta_prime;
+        if (++count > 1000000) {
+            S->proj_abs_tol = dmax(S->proj_abs_tol * 0.1, 1e-22);
+            S->proj_rel_tol = dmax(S->proj_rel_tol * 0.1, 1e-22);
+            *force = 1;
+            *eta_io = eta; *eta_prime_out = eta_prime;
+            return 0;
+        }
+    }
+}
+
+/* reflection_update, src/pdcs_gpu/cuda/utils.cu:38-57 */
+static void reflection(solver *S, double extra, int64_t inner, double eta_cum, double eta) {
+    double di = (double)inner, a = (di + 1.0) / (di + 2.0), b = 1.0 / (di + 2.0);
+    for (int64_t j = 0; j < S->n; ++j) {
+        S->x[j] = a * ((1.0 + extra) * S->x[j] - extra * S->xl[j]) + b * S->x[j];
+        S->xm[j] = (S->xm[j] * eta_cum + S->x[j] * eta) / (eta_cum + eta);
+    }
+    for (int64_t i = 0; i < S->m; ++i) {
+        S->y[i] = a * ((1.0 + extra) * S->y[i] - extra * S->yl[i]) + b * S->y[i];
+        S->ym[i] = (S->ym[i] * eta_cum + S->y[i] * eta) / (eta_cum + eta);
+    }
+}
+
+static int check_cones(const oracle_problem *P) {
+    int64_t s = P->mzero + P->mnonneg + 3 * (P->nexp + P->ndexp);
+    for (int64_t b = 0; b < P->nsoc; ++b) {
+        if (P->soc[b] < 2) return -2;
+        s += P->soc[b];
+    }
+    if (s != P->m) return -1;
+    if (P->nb > P->n) return -3;
+    return 0;
+}
+
+static int solve_impl(const oracle_problem *P, const oracle_params *prm, const double *xw,
+                      const double *yw, double *x_out, double *y_out, oracle_result *res,
+                      int64_t attempt_limit) {
+    int rc = check_cones(P);
+    if (rc) return rc;
+    solver S_, *S = &S_;
+    memset(S, 0, sizeof(*S));
+    S->attempt_limit = attempt_limit;
+    S->P = P;
+    S->prm = *prm;
+    S->n = P->n; S->m = P->m; S->nb = P->nb;
+    int64_t n = S->n, m = S->m;
+    S->t_start = now_s();
+    csr_pair_build(&S->Graw, m, n, P->rowptr, P->colidx, P->vals);
+    kkt_ws_init(&S->W, P, &S->Graw);
+    S->c = dalloc(n); S->h = dalloc(m); S->bl = dalloc(S->nb); S->bu = dalloc(S->nb);
+    S->Drp = dalloc(n); S->Dlp = dalloc(m); S->dsc = dalloc(m); S->dsc2 = dalloc(m);
+    S->x = dalloc(n); S->xl = dalloc(n); S->xm = dalloc(n);
+    S->y = dalloc(m); S->yl = dalloc(m); S->ym = dalloc(m);
+    S->x0 = dalloc(n); S->y0 = dalloc(m); S->x0l = dalloc(n); S->y0l = dalloc(m);
+    S->tn1 = dalloc(n); S->tn2 = dalloc(n); S->tn3 = dalloc(n);
+    S->tm1 = dalloc(m); S->tm2 = dalloc(m); S->tm3 = dalloc(m); S->tm4 = dalloc(m);
+    S->h1 = dalloc(n); S->h2 = dalloc(m);
+    S->xr = dalloc(n); S->yr = dalloc(m); S->xmr = dalloc(n); S->ymr = dalloc(m);
+    S->warm_dual = dalloc(P->nsoc);
+    for (int64_t b = 0; b < P->nsoc; ++b) S->warm_dual[b] = 1.0; /* t_warm_start .= 1.0, def_struct.jl:56 */
+    /* scaling: rescale_problem! + scale_preconditioner!, rpdhg_alg_gpu_gen.jl:937-1004 */
+    double *vals_s = dalloc(P->rowptr[m]);
+    if (prm->use_preconditioner)
+        oracle_scale(P, prm->ruiz_iters, prm->pock_chambolle, S->Drp, S->Dlp, vals_s);
+    else {
+        memcpy(vals_s, P->vals, sizeof(double) * (size_t)P->rowptr[m]);
+        for (int64_t j = 0; j < n; ++j) S->Drp[j] = 1.0;
+        for (int64_t i = 0; i < m; ++i) S->Dlp[i] = 1.0;
+    }
+    csr_pair_build(&S->G, m, n, P->rowptr, P->colidx, vals_s);
+    free(vals_s);
+    for (int64_t j = 0; j < n; ++j) S->c[j] = P->c[j] / S->Drp[j];
+    for (int64_t i = 0; i < m; ++i) S->h[i] = P->h[i] / S->Dlp[i];
+    for (int64_t j = 0; j < S->nb; ++j) { S->bl[j] = P->bl[j] * S->Drp[j]; S->bu[j] = P->bu[j] * S->Drp[j]; }
+    /* scale_preconditioner!, preprocess.jl:173-297 (dual side) */
+    for (int64_t i = 0; i < m; ++i) S->dsc[i] = 1.0;
+    {
+        int64_t off = P->mzero + P->mnonneg;
+        for (int64_t b = 0; b < P->nsoc; ++b) {
+            for (int64_t k = 0; k < P->soc[b]; ++k) S->dsc[off + k] = S->Dlp[off] / S->Dlp[off + k];
+            off += P->soc[b];
+        }
+        for (int64_t k = off; k < m; ++k) S->dsc[k] = 1.0 / S->Dlp[k];
+    }
+    for (int64_t i = 0; i < m; ++i) S->dsc2[i] = S->dsc[i] * S->dsc[i];
+    if (prm->warm_start && xw && yw) {
+        for (int64_t j = 0; j < n; ++j) S->x[j] = xw[j] * S->Drp[j];
+        for (int64_t i = 0; i < m; ++i) S->y[i] = yw[i] * S->Dlp[i];
+    }
+    memcpy(S->xl, S->x, sizeof(double) * (size_t)n); memcpy(S->xm, S->x, sizeof(double) * (size_t)n);
+    memcpy(S->yl, S->y, sizeof(double) * (size_t)m); memcpy(S->ym, S->y, sizeof(double) * (size_t)m);
+    S->tau = S->sigma = 0.9;             /* rpdhg_alg_gpu_gen.jl:1018-1019 */
+    S->proj_abs_tol = S->proj_rel_tol = 1e-11; /* def_struct.jl:832-833 */
+    S->bs_t0 = 1.0;
+    S->omega = 1.0;
+    S->max_kkt = 1e30;
+    S->min_kkt = 1e30;
+    S->restart_dg_flag = 1;
+    S->exit_code = 7;
+    recover(S);
+    converge_info(&S->W, S->xr, S->yr, &S->ci[0]);
+    /* pdhg_main_iter_..._aggressive!, rpdhg_alg_gpu_gen_scaling.jl:770-896 */
+    memcpy(S->x0, S->x, sizeof(double) * (size_t)n); memcpy(S->y0, S->y, sizeof(double) * (size_t)m);
+    memcpy(S->x0l, S->x, sizeof(double) * (size_t)n); memcpy(S->y0l, S->y, sizeof(double) * (size_t)m);
+    S->omega = init_primal_weight(S);
+    double ginf = nrminf(S->G.vals, S->G.nnz);
+    double eta_prime = 1.0 / ginf, eta = 0.0;
+    double *xdiff = dalloc(n), *ydiff = dalloc(m);
+    int done = 0;
+    S->t_loop = now_s();
+    for (int64_t outer = 1; outer <= prm->max_outer_iter && !done; ++outer) {
+        S->max_kkt = max_l2_kkt(&S->ci[0]);
+        S->min_kkt = dmin(dmin(dmin(S->ci[0].l2_rel_p, S->ci[0].l2_rel_d), S->ci[0].rel_gap), S->min_kkt);
+        double extra = -0.1 * log10(S->max_kkt) + 0.2;
+        memcpy(S->x0l, S->x0, sizeof(double) * (size_t)n); memcpy(S->y0l, S->y0, sizeof(double) * (size_t)m);
+        memcpy(S->x0, S->xm, sizeof(double) * (size_t)n); memcpy(S->y0, S->ym, sizeof(double) * (size_t)m);
+        memcpy(S->x, S->xm, sizeof(double) * (size_t)n); memcpy(S->y, S->ym, sizeof(double) * (size_t)m);
+        if (prm->use_kkt_restart) restart_threshold(S, 0);
+        else restart_threshold(S, S->restart_dg_flag);
+        int f1 = outer > 10 && S->max_kkt > 1e5;
+        double ratio = S->ci[0].l2_rel_p / S->ci[0].l2_rel_d;
+        int f2 = outer > 80 && (ratio > 1e10 || ratio < 1e-10);
+        int f3 = outer > 10 && S->tau < 1e-6 && S->sigma < 1e-6;
+        int f4 = S->iter > 1000000 && S->ci[0].rel_gap > 0.98 && S->ci[0].l2_rel_p > 1.0 && S->ci[0].l2_rel_d > 1.0;
+        if (f1 || f2 || f3 || f4) res->pessimistic_flag = 1;
+        int64_t inner = 0;
+        memcpy(xdiff, S->x, sizeof(double) * (size_t)n); memcpy(ydiff, S->y, sizeof(double) * (size_t)m);
+        double eta_cum = 0.0;
+        for (;;) {
+            inner++;
+            int force = 0;
+            eta = eta_prime;
+            if (adaptive_step(S, &eta, &eta_prime, &force)) { done = 1; break; }
+            eta = dmax(eta, 1e-30);
+            reflection(S, extra, inner, eta_cum, eta);
+            eta_cum += eta;
+            if ((inner >= S->iter * 0.223 && S->iter > 10000) || (force && inner > 100)) {
+                S->restart_used++;
+                break;
+            }
+            if (prm->use_kkt_restart && S->iter % prm->kkt_restart_freq == 0) {
+                if (restart_check(S, inner, 0)) break;
+                if (outer == 1) break;
+            }
+            if (prm->use_duality_gap_restart && S->iter % prm->duality_gap_restart_freq == 0) {
+                if (restart_check(S, inner, S->restart_dg_flag)) break;
+                if (outer == 1) break;
+            }
+        }
+        if (done) break;
+        for (int64_t j = 0; j < n; ++j) xdiff[j] -= S->x[j];
+        for (int64_t i = 0; i < m; ++i) ydiff[i] -= S->y[i];
+        S->omega = primal_weight_update(S, xdiff, ydiff, S->omega);
+    }
+    if (!done) S->exit_code = 1;
+    /* outputs */
+    const conv_info *c = &S->ci[S->exit_seq];
+    res->exit_code = S->exit_code;
+    res->sequence = S->exit_seq;
+    res->iter = S->iter;
+    res->restarts = S->restart_used;
+    res->pobj = c->pobj; res->dobj = c->dobj;
+    res->rel_primal_res = c->linf_rel_p; res->rel_dual_res = c->linf_rel_d; res->rel_gap = c->rel_gap;
+    res->abs_primal_res = c->linf_abs_p; res->abs_dual_res = c->linf_abs_d; res->abs_gap = c->abs_gap;
+    res->l2_rel_primal_res = c->l2_rel_p; res->l2_rel_dual_res = c->l2_rel_d;
+    res->rel_primal_res_code = c->linf_rel_p_code;
+    res->omega = S->omega;
+    res->time_solve = now_s() - (attempt_limit > 0 ? S->t_loop : S->t_start);
+    if (x_out) memcpy(x_out, S->exit_seq ? S->xmr : S->xr, sizeof(double) * (size_t)n);
+    if (y_out) memcpy(y_out, S->exit_seq ? S->ymr : S->yr, sizeof(double) * (size_t)m);
+    free(xdiff); free(ydiff);
+    double *bufs[] = {S->c, S->h, S->bl, S->bu, S->Drp, S->Dlp, S->dsc, S->dsc2, S->x, S->xl, S->xm,
+                      S->y, S->yl, S->ym, S->x0, S->y0, S->x0l, S->y0l, S->tn1, S->tn2, S->tn3,
+                      S->tm1, S->tm2, S->tm3, S->tm4, S->h1, S->h2, S->xr, S->yr, S->xmr, S->ymr,
+                      S->warm_dual};
+    for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i) free(bufs[i]);
+    kkt_ws_free(&S->W);
+    csr_pair_free(&S->G);
+    csr_pair_free(&S->Graw);
+    return 0;
+}
+
+int oracle_solve(const oracle_problem *P, const oracle_params *prm, const double *xw,
+                 const double *yw, double *x_out, double *y_out, oracle_result *res) {
+    memset(res, 0, sizeof(*res));
+    return solve_impl(P, prm, xw, yw, x_out, y_out, res, 0);
+}
+
+/* CPU baseline sample: the first `attempts` PDHG attempts of the same algorithm,
+ * restarts and KKT checks included; scaling and setup excluded from the time. */
+double oracle_time_attempts(const oracle_problem *P, int64_t attempts, int64_t *done) {
+    oracle_params prm;
+    oracle_default_params(&prm);
+    oracle_result res;
+    memset(&res, 0, sizeof(res));
+    solve_impl(P, &prm, NULL, NULL, NULL, NULL, &res, attempts);
+    if (done) *done = res.iter;
+    return res.time_solve;
+}

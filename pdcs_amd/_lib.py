@@ -1,0 +1,106 @@
+"""ctypes binding of libpdcs_hip.so (include/pdcs.h).
+
+The shared library is built in-tree by ``pdcs_amd/build.py`` (hipcc, gfx950).
+Loading fails loudly when it is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpdcs_hip.so")
+
+PDCS_OK = 0
+PDCS_FP64 = 0
+PDCS_MIXED = 1
+
+c_i64 = C.c_int64
+c_i32 = C.c_int32
+c_dp = C.POINTER(C.c_double)
+
+
+class Cones(C.Structure):
+    _fields_ = [("mzero", c_i64), ("mnonneg", c_i64), ("nsoc", c_i64), ("soc", C.POINTER(c_i64)),
+                ("nrsoc", c_i64), ("rsoc", C.POINTER(c_i64)), ("nexp", c_i64), ("ndualexp", c_i64)]
+
+
+class Params(C.Structure):
+    _fields_ = [("max_outer_iter", c_i64), ("max_inner_iter", c_i64), ("abs_tol", C.c_double),
+                ("rel_tol", C.c_double), ("time_limit", C.c_double), ("check_terminate_freq", c_i64),
+                ("duality_gap_restart_freq", c_i64), ("kkt_restart_freq", c_i64),
+                ("use_kkt_restart", c_i32), ("use_duality_gap_restart", c_i32),
+                ("use_preconditioner", c_i32), ("verbose", c_i32), ("warm_start", c_i32),
+                ("precision", c_i32), ("polish", c_i32), ("bisect_intent", c_i32),
+                ("code_primal_denominator", c_i32), ("use_graph", c_i32), ("mixed_check_freq", c_i64),
+                ("polish_check_freq", c_i64), ("mixed_switch_tol", C.c_double)]
+
+
+class Result(C.Structure):
+    _fields_ = [("exit_code", c_i32), ("sequence", c_i32), ("iter", c_i64), ("iter_fp32", c_i64),
+                ("restarts", c_i64), ("pobj", C.c_double), ("dobj", C.c_double),
+                ("rel_primal_res", C.c_double), ("rel_dual_res", C.c_double), ("rel_gap", C.c_double),
+                ("abs_primal_res", C.c_double), ("abs_dual_res", C.c_double), ("abs_gap", C.c_double),
+                ("l2_rel_primal_res", C.c_double), ("l2_rel_dual_res", C.c_double),
+                ("rel_primal_res_code", C.c_double), ("omega", C.c_double),
+                ("time_setup", C.c_double), ("time_solve", C.c_double), ("time_fp32", C.c_double),
+                ("time_kkt", C.c_double), ("time_restart", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# every symbol declared in include/pdcs.h
+EXPORTS = ["pdcs_params_default", "pdcs_ctx_create", "pdcs_ctx_destroy", "pdcs_last_error",
+           "pdcs_problem_create_host", "pdcs_problem_create_device", "pdcs_problem_destroy",
+           "pdcs_solve", "pdcs_get_solution", "pdcs_get_scaling", "pdcs_evaluate", "pdcs_spmv",
+           "pdcs_project_dual", "pdcs_project_primal_cone"]
+
+_lib = None
+
+
+class PdcsError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libpdcs_hip.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PdcsError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.pdcs_params_default.argtypes = [C.POINTER(Params)]
+    L.pdcs_params_default.restype = None
+    L.pdcs_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.pdcs_ctx_destroy.argtypes = [vp]
+    L.pdcs_last_error.argtypes = [vp]
+    L.pdcs_last_error.restype = C.c_char_p
+    L.pdcs_problem_create_host.argtypes = [vp, c_i64, c_i64, c_i64, c_dp, C.POINTER(c_i32), C.POINTER(c_i32),
+                                           c_dp, c_dp, c_dp, c_dp, C.POINTER(Cones), C.POINTER(vp)]
+    L.pdcs_problem_create_device.argtypes = [vp, c_i64, c_i64, c_i64, vp, vp, vp, vp, vp, vp, vp,
+                                             C.POINTER(Cones), C.POINTER(vp)]
+    L.pdcs_problem_destroy.argtypes = [vp]
+    L.pdcs_solve.argtypes = [vp, C.POINTER(Params), c_dp, c_dp, C.POINTER(Result)]
+    L.pdcs_get_solution.argtypes = [vp, c_dp, c_dp, c_dp]
+    L.pdcs_get_scaling.argtypes = [vp, c_dp, c_dp]
+    L.pdcs_evaluate.argtypes = [vp, c_dp, c_dp, C.POINTER(Result)]
+    L.pdcs_spmv.argtypes = [vp, c_i32, c_i64, c_i64, vp, vp, vp, vp, vp]
+    L.pdcs_project_dual.argtypes = [vp, c_i32, C.POINTER(Cones), c_i64, vp, vp]
+    L.pdcs_project_primal_cone.argtypes = [vp, C.POINTER(Cones), c_i64, vp]
+    _lib = L
+    return L
+
+
+def check(rc: int, ctx=None):
+    if rc != PDCS_OK:
+        msg = load().pdcs_last_error(ctx)
+        raise PdcsError(f"pdcs error {rc}: {msg.decode() if msg else ''}")
+
+
+def default_params() -> Params:
+    p = Params()
+    load().pdcs_params_default(C.byref(p))
+    return p

@@ -1,0 +1,152 @@
+// internal.h -- device-side data layout and kernel launchers of libpdcs_hip.so.
+// Host code (pdcs_api.cpp) talks to the kernels (kernels.hip) only through the
+// launchers declared here.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pdcs {
+
+constexpr int kBlock = 256;          // threads per workgroup (4 wavefronts)
+constexpr int64_t kKktStrideHost = 1 << 18;  // partial slots per residual quantity
+constexpr int kKktQuantities = 13;
+
+// CSR matrix on the device. `val` is float or double depending on the path.
+template <typename T>
+struct Csr {
+    const int32_t *rowptr;
+    const int32_t *col;
+    const T *val;
+    int64_t rows;
+    int32_t lanes;   // lanes per row for the wave-split SpMV (power of two, 1..64)
+};
+
+// Cone layout of the dual rows on the device.
+struct ConeLayout {
+    int64_t m;
+    int64_t mzero, mnonneg;
+    int64_t cone_begin;          // first row of the SOC/exp section
+    int32_t nsoc_small;          // SOCs handled by lane groups (dim <= 64 * kSocRegs)
+    const int32_t *soc_small_start, *soc_small_dim;
+    int32_t soc_small_group;     // lanes per cone for the small-SOC kernel
+    int32_t nsoc_big;            // SOCs handled by one workgroup each
+    const int32_t *soc_big_start, *soc_big_dim;
+    int32_t nexp;                // exponential cones (code 12: y in Kexp*)
+    int32_t ndexp;               // dual exponential cones (code 15: y in Kexp)
+    int64_t exp_begin;           // first row of the exponential section
+};
+
+// Device-resident control state of the adaptive-step PDHG iteration
+// (the scalars of pdhg_one_iter_diagonal_rescaling_adaptive_step_size!,
+// rpdhg_alg_gpu_gen_scaling.jl:656-767).  Lives in device memory; the host
+// reads it only at halts.
+struct StepState {
+    // inputs set by the host
+    double omega;
+    double dec_next;        // decay_step_size_ratio for the next series
+    double extra;           // extra_coeff of the epoch
+    int64_t check_freq;
+    int64_t restart_freq;   // 0 = no duality-gap restart checks
+    int64_t kkt_restart_freq; // 0 = none
+    int64_t stop_at;        // first attempt slot that must not run
+    // evolving state
+    double eta;             // eta of the next attempt
+    double dec;             // decay ratio of the running series
+    double eta_cum;
+    int64_t iter;           // attempts so far
+    int64_t iter_stepsize;
+    int64_t inner;          // accepted series in this epoch
+    int64_t count;          // rejections in the running series
+    // last attempt
+    double eta_try, eta_bar, eta_prime;
+    double dx2, dy2, dot;
+    double tau, sigma;
+    int32_t accepted, force;
+    int32_t reason;         // halt reason: 1 exit check, 2 restart check, 4 artificial, 8 force
+    int32_t pad;
+    // reflection order for k_reflect
+    int64_t reflect_slot;
+    double reflect_k, reflect_eta, reflect_cum;
+};
+
+// Pointers of one iterate set (float or double).
+template <typename T>
+struct Iter {
+    T *x, *xs, *xm, *xbar;   // candidate, series start, average, extrapolation
+    T *y, *ys, *ym, *dy;     // candidate, series start, average, y - ys
+};
+
+template <typename T>
+struct ProbVec {             // scaled problem data in precision T
+    const T *c, *h, *bl, *bu;
+    int64_t n, m, nb;
+};
+
+// ----- launchers (kernels.hip) -----
+template <typename T>
+void launch_attempt(hipStream_t s, int slot, const Csr<T> &G, const Csr<T> &Gt,
+                    const ProbVec<T> &P, const ConeLayout &K, const double *d2,
+                    Iter<T> it, StepState *st, double *part, int64_t part_stride);
+
+// continuation of an attempt whose decision was deferred to the host (exit check)
+void launch_continue(hipStream_t s, int slot, StepState *st);
+template <typename T>
+void launch_reflect(hipStream_t s, int slot, Iter<T> it, int64_t n, int64_t m, StepState *st);
+
+template <typename T>
+void launch_spmv(hipStream_t s, const Csr<T> &A, const T *x, T *y);
+template <typename T>
+void launch_spmv_acc(hipStream_t s, const Csr<T> &A, const T *x, double *y); // double output
+
+// out = clamp(a + alpha*b) on the box part (alpha in device memory if dalpha)
+template <typename T>
+void launch_axpy_box(hipStream_t s, T *out, const T *a, double alpha, const T *b, const T *bl,
+                     const T *bu, int64_t n, int64_t nb);
+template <typename T>
+void launch_axpy(hipStream_t s, T *out, const T *a, double alpha, const T *b, int64_t n);
+// in-place projection onto the dual cone (scaled): nonneg + SOC + exp
+template <typename T>
+void launch_project_dual(hipStream_t s, const ConeLayout &K, const double *d, const double *d2,
+                         T *y);
+// in-place projection onto K (unscaled, codes 2/3/5/13/16), FP64
+void launch_project_primal_cone(hipStream_t s, const ConeLayout &K, double *r);
+
+// deterministic multi-dot: out[k] = sum_i a_k[i] * b_k[i], k < nd (<= 4); out is device
+template <typename T>
+void launch_dots(hipStream_t s, int nd, const T *const *a, const T *const *b, int64_t n,
+                 double *part, double *out);
+
+// z = a - b
+template <typename T>
+void launch_sub(hipStream_t s, T *z, const T *a, const T *b, int64_t n);
+template <typename T>
+void launch_copy(hipStream_t s, T *dst, const T *src, int64_t n);
+template <typename Tdst, typename Tsrc>
+void launch_convert(hipStream_t s, Tdst *dst, const Tsrc *src, int64_t n);
+
+// KKT residuals of one sequence (x, y in scaled space, precision T) against the
+// original data (FP64 scaled matrix + scaling vectors).  Writes KktOut to `out`.
+struct KktData {
+    Csr<double> G, Gt;
+    const double *c_raw, *h_raw, *blf, *buf, *laml, *lamu, *Dr, *Dl;
+    int64_t n, m, nb;
+    ConeLayout K;
+    double *rtmp;   // m scratch
+};
+struct KktOut {
+    double pobj, dobj, atyInf, atyN1, sInf, sN1, d2, dInf, axInf, axN1, p2, pInf, projInf;
+};
+template <typename T>
+void launch_kkt(hipStream_t s, const KktData &D, const T *x, const T *y, double *part,
+                KktOut *out);
+
+// scaling (Ruiz l_inf / Pock-Chambolle alpha=1) on the FP64 CSR values
+void launch_row_colmax(hipStream_t s, const Csr<double> &G, const Csr<double> &Gt, double *rowv,
+                       double *colv, int mode /*0 max, 1 sum*/);
+void launch_scale_vals(hipStream_t s, double *vals, const int32_t *rowptr, const int32_t *col,
+                       int64_t m, const double *Dl, const double *Dr);
+void launch_sqrt_fix(hipStream_t s, double *v, int64_t n); // v = sqrt(v); 0 -> 1
+void launch_mul(hipStream_t s, double *a, const double *b, int64_t n);
+void launch_gather(hipStream_t s, double *dst, const double *src, const int32_t *map, int64_t n);
+
+}  // namespace pdcs

@@ -1,0 +1,964 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of the PDCS PDHG hot path.
+//
+// One PDHG attempt (pdhg_one_iter_diagonal_rescaling_adaptive_step_size!,
+// src/pdcs_gpu/rpdhg_alg_gpu_gen_scaling.jl:656-767) is this kernel chain,
+// with every scalar kept in device memory (StepState) so attempts run back to
+// back without host synchronisation:
+//   k_primal   x = clamp(xs + tau*dec*(G'ys - c)); xbar = 2x - xs; sum (x-xs)^2
+//   k_dual     y = ys - sigma*dec*(G xbar - h); elementwise cones; sum dy^2
+//   k_soc_*    scaled-SOC projections of the cone rows; dy; sum dy^2
+//   k_atdy     sum (x - xs) . (G' dy)
+//   k_step     eta_bar, eta', accept/reject, halt decisions (one workgroup)
+//   k_reflect  aggressive reflection + eta-weighted average (reflection_update,
+//              src/pdcs_gpu/cuda/utils.cu:38-57)
+// Every kernel of attempt slot `slot` returns immediately once an earlier
+// k_step has set stop_at <= slot, so a pre-recorded chunk of attempts drains
+// to no-ops after a halt.
+//
+// SpMV: CSR with a power-of-two group of lanes per row (wave64 shuffles); the
+// value/vector type T is double (FP64 path) or float (mixed path), products
+// are accumulated in double.  Reductions are two-stage and in fixed order, so
+// results are run-to-run deterministic.
+#include "internal.h"
+
+#include <math.h>
+
+namespace pdcs {
+
+// ------------------------------------------------------------ reductions --
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+    return v;
+}
+// block-wide sum over kBlock threads; result valid in thread 0
+__device__ __forceinline__ double block_sum(double v, double *lds) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += lds[i];
+    __syncthreads();
+    return r;
+}
+__device__ __forceinline__ double block_max(double v, double *lds) {
+    v = wave_max(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r = fmax(r, lds[i]);
+    __syncthreads();
+    return r;
+}
+// block-wide sum broadcast to all threads
+__device__ __forceinline__ double block_sum_all(double v, double *lds) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += lds[i];
+    __syncthreads();
+    return r;
+}
+
+template <typename T, int L>
+__device__ __forceinline__ double row_dot(const Csr<T> &A, int64_t row, int lane, const T *x) {
+    double acc = 0.0;
+    if (row < A.rows) {
+        const int32_t s = A.rowptr[row], e = A.rowptr[row + 1];
+        for (int32_t k = s + lane; k < e; k += L) acc += (double)A.val[k] * (double)x[A.col[k]];
+    }
+    if (L > 1) acc = group_sum<L>(acc);
+    return acc;
+}
+
+__device__ __forceinline__ bool halted(const StepState *st, int slot) {
+    return slot >= __builtin_nontemporal_load(&st->stop_at);
+}
+
+// ------------------------------------------------------------- attempt --
+template <typename T, int L>
+__global__ __launch_bounds__(kBlock) void k_primal(Csr<T> Gt, ProbVec<T> P, const T *__restrict__ ys,
+                                                   const T *__restrict__ xs, T *__restrict__ x,
+                                                   T *__restrict__ xbar, StepState *st, int slot,
+                                                   double *__restrict__ part) {
+    __shared__ double lds[kBlock / 64];
+    if (halted(st, slot)) return;
+    const double ps = st->eta / st->omega * st->dec;   // primal_step_size, scaling.jl:687-689
+    const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t row = gid / L;
+    const int lane = (int)(gid % L);
+    const double acc = row_dot<T, L>(Gt, row, lane, ys);
+    double d2 = 0.0;
+    if (row < P.n && lane == 0) {
+        const double xl = (double)xs[row];
+        double v = xl + ps * (acc - (double)P.c[row]);            // primal_update, utils.cu:8
+        if (row < P.nb) v = fmin(fmax(v, (double)P.bl[row]), (double)P.bu[row]);  // code 18
+        const T xv = (T)v;
+        x[row] = xv;
+        xbar[row] = (T)(2.0 * (double)xv - xl);                   // extrapolation_update, utils.cu:18
+        const double d = (double)xv - xl;
+        d2 = d * d;
+    }
+    d2 = block_sum(d2, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = d2;
+}
+
+template <typename T, int L>
+__global__ __launch_bounds__(kBlock) void k_dual(Csr<T> G, ProbVec<T> P, ConeLayout K,
+                                                 const T *__restrict__ xbar, const T *__restrict__ ys,
+                                                 T *__restrict__ y, T *__restrict__ dy, StepState *st,
+                                                 int slot, double *__restrict__ part) {
+    __shared__ double lds[kBlock / 64];
+    if (halted(st, slot)) return;
+    const double ds = st->eta * st->omega * st->dec;  // dual_step_size, scaling.jl:698-700
+    const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t row = gid / L;
+    const int lane = (int)(gid % L);
+    const double acc = row_dot<T, L>(G, row, lane, xbar);
+    double d2 = 0.0;
+    if (row < P.m && lane == 0) {
+        const double yl = (double)ys[row];
+        double v = yl - ds * (acc - (double)P.h[row]);             // dual_update, utils.cu:28
+        if (row < K.cone_begin) {
+            if (row >= K.mzero) v = fmax(v, 0.0);                   // code 4 (code 1: free)
+            const T yv = (T)v;
+            y[row] = yv;
+            const double d = (double)yv - yl;
+            dy[row] = (T)d;
+            d2 = d * d;
+        } else {
+            y[row] = (T)v;                                         // projected by the cone kernels
+        }
+    }
+    d2 = block_sum(d2, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = d2;
+}
+
+// Scaled-SOC projection onto {(t, z) : ||d .* z|| <= t} of (s, x).
+//
+// With multiplier lambda >= 0 the KKT conditions give z = x / (1 + lambda d^2)
+// and t = s / (1 - lambda).  Writing lambda = 1 - s/t turns the boundary
+// condition ||d z|| = t into phi(t) = S(t)^(-1/2) - 1 = 0 with
+// S(t) = sum_i a_i / w_i(t)^2, a_i = d_i^2 x_i^2, w_i = t (1 + d_i^2) - s d_i^2.
+// S^(-1/2) is a power mean of exponent -2 of the affine w_i(t), hence concave
+// and increasing in t: Newton started left of the root (t = max(s, 0), where
+// phi < 0 whenever the point is neither interior nor polar) increases
+// monotonically to the root with no overshoot.  A bracket [tL, tR] with
+// tR = ||(s, x)|| (non-expansiveness) guards round-off.  This replaces the
+// reference's bisection on xi = lambda/2 (soc_proj_diagonal!,
+// src/pdcs_cpu/soc_rsoc_proj.jl:257-373) with the same fixed point.
+struct SocSums {
+    double nd2, ni2, nx2;   // ||d x||^2, ||x/d||^2, ||x||^2
+};
+
+// per-element helpers shared by the group and block variants
+__device__ __forceinline__ void soc_newton_terms(double t, double s, double x, double dd2,
+                                                 double &S, double &S1) {
+    const double a = dd2 * x * x;
+    const double w = t * (1.0 + dd2) - s * dd2;
+    const double iw = 1.0 / w;
+    const double q = a * iw * iw;
+    S += q;
+    S1 += q * (1.0 + dd2) * iw;
+}
+
+// Decide the case from the sums; returns 0 interior, 1 polar (zero), 2 s==0
+// closed form, 3 Newton.
+__device__ __forceinline__ int soc_case(double s, const SocSums &z) {
+    if (sqrt(z.ni2) <= -s) return 1;
+    if (sqrt(z.nd2) <= s) return 0;
+    if (s == 0.0) return 2;
+    return 3;
+}
+
+// one Newton update given S, S1 at t; returns new t and convergence flag
+__device__ __forceinline__ double soc_newton_step(double t, double S, double S1, double &tL,
+                                                  double &tR, bool &done) {
+    const double isq = 1.0 / sqrt(S);
+    const double phi = isq - 1.0;
+    if (phi < 0.0) tL = fmax(tL, t); else tR = fmin(tR, t);
+    const double dphi = S1 * isq * isq * isq;           // phi'(t) = S1 / S^(3/2)
+    double tn = t - phi / dphi;
+    if (!(tn > tL && tn < tR)) tn = 0.5 * (tL + tR);
+    done = fabs(tn - t) <= 4e-16 * fabs(tn) || phi == 0.0 || tR - tL <= 4e-16 * tR;
+    return tn;
+}
+
+template <typename T, int G>
+__global__ __launch_bounds__(kBlock) void k_soc_group(ConeLayout K, const double *__restrict__ d2,
+                                                      T *__restrict__ y, const T *__restrict__ ys,
+                                                      T *__restrict__ dy, StepState *st, int slot,
+                                                      double *__restrict__ part) {
+    __shared__ double lds[kBlock / 64];
+    if (st != nullptr && halted(st, slot)) return;
+    const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t cone = gid / G;
+    const int lane = (int)(gid % G);
+    double my2 = 0.0;
+    if (cone < K.nsoc_small) {
+        const int64_t st0 = K.soc_small_start[cone];
+        const int k = K.soc_small_dim[cone] - 1;
+        const T *yt = y + st0 + 1;
+        const double *dt = d2 + st0 + 1;
+        const double s = (double)y[st0];
+        SocSums z = {0, 0, 0};
+        for (int e = lane; e < k; e += G) {
+            const double xv = (double)yt[e], dd = dt[e];
+            z.nd2 += dd * xv * xv;
+            z.ni2 += xv * xv / dd;
+            z.nx2 += xv * xv;
+        }
+        z.nd2 = group_sum<G>(z.nd2);
+        z.ni2 = group_sum<G>(z.ni2);
+        z.nx2 = group_sum<G>(z.nx2);
+        const int cs = soc_case(s, z);
+        double t = s, lam = 0.0;
+        if (cs == 2) {
+            double S = 0.0;
+            for (int e = lane; e < k; e += G) {
+                const double xv = (double)yt[e], dd = dt[e];
+                const double q = xv / (1.0 + dd);
+                S += dd * q * q;
+            }
+            t = sqrt(group_sum<G>(S));
+            lam = 1.0;
+        } else if (cs == 3) {
+            double tL = fmax(s, 0.0), tR = sqrt(s * s + z.nx2);
+            t = tL;
+            bool done = false;
+            for (int it = 0; it < 100 && !done; ++it) {
+                double S = 0.0, S1 = 0.0;
+                for (int e = lane; e < k; e += G) soc_newton_terms(t, s, (double)yt[e], dt[e], S, S1);
+                S = group_sum<G>(S);
+                S1 = group_sum<G>(S1);
+                t = soc_newton_step(t, S, S1, tL, tR, done);
+            }
+            lam = 1.0 - s / t;
+        }
+        // write back (cs 0: unchanged)
+        T *yh = y + st0;
+        if (cs == 1) {
+            for (int e = lane; e <= k; e += G) yh[e] = (T)0.0;
+        } else if (cs >= 2) {
+            if (lane == 0) yh[0] = (T)t;
+            for (int e = lane; e < k; e += G) yh[1 + e] = (T)((double)yh[1 + e] / (1.0 + lam * dt[e]));
+        }
+        if (dy != nullptr) {
+            for (int e = lane; e <= k; e += G) {
+                const double d = (double)yh[e] - (double)ys[st0 + e];
+                dy[st0 + e] = (T)d;
+                my2 += d * d;
+            }
+        }
+    }
+    if (part != nullptr) {
+        my2 = block_sum(my2, lds);
+        if (threadIdx.x == 0) part[blockIdx.x] = my2;
+    }
+}
+
+// one workgroup per cone (large cones)
+template <typename T>
+__global__ __launch_bounds__(1024) void k_soc_block(ConeLayout K, const double *__restrict__ d2,
+                                                    T *__restrict__ y, const T *__restrict__ ys,
+                                                    T *__restrict__ dy, StepState *st, int slot,
+                                                    double *__restrict__ part) {
+    __shared__ double lds[1024 / 64];
+    if (st != nullptr && halted(st, slot)) return;
+    const int64_t cone = blockIdx.x;
+    const int64_t st0 = K.soc_big_start[cone];
+    const int k = K.soc_big_dim[cone] - 1;
+    const T *yt = y + st0 + 1;
+    const double *dt = d2 + st0 + 1;
+    const double s = (double)y[st0];
+    SocSums z = {0, 0, 0};
+    for (int e = threadIdx.x; e < k; e += blockDim.x) {
+        const double xv = (double)yt[e], dd = dt[e];
+        z.nd2 += dd * xv * xv;
+        z.ni2 += xv * xv / dd;
+        z.nx2 += xv * xv;
+    }
+    z.nd2 = block_sum_all(z.nd2, lds);
+    z.ni2 = block_sum_all(z.ni2, lds);
+    z.nx2 = block_sum_all(z.nx2, lds);
+    const int cs = soc_case(s, z);
+    double t = s, lam = 0.0;
+    if (cs == 2) {
+        double S = 0.0;
+        for (int e = threadIdx.x; e < k; e += blockDim.x) {
+            const double xv = (double)yt[e], dd = dt[e];
+            const double q = xv / (1.0 + dd);
+            S += dd * q * q;
+        }
+        t = sqrt(block_sum_all(S, lds));
+        lam = 1.0;
+    } else if (cs == 3) {
+        double tL = fmax(s, 0.0), tR = sqrt(s * s + z.nx2);
+        t = tL;
+        bool done = false;
+        for (int it = 0; it < 100 && !done; ++it) {
+            double S = 0.0, S1 = 0.0;
+            for (int e = threadIdx.x; e < k; e += blockDim.x) soc_newton_terms(t, s, (double)yt[e], dt[e], S, S1);
+            S = block_sum_all(S, lds);
+            S1 = block_sum_all(S1, lds);
+            t = soc_newton_step(t, S, S1, tL, tR, done);
+        }
+        lam = 1.0 - s / t;
+    }
+    __syncthreads();
+    T *yh = y + st0;
+    if (cs == 1) {
+        for (int e = threadIdx.x; e <= k; e += blockDim.x) yh[e] = (T)0.0;
+    } else if (cs >= 2) {
+        for (int e = threadIdx.x; e < k; e += blockDim.x) yh[1 + e] = (T)((double)yh[1 + e] / (1.0 + lam * dt[e]));
+        if (threadIdx.x == 0) yh[0] = (T)t;
+    }
+    __syncthreads();
+    double my2 = 0.0;
+    if (dy != nullptr) {
+        for (int e = threadIdx.x; e <= k; e += blockDim.x) {
+            const double d = (double)yh[e] - (double)ys[st0 + e];
+            dy[st0 + e] = (T)d;
+            my2 += d * d;
+        }
+    }
+    if (part != nullptr) {
+        my2 = block_sum(my2, lds);
+        if (threadIdx.x == 0) part[blockIdx.x] = my2;
+    }
+}
+
+template <typename T, int L>
+__global__ __launch_bounds__(kBlock) void k_atdy(Csr<T> Gt, int64_t n, const T *__restrict__ dy,
+                                                 const T *__restrict__ x, const T *__restrict__ xs,
+                                                 StepState *st, int slot, double *__restrict__ part) {
+    __shared__ double lds[kBlock / 64];
+    if (halted(st, slot)) return;
+    const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t row = gid / L;
+    const int lane = (int)(gid % L);
+    const double acc = row_dot<T, L>(Gt, row, lane, dy);
+    double v = 0.0;
+    if (row < n && lane == 0) v = ((double)x[row] - (double)xs[row]) * acc;
+    v = block_sum(v, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// Post-decision logic of an attempt (the accept branch of
+// pdhg_one_iter_..._adaptive_step_size! and the inner loop of
+// pdhg_main_iter_..._aggressive!, rpdhg_alg_gpu_gen_scaling.jl:746-767,843-860).
+__device__ void step_continue(StepState *st, int slot) {
+    const bool accept = st->accepted != 0;
+    const bool force = st->force != 0;
+    if (accept || force) {
+        // force: the series returns eta = eta_prime (scaling.jl:752-765)
+        const double eta_acc = fmax(force && !accept ? st->eta_prime : st->eta_try, 1e-30);
+        st->reflect_k = (double)(st->inner + 1);
+        st->reflect_eta = eta_acc;
+        st->reflect_cum = st->eta_cum;
+        st->reflect_slot = slot;
+        st->eta_cum += eta_acc;
+        st->inner += 1;
+        st->eta = fmax(st->eta_prime, 1e-6);   // eta = max(eta, 1e-6) at series start, :667
+        st->count = 0;
+        st->dec = st->dec_next;
+        const bool art = ((double)st->inner >= (double)st->iter * 0.223 && st->iter > 10000) ||
+                         (force && st->inner > 100);
+        const bool rc = (st->restart_freq > 0 && st->iter % st->restart_freq == 0) ||
+                        (st->kkt_restart_freq > 0 && st->iter % st->kkt_restart_freq == 0);
+        if (art) { st->reason = force ? 8 : 4; st->stop_at = slot + 1; }
+        else if (rc) { st->reason = 2; st->stop_at = slot + 1; }
+    } else {
+        st->eta = st->eta_prime;
+        st->reflect_slot = -1;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_step(StepState *st, int slot, const double *__restrict__ part,
+                                                 int64_t stride, int n1, int n2, int n3) {
+    __shared__ double lds[kBlock / 64];
+    if (halted(st, slot)) return;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int i = threadIdx.x; i < n1; i += kBlock) a += part[i];
+    for (int i = threadIdx.x; i < n2; i += kBlock) b += part[stride + i];
+    for (int i = threadIdx.x; i < n3; i += kBlock) c += part[2 * stride + i];
+    a = block_sum(a, lds);
+    b = block_sum(b, lds);
+    c = block_sum(c, lds);
+    if (threadIdx.x != 0) return;
+    st->iter += 1;
+    st->iter_stepsize += 1;
+    const double omega = st->omega, eta = st->eta;
+    st->eta_try = eta;
+    st->tau = eta / omega;
+    st->sigma = eta * omega;
+    st->dx2 = a;
+    st->dy2 = b;
+    st->dot = c;
+    // eta_bar and eta', scaling.jl:712-718
+    const double num = omega * a + b / omega;
+    const double den = 2.0 * fabs(c);
+    const double eta_bar = num / (den + 1e-20);
+    const double k1 = (double)(st->iter_stepsize + 1);
+    const double eta_prime = fmin((1.0 - pow(k1, -0.3)) * eta_bar, (1.0 + pow(k1, -0.6)) * eta);
+    st->eta_bar = eta_bar;
+    st->eta_prime = eta_prime;
+    const bool accept = eta <= eta_bar;
+    st->accepted = accept ? 1 : 0;
+    if (!accept) st->count += 1;
+    st->force = (!accept && st->count > 1000000) ? 1 : 0;
+    if (st->iter % st->check_freq == 0) {
+        // exit check is due before the acceptance test: the host decides
+        st->reason = 1;
+        st->stop_at = slot + 1;
+        st->reflect_slot = -1;
+        return;
+    }
+    step_continue(st, slot);
+}
+
+__global__ void k_continue(StepState *st, int slot) { step_continue(st, slot); }
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_reflect(Iter<T> it, int64_t n, int64_t m, StepState *st, int slot) {
+    if (st->reflect_slot != slot) return;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const double k = st->reflect_k, e = st->extra, eta = st->reflect_eta, cum = st->reflect_cum;
+    const double tmp = (k + 1.0) / (k + 2.0), tail = 1.0 / (k + 2.0);
+    if (i < n) {
+        const double xv = (double)it.x[i], xl = (double)it.xs[i];
+        const T nx = (T)(tmp * ((1.0 + e) * xv - e * xl) + tail * xv);
+        it.xs[i] = nx;
+        it.xm[i] = (T)(((double)it.xm[i] * cum + (double)nx * eta) / (cum + eta));
+    }
+    if (i < m) {
+        const double yv = (double)it.y[i], yl = (double)it.ys[i];
+        const T ny = (T)(tmp * ((1.0 + e) * yv - e * yl) + tail * yv);
+        it.ys[i] = ny;
+        it.ym[i] = (T)(((double)it.ym[i] * cum + (double)ny * eta) / (cum + eta));
+    }
+}
+
+static inline unsigned nblocks(int64_t work) { return (unsigned)((work + kBlock - 1) / kBlock); }
+
+#define PDCS_LANES(L, CALL) \
+    switch (L) {            \
+    case 1: CALL(1); break; \
+    case 2: CALL(2); break; \
+    case 4: CALL(4); break; \
+    case 8: CALL(8); break; \
+    case 16: CALL(16); break; \
+    case 32: CALL(32); break; \
+    default: CALL(64); break; \
+    }
+
+static inline int soc_blocks(const ConeLayout &K) {
+    return K.nsoc_small > 0 ? (int)nblocks((int64_t)K.nsoc_small * K.soc_small_group) : 0;
+}
+
+template <typename T>
+static void launch_soc(hipStream_t s, const ConeLayout &K, const double *d2, T *y, const T *ys,
+                       T *dy, StepState *st, int slot, double *part) {
+    if (K.nsoc_small > 0) {
+        const unsigned nb = (unsigned)soc_blocks(K);
+#define SOCG(G) k_soc_group<T, G><<<nb, kBlock, 0, s>>>(K, d2, y, ys, dy, st, slot, part)
+        switch (K.soc_small_group) {
+        case 8: SOCG(8); break;
+        case 16: SOCG(16); break;
+        case 32: SOCG(32); break;
+        default: SOCG(64); break;
+        }
+#undef SOCG
+    }
+    if (K.nsoc_big > 0)
+        k_soc_block<T><<<K.nsoc_big, 1024, 0, s>>>(K, d2, y, ys, dy, st, slot,
+                                                   part ? part + (K.nsoc_small > 0 ? soc_blocks(K) : 0) : nullptr);
+}
+
+template <typename T>
+void launch_attempt(hipStream_t s, int slot, const Csr<T> &G, const Csr<T> &Gt, const ProbVec<T> &P,
+                    const ConeLayout &K, const double *d2, Iter<T> it, StepState *st, double *part,
+                    int64_t stride) {
+    const unsigned bp = nblocks(P.n * Gt.lanes);
+    const unsigned bd = nblocks(P.m * G.lanes);
+#define KP(L) k_primal<T, L><<<bp, kBlock, 0, s>>>(Gt, P, it.ys, it.xs, it.x, it.xbar, st, slot, part)
+    PDCS_LANES(Gt.lanes, KP)
+#undef KP
+#define KD(L) k_dual<T, L><<<bd, kBlock, 0, s>>>(G, P, K, it.xbar, it.ys, it.y, it.dy, st, slot, part + stride)
+    PDCS_LANES(G.lanes, KD)
+#undef KD
+    launch_soc<T>(s, K, d2, it.y, it.ys, it.dy, st, slot, part + stride + bd);
+#define KA(L) k_atdy<T, L><<<bp, kBlock, 0, s>>>(Gt, P.n, it.dy, it.x, it.xs, st, slot, part + 2 * stride)
+    PDCS_LANES(Gt.lanes, KA)
+#undef KA
+    const int ncone = soc_blocks(K) + K.nsoc_big;
+    k_step<<<1, kBlock, 0, s>>>(st, slot, part, stride, (int)bp, (int)(bd + ncone), (int)bp);
+    k_reflect<T><<<nblocks(P.n > P.m ? P.n : P.m), kBlock, 0, s>>>(it, P.n, P.m, st, slot);
+}
+template void launch_attempt<double>(hipStream_t, int, const Csr<double> &, const Csr<double> &,
+                                     const ProbVec<double> &, const ConeLayout &, const double *,
+                                     Iter<double>, StepState *, double *, int64_t);
+template void launch_attempt<float>(hipStream_t, int, const Csr<float> &, const Csr<float> &,
+                                    const ProbVec<float> &, const ConeLayout &, const double *,
+                                    Iter<float>, StepState *, double *, int64_t);
+
+void launch_continue(hipStream_t s, int slot, StepState *st) { k_continue<<<1, 1, 0, s>>>(st, slot); }
+
+template <typename T>
+void launch_reflect(hipStream_t s, int slot, Iter<T> it, int64_t n, int64_t m, StepState *st) {
+    k_reflect<T><<<nblocks(n > m ? n : m), kBlock, 0, s>>>(it, n, m, st, slot);
+}
+template void launch_reflect<double>(hipStream_t, int, Iter<double>, int64_t, int64_t, StepState *);
+template void launch_reflect<float>(hipStream_t, int, Iter<float>, int64_t, int64_t, StepState *);
+
+// ---------------------------------------------------------- generic ops --
+template <typename T, int L>
+__global__ __launch_bounds__(kBlock) void k_spmv(Csr<T> A, const T *__restrict__ x, T *__restrict__ y) {
+    const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t row = gid / L;
+    const int lane = (int)(gid % L);
+    const double acc = row_dot<T, L>(A, row, lane, x);
+    if (row < A.rows && lane == 0) y[row] = (T)acc;
+}
+template <typename T, int L>
+__global__ __launch_bounds__(kBlock) void k_spmv_acc(Csr<T> A, const T *__restrict__ x, double *__restrict__ y) {
+    const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t row = gid / L;
+    const int lane = (int)(gid % L);
+    const double acc = row_dot<T, L>(A, row, lane, x);
+    if (row < A.rows && lane == 0) y[row] = acc;
+}
+template <typename T>
+void launch_spmv(hipStream_t s, const Csr<T> &A, const T *x, T *y) {
+    const unsigned b = nblocks(A.rows * A.lanes);
+#define KS(L) k_spmv<T, L><<<b, kBlock, 0, s>>>(A, x, y)
+    PDCS_LANES(A.lanes, KS)
+#undef KS
+}
+template void launch_spmv<double>(hipStream_t, const Csr<double> &, const double *, double *);
+template void launch_spmv<float>(hipStream_t, const Csr<float> &, const float *, float *);
+template <typename T>
+void launch_spmv_acc(hipStream_t s, const Csr<T> &A, const T *x, double *y) {
+    const unsigned b = nblocks(A.rows * A.lanes);
+#define KS(L) k_spmv_acc<T, L><<<b, kBlock, 0, s>>>(A, x, y)
+    PDCS_LANES(A.lanes, KS)
+#undef KS
+}
+template void launch_spmv_acc<double>(hipStream_t, const Csr<double> &, const double *, double *);
+template void launch_spmv_acc<float>(hipStream_t, const Csr<float> &, const float *, double *);
+
+template <typename T>
+__global__ void k_axpy_box(T *out, const T *a, double alpha, const T *b, const T *bl, const T *bu,
+                           int64_t n, int64_t nb) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    double v = (double)a[i] + alpha * (double)b[i];
+    if (i < nb) v = fmin(fmax(v, (double)bl[i]), (double)bu[i]);
+    out[i] = (T)v;
+}
+template <typename T>
+void launch_axpy_box(hipStream_t s, T *out, const T *a, double alpha, const T *b, const T *bl,
+                     const T *bu, int64_t n, int64_t nb) {
+    k_axpy_box<T><<<nblocks(n), kBlock, 0, s>>>(out, a, alpha, b, bl, bu, n, nb);
+}
+template void launch_axpy_box<double>(hipStream_t, double *, const double *, double, const double *,
+                                      const double *, const double *, int64_t, int64_t);
+template void launch_axpy_box<float>(hipStream_t, float *, const float *, double, const float *,
+                                     const float *, const float *, int64_t, int64_t);
+
+template <typename T>
+__global__ void k_axpy(T *out, const T *a, double alpha, const T *b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) out[i] = (T)((double)a[i] + alpha * (double)b[i]);
+}
+template <typename T>
+void launch_axpy(hipStream_t s, T *out, const T *a, double alpha, const T *b, int64_t n) {
+    k_axpy<T><<<nblocks(n), kBlock, 0, s>>>(out, a, alpha, b, n);
+}
+template void launch_axpy<double>(hipStream_t, double *, const double *, double, const double *, int64_t);
+template void launch_axpy<float>(hipStream_t, float *, const float *, double, const float *, int64_t);
+
+template <typename T>
+__global__ void k_nonneg(T *y, int64_t b, int64_t e) {
+    const int64_t i = b + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < e) y[i] = (T)fmax((double)y[i], 0.0);
+}
+template <typename T>
+void launch_project_dual(hipStream_t s, const ConeLayout &K, const double *d, const double *d2, T *y) {
+    (void)d;
+    if (K.mnonneg > 0) k_nonneg<T><<<nblocks(K.mnonneg), kBlock, 0, s>>>(y, K.mzero, K.mzero + K.mnonneg);
+    launch_soc<T>(s, K, d2, y, nullptr, nullptr, nullptr, 0, nullptr);
+}
+template void launch_project_dual<double>(hipStream_t, const ConeLayout &, const double *, const double *, double *);
+template void launch_project_dual<float>(hipStream_t, const ConeLayout &, const double *, const double *, float *);
+
+// plain SOC projection (soc_proj!, soc_rsoc_proj.jl:4-18), wave per cone
+__global__ __launch_bounds__(kBlock) void k_soc_plain(const int32_t *start, const int32_t *dim, int64_t ncone,
+                                                      double *r) {
+    const int64_t cone = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / 64;
+    const int lane = threadIdx.x & 63;
+    if (cone >= ncone) return;
+    double *v = r + start[cone];
+    const int k = dim[cone] - 1;
+    double nx = 0.0;
+    for (int e = lane; e < k; e += 64) nx += v[1 + e] * v[1 + e];
+    nx = sqrt(wave_sum(nx));
+    const double t = v[0];
+    if (nx <= -t) {
+        for (int e = lane; e <= k; e += 64) v[e] = 0.0;
+    } else if (nx > t) {
+        const double c = (1.0 + t / nx) / 2.0;
+        for (int e = lane; e < k; e += 64) v[1 + e] *= c;
+        if (lane == 0) v[0] = nx * c;
+    }
+}
+__global__ void k_zero_nonneg(double *r, int64_t mzero, int64_t mnonneg) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < mzero) r[i] = 0.0;
+    else if (i < mzero + mnonneg) r[i] = fmax(r[i], 0.0);
+}
+void launch_project_primal_cone(hipStream_t s, const ConeLayout &K, double *r) {
+    if (K.mzero + K.mnonneg > 0) k_zero_nonneg<<<nblocks(K.mzero + K.mnonneg), kBlock, 0, s>>>(r, K.mzero, K.mnonneg);
+    if (K.nsoc_small > 0)
+        k_soc_plain<<<nblocks((int64_t)K.nsoc_small * 64), kBlock, 0, s>>>(K.soc_small_start, K.soc_small_dim, K.nsoc_small, r);
+    if (K.nsoc_big > 0)
+        k_soc_plain<<<nblocks((int64_t)K.nsoc_big * 64), kBlock, 0, s>>>(K.soc_big_start, K.soc_big_dim, K.nsoc_big, r);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dots(int nd, const T *a0, const T *b0, const T *a1, const T *b1,
+                                                 const T *a2, const T *b2, const T *a3, const T *b3, int64_t n,
+                                                 double *part, int64_t stride) {
+    __shared__ double lds[kBlock / 64];
+    const T *A[4] = {a0, a1, a2, a3};
+    const T *B[4] = {b0, b1, b2, b3};
+    for (int q = 0; q < nd; ++q) {
+        double v = 0.0;
+        for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+            v += (double)A[q][i] * (double)B[q][i];
+        v = block_sum(v, lds);
+        if (threadIdx.x == 0) part[q * stride + blockIdx.x] = v;
+    }
+}
+__global__ __launch_bounds__(kBlock) void k_dots_final(int nd, const double *part, int64_t stride, int nb, double *out) {
+    __shared__ double lds[kBlock / 64];
+    for (int q = 0; q < nd; ++q) {
+        double v = 0.0;
+        for (int i = threadIdx.x; i < nb; i += kBlock) v += part[q * stride + i];
+        v = block_sum(v, lds);
+        if (threadIdx.x == 0) out[q] = v;
+    }
+}
+template <typename T>
+void launch_dots(hipStream_t s, int nd, const T *const *a, const T *const *b, int64_t n, double *part,
+                 double *out) {
+    const int nb = (int)(nblocks(n) < 512 ? nblocks(n) : 512);
+    const T *A[4] = {nullptr, nullptr, nullptr, nullptr}, *B[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (int q = 0; q < nd; ++q) { A[q] = a[q]; B[q] = b[q]; }
+    k_dots<T><<<nb > 0 ? nb : 1, kBlock, 0, s>>>(nd, A[0], B[0], A[1], B[1], A[2], B[2], A[3], B[3], n, part, 512);
+    k_dots_final<<<1, kBlock, 0, s>>>(nd, part, 512, nb > 0 ? nb : 1, out);
+}
+template void launch_dots<double>(hipStream_t, int, const double *const *, const double *const *, int64_t,
+                                  double *, double *);
+template void launch_dots<float>(hipStream_t, int, const float *const *, const float *const *, int64_t,
+                                 double *, double *);
+
+template <typename T>
+__global__ void k_sub(T *z, const T *a, const T *b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) z[i] = (T)((double)a[i] - (double)b[i]);
+}
+template <typename T>
+void launch_sub(hipStream_t s, T *z, const T *a, const T *b, int64_t n) {
+    k_sub<T><<<nblocks(n), kBlock, 0, s>>>(z, a, b, n);
+}
+template void launch_sub<double>(hipStream_t, double *, const double *, const double *, int64_t);
+template void launch_sub<float>(hipStream_t, float *, const float *, const float *, int64_t);
+
+template <typename T>
+void launch_copy(hipStream_t s, T *dst, const T *src, int64_t n) {
+    hipMemcpyAsync(dst, src, sizeof(T) * (size_t)n, hipMemcpyDeviceToDevice, s);
+}
+template void launch_copy<double>(hipStream_t, double *, const double *, int64_t);
+template void launch_copy<float>(hipStream_t, float *, const float *, int64_t);
+
+template <typename Td, typename Ts>
+__global__ void k_convert(Td *d, const Ts *s, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) d[i] = (Td)s[i];
+}
+template <typename Td, typename Ts>
+void launch_convert(hipStream_t s, Td *dst, const Ts *src, int64_t n) {
+    k_convert<Td, Ts><<<nblocks(n), kBlock, 0, s>>>(dst, src, n);
+}
+template void launch_convert<float, double>(hipStream_t, float *, const double *, int64_t);
+template void launch_convert<double, float>(hipStream_t, double *, const float *, int64_t);
+
+// ------------------------------------------------------------------ KKT --
+// Partial layout: quantity q of block b at part[q * kKktStride + b].
+constexpr int64_t kKktStride = kKktStrideHost;
+enum { Q_POBJ, Q_ATYN1, Q_DOBJ, Q_SN1, Q_D2, Q_ATYINF, Q_SINF, Q_DINF, Q_AXN1, Q_P2, Q_AXINF, Q_PINF, Q_PROJINF, Q_N };
+
+// n side: G'y, dual residual, objective pieces (converge_info_calculation_diagonal!,
+// rpdhg_alg_gpu_gen_scaling.jl:128-140,160-166)
+template <typename T, int L>
+__global__ __launch_bounds__(kBlock) void k_kkt_n(KktData D, const T *__restrict__ x, const T *__restrict__ y,
+                                                  double *__restrict__ part, int64_t bstride) {
+    __shared__ double lds[kBlock / 64];
+    const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t j = gid / L;
+    const int lane = (int)(gid % L);
+    double s_[5] = {0, 0, 0, 0, 0}, m_[3] = {0, 0, 0};
+    // row dot with double matrix and T vector
+    double at = 0.0;
+    if (j < D.n) {
+        const int32_t s = D.Gt.rowptr[j], e = D.Gt.rowptr[j + 1];
+        for (int32_t k = s + lane; k < e; k += L) at += D.Gt.val[k] * (double)y[D.Gt.col[k]];
+    }
+    if (L > 1) at = group_sum<L>(at);
+    if (j < D.n && lane == 0) {
+        const double aty = at * D.Dr[j];              // G' y_raw = Dr .* (Gs' ys)
+        const double xr = (double)x[j] / D.Dr[j];
+        const double lam = D.c_raw[j] - aty;
+        s_[0] = D.c_raw[j] * xr;
+        s_[1] = fabs(aty);
+        m_[0] = fabs(aty);
+        double sp = 0.0;
+        if (j < D.nb) {
+            s_[2] = D.blf[j] * fmax(lam, 0.0) + D.buf[j] * fmin(lam, 0.0);
+            sp = fmin(fmax(lam, D.laml[j]), D.lamu[j]);
+        }
+        s_[3] = fabs(sp);
+        m_[1] = fabs(sp);
+        const double r = sp - lam;
+        s_[4] = r * r;
+        m_[2] = fabs(r);
+    }
+    const int qs[5] = {Q_POBJ, Q_ATYN1, Q_DOBJ, Q_SN1, Q_D2};
+    const int qm[3] = {Q_ATYINF, Q_SINF, Q_DINF};
+    for (int q = 0; q < 5; ++q) {
+        const double v = block_sum(s_[q], lds);
+        if (threadIdx.x == 0) part[qs[q] * kKktStride + bstride + blockIdx.x] = v;
+    }
+    for (int q = 0; q < 3; ++q) {
+        const double v = block_max(m_[q], lds);
+        if (threadIdx.x == 0) part[qm[q] * kKktStride + bstride + blockIdx.x] = v;
+    }
+}
+
+// m side: G x, h'y, residual of the elementwise cones; cone rows go to rtmp
+template <typename T, int L>
+__global__ __launch_bounds__(kBlock) void k_kkt_m(KktData D, const T *__restrict__ x, const T *__restrict__ y,
+                                                  double *__restrict__ part, int64_t bstride, int64_t dobj_off) {
+    __shared__ double lds[kBlock / 64];
+    const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t i = gid / L;
+    const int lane = (int)(gid % L);
+    double ax = 0.0;
+    if (i < D.m) {
+        const int32_t s = D.G.rowptr[i], e = D.G.rowptr[i + 1];
+        for (int32_t k = s + lane; k < e; k += L) ax += D.G.val[k] * (double)x[D.G.col[k]];
+    }
+    if (L > 1) ax = group_sum<L>(ax);
+    double s_[3] = {0, 0, 0}, m_[3] = {0, 0, 0};
+    if (i < D.m && lane == 0) {
+        const double axr = ax * D.Dl[i];             // G x_raw = Dl .* (Gs xs)
+        const double yr = (double)y[i] / D.Dl[i];
+        s_[0] = D.h_raw[i] * yr;
+        s_[1] = fabs(axr);
+        m_[0] = fabs(axr);
+        const double r = axr - D.h_raw[i];
+        if (i < D.K.cone_begin) {
+            const double p = i < D.K.mzero ? 0.0 : fmax(r, 0.0);
+            const double pr = p - r;
+            s_[2] = pr * pr;
+            m_[1] = fabs(pr);
+            m_[2] = fabs(p);
+        } else {
+            D.rtmp[i] = r;
+        }
+    }
+    const int qs[3] = {Q_DOBJ, Q_AXN1, Q_P2};
+    const int qm[3] = {Q_AXINF, Q_PINF, Q_PROJINF};
+    for (int q = 0; q < 3; ++q) {
+        const double v = block_sum(s_[q], lds);
+        if (threadIdx.x == 0) part[qs[q] * kKktStride + (q == 0 ? dobj_off : bstride) + blockIdx.x] = v;
+    }
+    for (int q = 0; q < 3; ++q) {
+        const double v = block_max(m_[q], lds);
+        if (threadIdx.x == 0) part[qm[q] * kKktStride + bstride + blockIdx.x] = v;
+    }
+}
+
+// cone rows: plain SOC projection of r = Gx - h in place on a copy, residual sums
+__global__ __launch_bounds__(kBlock) void k_kkt_soc(const int32_t *start, const int32_t *dim, int64_t ncone,
+                                                    const double *__restrict__ r, double *__restrict__ part,
+                                                    int64_t bstride) {
+    __shared__ double lds[kBlock / 64];
+    const int64_t cone = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / 64;
+    const int lane = threadIdx.x & 63;
+    double p2 = 0.0, pinf = 0.0, projinf = 0.0;
+    if (cone < ncone) {
+        const double *v = r + start[cone];
+        const int k = dim[cone] - 1;
+        double nx = 0.0;
+        for (int e = lane; e < k; e += 64) nx += v[1 + e] * v[1 + e];
+        nx = sqrt(wave_sum(nx));
+        const double t = v[0];
+        double c = 1.0, th = t;
+        if (nx <= -t) { c = 0.0; th = 0.0; }
+        else if (nx > t) { c = (1.0 + t / nx) / 2.0; th = nx * c; }
+        for (int e = lane; e <= k; e += 64) {
+            const double p = e == 0 ? th : v[e] * c;
+            const double pr = p - v[e];
+            p2 += pr * pr;
+            pinf = fmax(pinf, fabs(pr));
+            projinf = fmax(projinf, fabs(p));
+        }
+    }
+    p2 = block_sum(p2, lds);
+    pinf = block_max(pinf, lds);
+    projinf = block_max(projinf, lds);
+    if (threadIdx.x == 0) {
+        part[Q_P2 * kKktStride + bstride + blockIdx.x] = p2;
+        part[Q_PINF * kKktStride + bstride + blockIdx.x] = pinf;
+        part[Q_PROJINF * kKktStride + bstride + blockIdx.x] = projinf;
+    }
+}
+
+struct KktCounts {
+    int c[Q_N];
+};
+__global__ __launch_bounds__(kBlock) void k_kkt_final(const double *part, KktCounts counts, KktOut *out) {
+    __shared__ double lds[kBlock / 64];
+    const bool ismax[Q_N] = {false, false, false, false, false, true, true, true, false, false, true, true, true};
+    double res[Q_N];
+    for (int q = 0; q < Q_N; ++q) {
+        double v = 0.0;
+        for (int i = threadIdx.x; i < counts.c[q]; i += kBlock)
+            v = ismax[q] ? fmax(v, part[q * kKktStride + i]) : v + part[q * kKktStride + i];
+        res[q] = ismax[q] ? block_max(v, lds) : block_sum(v, lds);
+    }
+    if (threadIdx.x == 0) {
+        out->pobj = res[Q_POBJ];
+        out->dobj = res[Q_DOBJ];
+        out->atyInf = res[Q_ATYINF];
+        out->atyN1 = res[Q_ATYN1];
+        out->sInf = res[Q_SINF];
+        out->sN1 = res[Q_SN1];
+        out->d2 = res[Q_D2];
+        out->dInf = res[Q_DINF];
+        out->axInf = res[Q_AXINF];
+        out->axN1 = res[Q_AXN1];
+        out->p2 = res[Q_P2];
+        out->pInf = res[Q_PINF];
+        out->projInf = res[Q_PROJINF];
+    }
+}
+
+template <typename T>
+void launch_kkt(hipStream_t s, const KktData &D, const T *x, const T *y, double *part, KktOut *out) {
+    const unsigned bn = nblocks(D.n * D.Gt.lanes);
+    const unsigned bm = nblocks(D.m * D.G.lanes);
+    const unsigned bs1 = D.K.nsoc_small > 0 ? nblocks((int64_t)D.K.nsoc_small * 64) : 0;
+    const unsigned bs2 = D.K.nsoc_big > 0 ? nblocks((int64_t)D.K.nsoc_big * 64) : 0;
+    // counts per quantity; the n-side and m-side share Q_DOBJ (n side first)
+    KktCounts kc;
+    int *counts = kc.c;
+    for (int q = 0; q < Q_N; ++q) counts[q] = 0;
+    counts[Q_POBJ] = counts[Q_ATYN1] = counts[Q_SN1] = counts[Q_D2] = (int)bn;
+    counts[Q_ATYINF] = counts[Q_SINF] = counts[Q_DINF] = (int)bn;
+    counts[Q_DOBJ] = (int)(bn + bm);
+    counts[Q_AXN1] = counts[Q_AXINF] = (int)bm;
+    counts[Q_P2] = counts[Q_PINF] = counts[Q_PROJINF] = (int)(bm + bs1 + bs2);
+#define KN(L) k_kkt_n<T, L><<<bn, kBlock, 0, s>>>(D, x, y, part, 0)
+    PDCS_LANES(D.Gt.lanes, KN)
+#undef KN
+    // the m side writes its Q_DOBJ partials after the n side's
+#define KM(L) k_kkt_m<T, L><<<bm, kBlock, 0, s>>>(D, x, y, part, 0, (int64_t)bn)
+    PDCS_LANES(D.G.lanes, KM)
+#undef KM
+    if (bs1) k_kkt_soc<<<bs1, kBlock, 0, s>>>(D.K.soc_small_start, D.K.soc_small_dim, D.K.nsoc_small, D.rtmp, part, bm);
+    if (bs2) k_kkt_soc<<<bs2, kBlock, 0, s>>>(D.K.soc_big_start, D.K.soc_big_dim, D.K.nsoc_big, D.rtmp, part, bm + bs1);
+    k_kkt_final<<<1, kBlock, 0, s>>>(part, kc, out);
+}
+template void launch_kkt<double>(hipStream_t, const KktData &, const double *, const double *, double *, KktOut *);
+template void launch_kkt<float>(hipStream_t, const KktData &, const float *, const float *, double *, KktOut *);
+
+// -------------------------------------------------------------- scaling --
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_rowstat(Csr<double> A, double *out, int mode) {
+    const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t row = gid / L;
+    const int lane = (int)(gid % L);
+    double v = 0.0;
+    if (row < A.rows) {
+        const int32_t s = A.rowptr[row], e = A.rowptr[row + 1];
+        for (int32_t k = s + lane; k < e; k += L) v = mode ? v + fabs(A.val[k]) : fmax(v, fabs(A.val[k]));
+    }
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o, L);
+        v = mode ? v + w : fmax(v, w);
+    }
+    if (row < A.rows && lane == 0) out[row] = v;
+}
+void launch_row_colmax(hipStream_t s, const Csr<double> &G, const Csr<double> &Gt, double *rowv, double *colv,
+                       int mode) {
+#define KR(L) k_rowstat<L><<<nblocks(G.rows * G.lanes), kBlock, 0, s>>>(G, rowv, mode)
+    PDCS_LANES(G.lanes, KR)
+#undef KR
+#define KC(L) k_rowstat<L><<<nblocks(Gt.rows * Gt.lanes), kBlock, 0, s>>>(Gt, colv, mode)
+    PDCS_LANES(Gt.lanes, KC)
+#undef KC
+}
+__global__ void k_scale_vals(double *vals, const int32_t *rowptr, const int32_t *col, int64_t m, const double *Dl,
+                             const double *Dr) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= m) return;
+    const double dl = Dl[i];
+    for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) vals[k] /= (dl * Dr[col[k]]);
+}
+void launch_scale_vals(hipStream_t s, double *vals, const int32_t *rowptr, const int32_t *col, int64_t m,
+                       const double *Dl, const double *Dr) {
+    k_scale_vals<<<nblocks(m), kBlock, 0, s>>>(vals, rowptr, col, m, Dl, Dr);
+}
+__global__ void k_sqrt_fix(double *v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        const double w = sqrt(v[i]);
+        v[i] = w == 0.0 ? 1.0 : w;
+    }
+}
+void launch_sqrt_fix(hipStream_t s, double *v, int64_t n) { k_sqrt_fix<<<nblocks(n), kBlock, 0, s>>>(v, n); }
+__global__ void k_mul(double *a, const double *b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) a[i] *= b[i];
+}
+void launch_mul(hipStream_t s, double *a, const double *b, int64_t n) { k_mul<<<nblocks(n), kBlock, 0, s>>>(a, b, n); }
+__global__ void k_gather(double *dst, const double *src, const int32_t *map, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) dst[i] = src[map[i]];
+}
+void launch_gather(hipStream_t s, double *dst, const double *src, const int32_t *map, int64_t n) {
+    k_gather<<<nblocks(n), kBlock, 0, s>>>(dst, src, map, n);
+}
+
+}  // namespace pdcs

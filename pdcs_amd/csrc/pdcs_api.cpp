@@ -1,0 +1,1204 @@
+// pdcs_api.cpp -- C ABI (include/pdcs.h) and host driver of libpdcs_hip.so.
+//
+// The driver restates the control flow of the reference GPU solver
+// (pdhg_main_iter_average_diagonal_rescaling_restarts_adaptive_weight_resolving_aggressive!,
+// src/pdcs_gpu/rpdhg_alg_gpu_gen_scaling.jl:770-896) around the device-resident
+// attempt chain of kernels.hip.  The host only steps in at halts: every
+// check_terminate_freq attempts (residual check), at restart checks and at
+// artificial restarts; between halts attempts are enqueued in chunks (replayed
+// as hipGraphs when params.use_graph) without any host synchronisation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pdcs.h"
+#include "internal.h"
+
+using namespace pdcs;
+
+struct pdcs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_tls_err;
+
+int fail(pdcs_ctx *ctx, int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    g_tls_err = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail((ctx), PDCS_ERR_HIP, "%s failed: %s (%s:%d)", #call,                \
+                        hipGetErrorString(e_), __FILE__, __LINE__);                         \
+    } while (0)
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int lanes_for(int64_t nnz, int64_t rows) {
+    const double avg = rows > 0 ? (double)nnz / (double)rows : 1.0;
+    int L = 1;
+    while (L < 64 && L < avg) L <<= 1;
+    return L;
+}
+
+struct DevBuf {
+    std::vector<void *> ptrs;
+    template <typename T>
+    T *alloc(int64_t n) {
+        void *p = nullptr;
+        if (hipMalloc(&p, sizeof(T) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess) return nullptr;
+        hipMemset(p, 0, sizeof(T) * (size_t)std::max<int64_t>(n, 1));
+        ptrs.push_back(p);
+        return (T *)p;
+    }
+    ~DevBuf() {
+        for (void *p : ptrs) hipFree(p);
+    }
+};
+
+// iterate vectors of one precision
+template <typename T>
+struct IterSet {
+    Iter<T> it{};
+    T *x0 = nullptr, *y0 = nullptr, *x0l = nullptr, *y0l = nullptr;   // restart anchors
+    T *tn1 = nullptr, *tn2 = nullptr, *tm1 = nullptr, *tm2 = nullptr;  // scratch
+    T *h1 = nullptr, *h2 = nullptr, *xd = nullptr, *yd = nullptr;
+    T *c = nullptr, *h = nullptr, *bl = nullptr, *bu = nullptr;        // scaled data
+    T *vals = nullptr, *tvals = nullptr;
+    bool ready = false;
+};
+
+struct ConvInfo {
+    double pobj, dobj, abs_gap, rel_gap;
+    double l2_abs_p, l2_rel_p, linf_abs_p, linf_rel_p, linf_rel_p_code;
+    double l2_abs_d, l2_rel_d, linf_abs_d, linf_rel_d;
+};
+
+}  // namespace
+
+struct pdcs_problem {
+    pdcs_ctx *ctx = nullptr;
+    int64_t n = 0, m = 0, nb = 0, nnz = 0;
+    DevBuf mem;
+    // host copies of the original data
+    std::vector<int32_t> rowptr, col, trowptr, tcol, tmap;
+    std::vector<double> vals, c, h, bl, bu;
+    int64_t mzero = 0, mnonneg = 0, nexp = 0, ndexp = 0;
+    std::vector<int64_t> soc;
+    // device structure
+    int32_t *d_rowptr = nullptr, *d_col = nullptr, *d_trowptr = nullptr, *d_tcol = nullptr, *d_tmap = nullptr;
+    double *d_vals = nullptr, *d_tvals = nullptr;     // scaled FP64
+    double *c_raw = nullptr, *h_raw = nullptr, *blf = nullptr, *buf = nullptr, *laml = nullptr, *lamu = nullptr;
+    double *Dr = nullptr, *Dl = nullptr, *dsc = nullptr, *dsc2 = nullptr;
+    double *rtmp = nullptr, *kpart = nullptr, *dpart = nullptr, *dout = nullptr, *apart = nullptr;
+    int64_t apart_stride = 0;
+    KktOut *kout = nullptr;
+    StepState *st = nullptr;
+    ConeLayout K{};
+    int32_t *ss_start = nullptr, *ss_dim = nullptr, *sb_start = nullptr, *sb_dim = nullptr;
+    int lanesG = 1, lanesGt = 1;
+    IterSet<double> f64;
+    IterSet<float> f32;
+    // host staging
+    StepState *h_st = nullptr;   // pinned
+    KktOut *h_kout = nullptr;    // pinned
+    double *h_dout = nullptr;    // pinned
+    double hN1 = 0, hNinf = 0, cN1 = 0, cNinf = 0;
+    // results of the last solve (unscaled)
+    std::vector<double> sol_x, sol_y;
+    bool solved = false;
+    std::vector<hipGraphExec_t> graphs64, graphs32;
+    int graph_chunk = 0;
+
+    ~pdcs_problem() {
+        for (auto g : graphs64) hipGraphExecDestroy(g);
+        for (auto g : graphs32) hipGraphExecDestroy(g);
+        if (h_st) hipHostFree(h_st);
+        if (h_kout) hipHostFree(h_kout);
+        if (h_dout) hipHostFree(h_dout);
+    }
+};
+
+// ===================================================================== ABI
+extern "C" {
+
+void pdcs_params_default(pdcs_params *p) {
+    std::memset(p, 0, sizeof(*p));
+    p->max_outer_iter = 10000;
+    p->max_inner_iter = 500000;
+    p->abs_tol = 1e-6;
+    p->rel_tol = 1e-6;
+    p->time_limit = INFINITY;
+    p->check_terminate_freq = 2000;
+    p->duality_gap_restart_freq = 2000;
+    p->kkt_restart_freq = 2000;
+    p->use_kkt_restart = 0;
+    p->use_duality_gap_restart = 1;
+    p->use_preconditioner = 1;
+    p->verbose = 0;
+    p->warm_start = 0;
+    p->precision = PDCS_FP64;
+    p->polish = 1;
+    p->bisect_intent = 0;
+    p->code_primal_denominator = 0;
+    p->use_graph = 1;
+    p->mixed_check_freq = 0;
+    p->polish_check_freq = 0;
+    p->mixed_switch_tol = 1e-5;
+}
+
+int pdcs_ctx_create(int device, pdcs_ctx **out) {
+    if (!out) return fail(nullptr, PDCS_ERR_ARG, "out is NULL");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(nullptr, PDCS_ERR_HIP, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(nullptr, PDCS_ERR_ARG, "device %d out of range", device);
+    auto *ctx = new pdcs_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return fail(nullptr, PDCS_ERR_HIP, "stream creation failed");
+    }
+    *out = ctx;
+    return PDCS_OK;
+}
+
+int pdcs_ctx_destroy(pdcs_ctx *ctx) {
+    if (!ctx) return PDCS_OK;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PDCS_OK;
+}
+
+const char *pdcs_last_error(pdcs_ctx *ctx) { return ctx ? ctx->err.c_str() : g_tls_err.c_str(); }
+
+}  // extern "C"
+
+namespace {
+
+int validate_cones(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const pdcs_cones *k) {
+    if (!k) return fail(ctx, PDCS_ERR_ARG, "cones is NULL");
+    if (k->nrsoc > 0) return fail(ctx, PDCS_ERR_UNSUPPORTED, "rotated SOC rows are not supported");
+    if (n <= 0 || m <= 0 || nb < 0 || nb > n) return fail(ctx, PDCS_ERR_ARG, "bad sizes n=%ld m=%ld nb=%ld", (long)n, (long)m, (long)nb);
+    int64_t s = k->mzero + k->mnonneg + 3 * (k->nexp + k->ndualexp);
+    for (int64_t b = 0; b < k->nsoc; ++b) {
+        if (k->soc[b] < 2) return fail(ctx, PDCS_ERR_ARG, "The size of the SOC must be greater than 1");
+        s += k->soc[b];
+    }
+    // rpdhg_alg_gpu_gen.jl:1333-1336
+    if (s != m) return fail(ctx, PDCS_ERR_ARG, "m does not match the sum of mGzero, mGnonnegative, socG, rsocG, and (expG + dual_expG) * 3");
+    if (k->nexp + k->ndualexp > 0) return fail(ctx, PDCS_ERR_UNSUPPORTED, "exponential cones are not supported by this build");
+    if (m >= (int64_t)1 << 31 || n >= (int64_t)1 << 31) return fail(ctx, PDCS_ERR_ARG, "sizes exceed int32 indexing");
+    return PDCS_OK;
+}
+
+template <typename T>
+int upload(pdcs_ctx *ctx, T *dst, const T *src, int64_t n) {
+    HIPCHK(ctx, hipMemcpyAsync(dst, src, sizeof(T) * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+    return PDCS_OK;
+}
+
+// Ruiz l_inf (10 passes) then Pock-Chambolle alpha=1 on the device
+// (rescale_problem!, src/pdcs_gpu/preprocess.jl:347-575).
+int scale_device(pdcs_problem *P) {
+    pdcs_ctx *ctx = P->ctx;
+    hipStream_t s = ctx->stream;
+    double *rowv = P->mem.alloc<double>(P->m), *colv = P->mem.alloc<double>(P->n);
+    if (!rowv || !colv) return fail(ctx, PDCS_ERR_HIP, "alloc failed");
+    Csr<double> G{P->d_rowptr, P->d_col, P->d_vals, P->m, P->lanesG};
+    Csr<double> Gt{P->d_trowptr, P->d_tcol, P->d_tvals, P->n, P->lanesGt};
+    for (int pass = 0; pass < 11; ++pass) {
+        const int mode = pass < 10 ? 0 : 1;
+        launch_row_colmax(s, G, Gt, rowv, colv, mode);
+        launch_sqrt_fix(s, rowv, P->m);
+        launch_sqrt_fix(s, colv, P->n);
+        launch_scale_vals(s, P->d_vals, P->d_rowptr, P->d_col, P->m, rowv, colv);
+        launch_gather(s, P->d_tvals, P->d_vals, P->d_tmap, P->nnz);
+        launch_mul(s, P->Dl, rowv, P->m);
+        launch_mul(s, P->Dr, colv, P->n);
+    }
+    HIPCHK(ctx, hipGetLastError());
+    return PDCS_OK;
+}
+
+template <typename T>
+IterSet<T> &iters(pdcs_problem *P);
+template <>
+IterSet<double> &iters<double>(pdcs_problem *P) { return P->f64; }
+template <>
+IterSet<float> &iters<float>(pdcs_problem *P) { return P->f32; }
+
+template <typename T>
+int alloc_iters(pdcs_problem *P) {
+    IterSet<T> &I = iters<T>(P);
+    if (I.ready) return PDCS_OK;
+    const int64_t n = P->n, m = P->m;
+    auto &M = P->mem;
+    I.it.x = M.alloc<T>(n); I.it.xs = M.alloc<T>(n); I.it.xm = M.alloc<T>(n); I.it.xbar = M.alloc<T>(n);
+    I.it.y = M.alloc<T>(m); I.it.ys = M.alloc<T>(m); I.it.ym = M.alloc<T>(m); I.it.dy = M.alloc<T>(m);
+    I.x0 = M.alloc<T>(n); I.y0 = M.alloc<T>(m); I.x0l = M.alloc<T>(n); I.y0l = M.alloc<T>(m);
+    I.tn1 = M.alloc<T>(n); I.tn2 = M.alloc<T>(n); I.tm1 = M.alloc<T>(m); I.tm2 = M.alloc<T>(m);
+    I.h1 = M.alloc<T>(n); I.h2 = M.alloc<T>(m); I.xd = M.alloc<T>(n); I.yd = M.alloc<T>(m);
+    I.c = M.alloc<T>(n); I.h = M.alloc<T>(m); I.bl = M.alloc<T>(P->nb); I.bu = M.alloc<T>(P->nb);
+    if (std::is_same<T, double>::value) {
+        I.vals = (T *)P->d_vals;
+        I.tvals = (T *)P->d_tvals;
+    } else {
+        I.vals = M.alloc<T>(P->nnz);
+        I.tvals = M.alloc<T>(P->nnz);
+        if (!I.vals || !I.tvals) return fail(P->ctx, PDCS_ERR_HIP, "alloc failed");
+        launch_convert(P->ctx->stream, (float *)I.vals, P->d_vals, P->nnz);
+        launch_convert(P->ctx->stream, (float *)I.tvals, P->d_tvals, P->nnz);
+    }
+    if (!I.it.x || !I.h2 || !I.bu || !I.yd) return fail(P->ctx, PDCS_ERR_HIP, "iterate allocation failed");
+    // scaled data in precision T: c/Dr, h/Dl, bl*Dr, bu*Dr
+    std::vector<double> hDr(P->n), hDl(P->m);
+    HIPCHK(P->ctx, hipMemcpy(hDr.data(), P->Dr, sizeof(double) * P->n, hipMemcpyDeviceToHost));
+    HIPCHK(P->ctx, hipMemcpy(hDl.data(), P->Dl, sizeof(double) * P->m, hipMemcpyDeviceToHost));
+    std::vector<T> c(n), h(m), bl(P->nb), bu(P->nb);
+    for (int64_t j = 0; j < n; ++j) c[j] = (T)(P->c[j] / hDr[j]);
+    for (int64_t i = 0; i < m; ++i) h[i] = (T)(P->h[i] / hDl[i]);
+    for (int64_t j = 0; j < P->nb; ++j) { bl[j] = (T)(P->bl[j] * hDr[j]); bu[j] = (T)(P->bu[j] * hDr[j]); }
+    HIPCHK(P->ctx, hipMemcpy(I.c, c.data(), sizeof(T) * n, hipMemcpyHostToDevice));
+    HIPCHK(P->ctx, hipMemcpy(I.h, h.data(), sizeof(T) * m, hipMemcpyHostToDevice));
+    if (P->nb > 0) {
+        HIPCHK(P->ctx, hipMemcpy(I.bl, bl.data(), sizeof(T) * P->nb, hipMemcpyHostToDevice));
+        HIPCHK(P->ctx, hipMemcpy(I.bu, bu.data(), sizeof(T) * P->nb, hipMemcpyHostToDevice));
+    }
+    I.ready = true;
+    return PDCS_OK;
+}
+
+int build_problem(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double *c, const int32_t *rowptr,
+                  const int32_t *colidx, const double *vals, const double *h, const double *bl, const double *bu,
+                  const pdcs_cones *cones, pdcs_problem **out) {
+    int rc = validate_cones(ctx, n, m, nb, cones);
+    if (rc) return rc;
+    if (!c || !rowptr || !colidx || !vals || !h || (nb > 0 && (!bl || !bu)) || !out)
+        return fail(ctx, PDCS_ERR_ARG, "NULL input array");
+    const int64_t nnz = rowptr[m];
+    if (rowptr[0] != 0 || nnz < 0) return fail(ctx, PDCS_ERR_ARG, "bad rowptr");
+    for (int64_t i = 0; i < m; ++i)
+        if (rowptr[i + 1] < rowptr[i]) return fail(ctx, PDCS_ERR_ARG, "rowptr not monotone at %ld", (long)i);
+    for (int64_t k = 0; k < nnz; ++k)
+        if (colidx[k] < 0 || colidx[k] >= n) return fail(ctx, PDCS_ERR_ARG, "column index out of range at %ld", (long)k);
+    for (int64_t j = 0; j < nb; ++j)
+        if (!(bl[j] <= bu[j])) return fail(ctx, PDCS_ERR_ARG, "Not all elements of bl are less than or equal to the corresponding elements in bu");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    auto *P = new pdcs_problem();
+    P->ctx = ctx;
+    P->n = n; P->m = m; P->nb = nb; P->nnz = nnz;
+    P->rowptr.assign(rowptr, rowptr + m + 1);
+    P->col.assign(colidx, colidx + nnz);
+    P->vals.assign(vals, vals + nnz);
+    P->c.assign(c, c + n);
+    P->h.assign(h, h + m);
+    P->bl.assign(bl, bl + nb);
+    P->bu.assign(bu, bu + nb);
+    P->mzero = cones->mzero; P->mnonneg = cones->mnonneg;
+    P->soc.assign(cones->soc, cones->soc + cones->nsoc);
+    // transpose (CSR of G') with the slot map used to keep G' values in sync
+    P->trowptr.assign(n + 1, 0);
+    for (int64_t k = 0; k < nnz; ++k) P->trowptr[colidx[k] + 1]++;
+    for (int64_t j = 0; j < n; ++j) P->trowptr[j + 1] += P->trowptr[j];
+    P->tcol.resize(nnz);
+    P->tmap.resize(nnz);
+    {
+        std::vector<int32_t> fill(P->trowptr.begin(), P->trowptr.end() - 1);
+        for (int64_t i = 0; i < m; ++i)
+            for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+                const int32_t sl = fill[colidx[k]]++;
+                P->tcol[sl] = (int32_t)i;
+                P->tmap[sl] = k;
+            }
+    }
+    std::vector<double> tvals(nnz);
+    for (int64_t k = 0; k < nnz; ++k) tvals[k] = vals[P->tmap[k]];
+    auto &M = P->mem;
+    P->d_rowptr = M.alloc<int32_t>(m + 1); P->d_col = M.alloc<int32_t>(nnz);
+    P->d_trowptr = M.alloc<int32_t>(n + 1); P->d_tcol = M.alloc<int32_t>(nnz); P->d_tmap = M.alloc<int32_t>(nnz);
+    P->d_vals = M.alloc<double>(nnz); P->d_tvals = M.alloc<double>(nnz);
+    P->c_raw = M.alloc<double>(n); P->h_raw = M.alloc<double>(m);
+    P->blf = M.alloc<double>(nb); P->buf = M.alloc<double>(nb); P->laml = M.alloc<double>(nb); P->lamu = M.alloc<double>(nb);
+    P->Dr = M.alloc<double>(n); P->Dl = M.alloc<double>(m); P->dsc = M.alloc<double>(m); P->dsc2 = M.alloc<double>(m);
+    P->rtmp = M.alloc<double>(m);
+    P->kpart = M.alloc<double>((kKktQuantities + 1) * kKktStrideHost);
+    P->dpart = M.alloc<double>(4 * 512);
+    P->dout = M.alloc<double>(8);
+    P->kout = M.alloc<KktOut>(1);
+    P->st = M.alloc<StepState>(1);
+    if (!P->kout || !P->st || !P->rtmp) { delete P; return fail(ctx, PDCS_ERR_HIP, "device allocation failed"); }
+    if (hipHostMalloc((void **)&P->h_st, sizeof(StepState), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&P->h_kout, sizeof(KktOut), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&P->h_dout, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        delete P;
+        return fail(ctx, PDCS_ERR_HIP, "pinned allocation failed");
+    }
+    P->lanesG = lanes_for(nnz, m);
+    P->lanesGt = lanes_for(nnz, n);
+    // residual partial arrays hold kKktStrideHost blocks per quantity
+    {
+        const int64_t bn = (n * P->lanesGt + kBlock - 1) / kBlock, bm = (m * P->lanesG + kBlock - 1) / kBlock;
+        if (bn + bm + (m + 3) / 4 + 1024 >= kKktStrideHost) {
+            delete P;
+            return fail(ctx, PDCS_ERR_UNSUPPORTED, "problem too large for the residual reduction layout");
+        }
+    }
+    // cone layout: small SOCs (dim <= 1024) in lane groups, larger ones a workgroup each
+    std::vector<int32_t> ss_start, ss_dim, sb_start, sb_dim;
+    int64_t off = cones->mzero + cones->mnonneg;
+    int maxsmall = 0;
+    for (int64_t b = 0; b < cones->nsoc; ++b) {
+        const int64_t q = cones->soc[b];
+        if (q <= 1024) { ss_start.push_back((int32_t)off); ss_dim.push_back((int32_t)q); maxsmall = std::max<int>(maxsmall, (int)q); }
+        else { sb_start.push_back((int32_t)off); sb_dim.push_back((int32_t)q); }
+        off += q;
+    }
+    P->K.m = m;
+    P->K.mzero = cones->mzero;
+    P->K.mnonneg = cones->mnonneg;
+    P->K.cone_begin = cones->mzero + cones->mnonneg;
+    P->K.nsoc_small = (int32_t)ss_start.size();
+    P->K.nsoc_big = (int32_t)sb_start.size();
+    int g = 8;
+    while (g < 64 && g < maxsmall - 1) g <<= 1;
+    P->K.soc_small_group = g;
+    P->K.nexp = 0; P->K.ndexp = 0; P->K.exp_begin = off;
+    if (!ss_start.empty()) {
+        P->ss_start = M.alloc<int32_t>((int64_t)ss_start.size()); P->ss_dim = M.alloc<int32_t>((int64_t)ss_dim.size());
+        upload(ctx, P->ss_start, ss_start.data(), (int64_t)ss_start.size());
+        upload(ctx, P->ss_dim, ss_dim.data(), (int64_t)ss_dim.size());
+    }
+    if (!sb_start.empty()) {
+        P->sb_start = M.alloc<int32_t>((int64_t)sb_start.size()); P->sb_dim = M.alloc<int32_t>((int64_t)sb_dim.size());
+        upload(ctx, P->sb_start, sb_start.data(), (int64_t)sb_start.size());
+        upload(ctx, P->sb_dim, sb_dim.data(), (int64_t)sb_dim.size());
+    }
+    P->K.soc_small_start = P->ss_start; P->K.soc_small_dim = P->ss_dim;
+    P->K.soc_big_start = P->sb_start; P->K.soc_big_dim = P->sb_dim;
+    // uploads
+    upload(ctx, P->d_rowptr, rowptr, m + 1);
+    upload(ctx, P->d_col, colidx, nnz);
+    upload(ctx, P->d_trowptr, P->trowptr.data(), n + 1);
+    upload(ctx, P->d_tcol, P->tcol.data(), nnz);
+    upload(ctx, P->d_tmap, P->tmap.data(), nnz);
+    upload(ctx, P->d_vals, vals, nnz);
+    upload(ctx, P->d_tvals, tvals.data(), nnz);
+    upload(ctx, P->c_raw, c, n);
+    upload(ctx, P->h_raw, h, m);
+    std::vector<double> blf(nb), buf(nb), laml(nb), lamu(nb), ones_n(n, 1.0), ones_m(m, 1.0);
+    for (int64_t j = 0; j < nb; ++j) {
+        blf[j] = std::isinf(bl[j]) ? 0.0 : bl[j];            // replace_inf_with_zero, utils.cu:192
+        buf[j] = std::isinf(bu[j]) ? 0.0 : bu[j];
+        laml[j] = bu[j] == INFINITY ? 0.0 : -INFINITY;       // gen_lambd, def_rpdhg_gen.jl:5-17
+        lamu[j] = bl[j] == -INFINITY ? 0.0 : INFINITY;
+    }
+    if (nb > 0) {
+        upload(ctx, P->blf, blf.data(), nb); upload(ctx, P->buf, buf.data(), nb);
+        upload(ctx, P->laml, laml.data(), nb); upload(ctx, P->lamu, lamu.data(), nb);
+    }
+    upload(ctx, P->Dr, ones_n.data(), n);
+    upload(ctx, P->Dl, ones_m.data(), m);
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    // norms of the original data (cal_constant, def_struct.jl:936)
+    for (int64_t i = 0; i < m; ++i) { P->hN1 += std::fabs(h[i]); P->hNinf = std::max(P->hNinf, std::fabs(h[i])); }
+    for (int64_t j = 0; j < n; ++j) { P->cN1 += std::fabs(c[j]); P->cNinf = std::max(P->cNinf, std::fabs(c[j])); }
+    *out = P;
+    return PDCS_OK;
+}
+
+// scaling vectors of the dual cone projections (scale_preconditioner!, preprocess.jl:173-297)
+int finish_scaling(pdcs_problem *P) {
+    std::vector<double> Dl(P->m), d(P->m, 1.0), d2(P->m);
+    HIPCHK(P->ctx, hipMemcpy(Dl.data(), P->Dl, sizeof(double) * P->m, hipMemcpyDeviceToHost));
+    int64_t off = P->mzero + P->mnonneg;
+    for (int64_t q : P->soc) {
+        for (int64_t k = 0; k < q; ++k) d[off + k] = Dl[off] / Dl[off + k];
+        off += q;
+    }
+    for (int64_t k = off; k < P->m; ++k) d[k] = 1.0 / Dl[k];
+    for (int64_t i = 0; i < P->m; ++i) d2[i] = d[i] * d[i];
+    HIPCHK(P->ctx, hipMemcpy(P->dsc, d.data(), sizeof(double) * P->m, hipMemcpyHostToDevice));
+    HIPCHK(P->ctx, hipMemcpy(P->dsc2, d2.data(), sizeof(double) * P->m, hipMemcpyHostToDevice));
+    return PDCS_OK;
+}
+
+KktData kkt_data(pdcs_problem *P) {
+    KktData D;
+    D.G = Csr<double>{P->d_rowptr, P->d_col, P->d_vals, P->m, P->lanesG};
+    D.Gt = Csr<double>{P->d_trowptr, P->d_tcol, P->d_tvals, P->n, P->lanesGt};
+    D.c_raw = P->c_raw; D.h_raw = P->h_raw; D.blf = P->blf; D.buf = P->buf; D.laml = P->laml; D.lamu = P->lamu;
+    D.Dr = P->Dr; D.Dl = P->Dl;
+    D.n = P->n; D.m = P->m; D.nb = P->nb;
+    D.K = P->K;
+    D.rtmp = P->rtmp;
+    return D;
+}
+
+// converge_info_calculation_diagonal! on the device; host assembles the ratios
+template <typename T>
+int conv_info(pdcs_problem *P, const T *x, const T *y, ConvInfo &ci) {
+    KktData D = kkt_data(P);
+    launch_kkt<T>(P->ctx->stream, D, x, y, P->kpart, P->kout);
+    HIPCHK(P->ctx, hipMemcpyAsync(P->h_kout, P->kout, sizeof(KktOut), hipMemcpyDeviceToHost, P->ctx->stream));
+    HIPCHK(P->ctx, hipStreamSynchronize(P->ctx->stream));
+    const KktOut &o = *P->h_kout;
+    ci.pobj = o.pobj;
+    ci.dobj = o.dobj;
+    ci.abs_gap = std::fabs(o.pobj - o.dobj);
+    ci.rel_gap = ci.abs_gap / (1 + std::max(std::fabs(o.pobj), std::fabs(o.dobj)));
+    ci.l2_abs_p = std::sqrt(o.p2);
+    ci.l2_rel_p = ci.l2_abs_p / (1 + std::max(o.sN1, std::max(P->hN1, o.axN1)));
+    ci.linf_abs_p = o.pInf;
+    ci.linf_rel_p = o.pInf / (1 + std::max(P->hNinf, std::max(o.axInf, o.projInf)));
+    ci.linf_rel_p_code = o.pInf / (1 + std::max(o.sInf, std::max(P->hNinf, o.axInf)));
+    ci.l2_abs_d = std::sqrt(o.d2);
+    ci.l2_rel_d = ci.l2_abs_d / (1 + std::max(P->cN1, o.atyN1));
+    ci.linf_abs_d = o.dInf;
+    ci.linf_rel_d = o.dInf / (1 + std::max(P->cNinf, o.atyInf));
+    return PDCS_OK;
+}
+
+void fill_result(const ConvInfo &c, pdcs_result *r) {
+    r->pobj = c.pobj; r->dobj = c.dobj;
+    r->rel_primal_res = c.linf_rel_p; r->rel_dual_res = c.linf_rel_d; r->rel_gap = c.rel_gap;
+    r->abs_primal_res = c.linf_abs_p; r->abs_dual_res = c.linf_abs_d; r->abs_gap = c.abs_gap;
+    r->l2_rel_primal_res = c.l2_rel_p; r->l2_rel_dual_res = c.l2_rel_d;
+    r->rel_primal_res_code = c.linf_rel_p_code;
+}
+
+// ---------------------------------------------------------------- solver --
+template <typename T>
+struct Solver {
+    pdcs_problem *P;
+    const pdcs_params *prm;
+    IterSet<T> &I;
+    hipStream_t s;
+    Csr<T> G, Gt;
+    ProbVec<T> PV;
+    int64_t n, m;
+    // host-side state mirrored from the reference's Solution/info
+    double omega = 1.0, tau = 0.9, sigma = 0.9, bs_t0 = 1.0;
+    double max_kkt = 1e30, min_kkt = 1e30;
+    bool restart_dg_flag = true;
+    double dg_threshold = 0.0, kkt_threshold = 0.0, kkt_err[2] = {0, 0};
+    int64_t restart_used = 0;
+    ConvInfo ci[2]{};
+    int exit_code = 7, exit_seq = 0;
+    double t_start = 0.0, time_limit = INFINITY, t_kkt = 0.0, t_restart = 0.0;
+    int64_t check_freq = 2000;
+    bool pessimistic = false;
+    int rc = PDCS_OK;
+
+    Solver(pdcs_problem *P_, const pdcs_params *p_) : P(P_), prm(p_), I(iters<T>(P_)), s(P_->ctx->stream) {
+        n = P->n;
+        m = P->m;
+        G = Csr<T>{P->d_rowptr, P->d_col, I.vals, m, P->lanesG};
+        Gt = Csr<T>{P->d_trowptr, P->d_tcol, I.tvals, n, P->lanesGt};
+        PV = ProbVec<T>{I.c, I.h, I.bl, I.bu, n, m, P->nb};
+    }
+
+    int sync() {
+        hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return fail(P->ctx, PDCS_ERR_HIP, "kernel failure: %s", hipGetErrorString(e));
+        return PDCS_OK;
+    }
+    int read_state() {
+        HIPCHK(P->ctx, hipMemcpyAsync(P->h_st, P->st, sizeof(StepState), hipMemcpyDeviceToHost, s));
+        return sync();
+    }
+    int write_state() {
+        HIPCHK(P->ctx, hipMemcpyAsync(P->st, P->h_st, sizeof(StepState), hipMemcpyHostToDevice, s));
+        return sync();
+    }
+    // dots of up to 4 pairs, returned on the host
+    template <int N>
+    int dots(const T *const (&a)[N], const T *const (&b)[N], int64_t len, double (&out)[N]) {
+        launch_dots<T>(s, N, a, b, len, P->dpart, P->dout);
+        HIPCHK(P->ctx, hipMemcpyAsync(P->h_dout, P->dout, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+        int r = sync();
+        for (int q = 0; q < N; ++q) out[q] = P->h_dout[q];
+        return r;
+    }
+    double nrm2(const T *a, int64_t len) {
+        const T *A[1] = {a};
+        double o[1];
+        dots<1>(A, A, len, o);
+        return std::sqrt(o[0]);
+    }
+    void proj_dual(T *y) { launch_project_dual<T>(s, P->K, P->dsc, P->dsc2, y); }
+
+    // Mnorm, rpdhg_alg_gpu_gen.jl:133-138
+    double Mnorm2(const T *x, const T *y, double tau_, double sigma_) {
+        launch_spmv<T>(s, G, x, I.tm2);
+        const T *A1[1] = {x};
+        double o1[1];
+        dots<1>(A1, A1, n, o1);
+        const T *A2[2] = {y, y};
+        const T *B2[2] = {I.tm2, y};
+        double o2[2];
+        dots<2>(A2, B2, m, o2);
+        return o1[0] / tau_ - 2 * o2[0] + o2[1] / sigma_;
+    }
+    // approximate_cal_diagonal! + diffs, rpdhg_alg_gpu_gen_scaling.jl:1-13,41-45
+    double bs_eval(double t, const T *primal, const T *dual, double tau_, double sigma_) {
+        launch_axpy_box<T>(s, I.tn1, primal, t * tau_ / 2, I.h1, I.bl, I.bu, n, P->nb);
+        launch_axpy<T>(s, I.tm1, dual, t * sigma_ / 2, I.h2, m);
+        proj_dual(I.tm1);
+        launch_sub<T>(s, I.tn2, I.tn1, primal, n);
+        launch_sub<T>(s, I.tm1, I.tm1, dual, m);   // tm1 := ys - dual
+        return Mnorm2(I.tn2, I.tm1, tau_, sigma_);
+    }
+    // binary_search_duality_gap_diagonal!, rpdhg_alg_gpu_gen_scaling.jl:19-117
+    double binary_search(double r, const T *primal, const T *dual, double tau_, double sigma_) {
+        double t = bs_t0, tR = t, tL = 0.0;
+        launch_spmv<T>(s, Gt, dual, I.h1);
+        launch_sub<T>(s, I.h1, I.h1, I.c, n);
+        launch_spmv<T>(s, G, primal, I.h2);
+        launch_sub<T>(s, I.h2, I.h, I.h2, m);
+        for (int k = 1; k <= 1000; ++k) {
+            double v = bs_eval(t, primal, dual, tau_, sigma_);
+            if (v > r && v < r + 1e-6) {
+                tR = t;
+                tL = std::max(t - 0.01, 0.0);
+                v = bs_eval(tL, primal, dual, tau_, sigma_);
+                if (v < r) break;
+            }
+            if (v < r && v > r - 1e-6) {
+                tL = t;
+                tR = t + 0.01;
+                v = bs_eval(tR, primal, dual, tau_, sigma_);
+                if (v > r) break;
+            }
+            if (v > r) {
+                if (k == 1) break;
+                if (prm->bisect_intent) { tR = t; tL = t / 2; }
+                break;
+            }
+            t *= 2;
+        }
+        double tM = (tR + tL) / 2;
+        int64_t count = 0;
+        while (tR - tL > 1e-6) {
+            tM = (tR + tL) / 2;
+            const double v = bs_eval(tM, primal, dual, tau_, sigma_);
+            if (v < r) tL = tM; else tR = tM;
+            if (++count > 1000000) break;
+        }
+        bs_t0 = tM;
+        // tn2 = xs - primal, tm1 = ys - dual from the last evaluation
+        const T *A1[1] = {I.h1};
+        const T *B1[1] = {I.tn2};
+        double o1[1];
+        dots<1>(A1, B1, n, o1);
+        const T *A2[1] = {I.h2};
+        const T *B2[1] = {I.tm1};
+        double o2[1];
+        dots<1>(A2, B2, m, o2);
+        return (o1[0] + o2[0]) / r;
+    }
+    static double kkt_error(double om, const ConvInfo &c) {
+        return std::sqrt(om * om * c.l2_abs_p * c.l2_abs_p + c.l2_abs_d * c.l2_abs_d / (om * om) + c.abs_gap * c.abs_gap);
+    }
+    static double max_l2(const ConvInfo &c) { return std::max(std::max(c.l2_rel_p, c.l2_rel_d), c.rel_gap); }
+
+    // exit_condition_check_diagonal! + check_termination_criteria
+    bool exit_check(const T *x, const T *y, int64_t iter) {
+        const double t0 = now_s();
+        rc = conv_info<T>(P, x, y, ci[0]);
+        if (!rc) rc = conv_info<T>(P, I.it.xm, I.it.ym, ci[1]);
+        t_kkt += now_s() - t0;
+        if (rc) return true;
+        if (prm->verbose)
+            fprintf(stderr, "[pdcs %s] iter %ld relp %.3e reld %.3e gap %.3e pobj %.10e dobj %.10e restarts %ld\n",
+                    sizeof(T) == 4 ? "fp32" : "fp64", (long)iter, ci[0].linf_rel_p, ci[0].linf_rel_d, ci[0].rel_gap,
+                    ci[0].pobj, ci[0].dobj, (long)restart_used);
+        for (int q = 0; q < 2; ++q) {
+            const ConvInfo &c = ci[q];
+            const double rp = prm->code_primal_denominator ? c.linf_rel_p_code : c.linf_rel_p;
+            const bool opt = std::max(rp, std::max(c.linf_rel_d, c.rel_gap)) < prm->rel_tol ||
+                             std::max(c.linf_abs_p, std::max(c.linf_abs_d, c.abs_gap)) < prm->abs_tol;
+            if (std::isnan(c.rel_gap)) { exit_code = 8; exit_seq = q; return true; }
+            if (std::isinf(c.rel_gap)) { exit_code = 9; exit_seq = q; return true; }
+            if (opt) { exit_code = 0; exit_seq = q; return true; }
+        }
+        if (iter >= prm->max_outer_iter * prm->max_inner_iter) { exit_code = 1; exit_seq = 0; return true; }
+        if (now_s() - t_start > time_limit) { exit_code = 6; exit_seq = 0; return true; }
+        return false;
+    }
+
+    // restart_threshold_calculation_diagonal!
+    void restart_threshold(bool dg) {
+        const double t0 = now_s();
+        if (dg) {
+            launch_sub<T>(s, I.xd, I.x0, I.x0l, n);
+            launch_sub<T>(s, I.yd, I.y0, I.y0l, m);
+            const double r = Mnorm2(I.xd, I.yd, tau, sigma);
+            dg_threshold = binary_search(r, I.x0, I.y0, tau, sigma);
+            if (dg_threshold < 0) restart_dg_flag = false;
+        } else {
+            rc = conv_info<T>(P, I.it.xs, I.it.ys, ci[0]);
+            kkt_threshold = kkt_error(omega, ci[0]);
+        }
+        t_restart += now_s() - t0;
+    }
+    // restart_condition_check_diagonal!
+    bool restart_check(int64_t inner, int64_t iter, bool dg) {
+        const double t0 = now_s();
+        bool res = false;
+        Iter<T> &it = I.it;
+        if (dg) {
+            launch_sub<T>(s, I.xd, it.xs, I.x0, n);
+            launch_sub<T>(s, I.yd, it.ys, I.y0, m);
+            const double r = Mnorm2(I.xd, I.yd, tau, sigma);
+            const double rho = binary_search(r, it.xs, it.ys, tau, sigma);
+            const double rho_m = binary_search(r, it.xm, it.ym, tau, sigma);
+            for (int cond = 0; cond < 2 && !res; ++cond) {
+                const double beta = cond == 0 ? 0.4 : 0.8;
+                if (rho < beta * dg_threshold || rho_m < beta * dg_threshold) {
+                    restart_used++;
+                    if (rho < rho_m && restart_dg_flag) {
+                        launch_copy<T>(s, it.xm, it.xs, n);
+                        launch_copy<T>(s, it.ym, it.ys, m);
+                    }
+                    res = true;
+                }
+            }
+        } else {
+            rc = conv_info<T>(P, it.xs, it.ys, ci[0]);
+            if (!rc) rc = conv_info<T>(P, it.xm, it.ym, ci[1]);
+            const double lag0 = kkt_err[0], lag1 = kkt_err[1];
+            const double e0 = kkt_error(omega, ci[0]), e1 = kkt_error(omega, ci[1]);
+            kkt_err[0] = e0;
+            kkt_err[1] = e1;
+            const bool c1 = e0 < 0.4 * kkt_threshold || e1 < 0.4 * kkt_threshold;
+            const bool c2 = (e0 < 0.8 * kkt_threshold && e0 > lag0) || (e1 < 0.8 * kkt_threshold && e1 > lag1);
+            if (c1 || c2) {
+                restart_used++;
+                if (e0 < e1) {
+                    launch_copy<T>(s, it.xm, it.xs, n);
+                    launch_copy<T>(s, it.ym, it.ys, m);
+                }
+                res = true;
+            }
+        }
+        if (!res && inner >= iter * 0.223 && iter > 100000) { restart_used++; res = true; }
+        t_restart += now_s() - t0;
+        return res;
+    }
+    double init_primal_weight() {
+        const double hn = nrm2(I.h, m), cn = nrm2(I.c, n);
+        return (hn > 1e-10 && cn > 1e-10) ? cn / hn : 1.0;
+    }
+    double primal_weight_update(double om) {
+        const double xn = nrm2(I.xd, n), yn = nrm2(I.yd, m);
+        const int64_t ru = restart_used;
+        if (om > 10000 || om < 1.0 / 10000)
+            if (ru % 20 == 0 && ru != 0) return init_primal_weight();
+        if (om > 30000 || om < 1.0 / 30000)
+            if (ru % 10 == 0 && ru != 0) return init_primal_weight();
+        if (xn > 1e-10 && yn > 1e-10) return std::exp(0.5 * std::log(yn / xn) + 0.5 * std::log(om));
+        return om;
+    }
+
+    // graph of `len` attempt slots
+    hipGraphExec_t make_graph(int len) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+        for (int k = 0; k < len; ++k)
+            launch_attempt<T>(s, k, G, Gt, PV, P->K, P->dsc2, I.it, P->st, P->apart, P->apart_stride);
+        if (hipStreamEndCapture(s, &g) != hipSuccess) return nullptr;
+        if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) ge = nullptr;
+        hipGraphDestroy(g);
+        return ge;
+    }
+    std::vector<hipGraphExec_t> &graph_cache();
+
+    // enqueue `len` attempts starting at slot 0; stop_at reset to len first
+    int run_chunk(int len) {
+        P->h_st->stop_at = (int64_t)1 << 62;
+        HIPCHK(P->ctx, hipMemcpyAsync(&P->st->stop_at, &P->h_st->stop_at, sizeof(int64_t), hipMemcpyHostToDevice, s));
+        if (prm->use_graph && len == P->graph_chunk) {
+            auto &gc = graph_cache();
+            if (gc.empty()) {
+                hipGraphExec_t ge = make_graph(len);
+                if (!ge) return fail(P->ctx, PDCS_ERR_HIP, "graph capture failed");
+                gc.push_back(ge);
+            }
+            HIPCHK(P->ctx, hipGraphLaunch(gc[0], s));
+        } else {
+            for (int k = 0; k < len; ++k)
+                launch_attempt<T>(s, k, G, Gt, PV, P->K, P->dsc2, I.it, P->st, P->apart, P->apart_stride);
+        }
+        HIPCHK(P->ctx, hipGetLastError());
+        return read_state();
+    }
+
+    // the main loop; `iter0` attempts already done (polish); returns PDCS_OK
+    int run(int64_t iter_base, int64_t max_iter_total, double stop_tol, int64_t cfreq) {
+        check_freq = cfreq;
+        StepState &h = *P->h_st;
+        // device state init
+        std::memset(&h, 0, sizeof(h));
+        h.omega = omega;
+        h.check_freq = check_freq;
+        h.restart_freq = prm->use_duality_gap_restart ? prm->duality_gap_restart_freq : 0;
+        h.kkt_restart_freq = prm->use_kkt_restart ? prm->kkt_restart_freq : 0;
+        h.stop_at = (int64_t)1 << 62;
+        h.iter = iter_base;
+        h.iter_stepsize = iter_base;
+        h.reflect_slot = -1;
+        // initial eta' = 1 / ||G||_inf (max |entry| of the scaled matrix), scaling.jl:782-783
+        h.eta = std::max(eta_init, 1e-6);
+        h.dec = h.dec_next = std::min(0.9 + 0.1 * std::log10(max_kkt * 1e5), 1.0);
+        if ((rc = write_state())) return rc;
+        Iter<T> &it = I.it;
+        bool done = false;
+        for (int64_t outer = 1; outer <= prm->max_outer_iter && !done; ++outer) {
+            max_kkt = max_l2(ci[0]);
+            min_kkt = std::min(std::min(std::min(ci[0].l2_rel_p, ci[0].l2_rel_d), ci[0].rel_gap), min_kkt);
+            const double extra = -0.1 * std::log10(max_kkt) + 0.2;
+            launch_copy<T>(s, I.x0l, I.x0, n); launch_copy<T>(s, I.y0l, I.y0, m);
+            launch_copy<T>(s, I.x0, it.xm, n); launch_copy<T>(s, I.y0, it.ym, m);
+            launch_copy<T>(s, it.xs, it.xm, n); launch_copy<T>(s, it.ys, it.ym, m);
+            if (prm->use_kkt_restart) restart_threshold(false);
+            else restart_threshold(restart_dg_flag);
+            if (rc) return rc;
+            {
+                const bool f1 = outer > 10 && max_kkt > 1e5;
+                const double ratio = ci[0].l2_rel_p / ci[0].l2_rel_d;
+                const bool f2 = outer > 80 && (ratio > 1e10 || ratio < 1e-10);
+                const bool f3 = outer > 10 && tau < 1e-6 && sigma < 1e-6;
+                if (f1 || f2 || f3) pessimistic = true;
+            }
+            // epoch start: xd/yd hold the epoch's starting point
+            launch_copy<T>(s, I.xd, it.xs, n); launch_copy<T>(s, I.yd, it.ys, m);
+            if ((rc = read_state())) return rc;
+            h.omega = omega;
+            h.extra = extra;
+            h.inner = 0;
+            h.eta_cum = 0.0;
+            h.dec_next = std::min(0.9 + 0.1 * std::log10(max_kkt * 1e5), 1.0);
+            h.dec = h.dec_next;   // a new series starts with the epoch
+            if ((rc = write_state())) return rc;
+            for (;;) {
+                // attempts until the next residual check
+                const int64_t to_check = check_freq - (h.iter % check_freq);
+                const int len = (int)std::min<int64_t>(to_check, P->graph_chunk);
+                if ((rc = run_chunk(len))) return rc;
+                tau = h.tau;
+                sigma = h.sigma;
+                if (h.stop_at > len) continue;   // no halt in this chunk
+                int reason = h.reason;
+                const int slot = (int)h.stop_at - 1;
+                if (reason == 1) {
+                    if (exit_check(it.x, it.y, h.iter)) { done = true; break; }
+                    if (rc) return rc;
+                    max_kkt = max_l2(ci[0]);
+                    h.dec_next = std::min(0.9 + 0.1 * std::log10(max_kkt * 1e5), 1.0);
+                    if (stop_tol > 0 && std::max(ci[0].linf_rel_p, std::max(ci[0].linf_rel_d, ci[0].rel_gap)) < stop_tol) {
+                        exit_code = 10;   // phase switch (mixed precision)
+                        done = true;
+                        break;
+                    }
+                    if (h.iter >= max_iter_total) { exit_code = 1; done = true; break; }
+                    if ((rc = write_state())) return rc;
+                    launch_continue(s, slot, P->st);
+                    launch_reflect<T>(s, slot, it, n, m, P->st);
+                    if ((rc = read_state())) return rc;
+                    if (h.stop_at > slot + 1 || h.reason == 1) continue;  // rejected or no further halt
+                    reason = h.reason;
+                }
+                if (reason == 4 || reason == 8) { restart_used++; break; }
+                if (reason == 2) {
+                    bool brk = false;
+                    if (prm->use_kkt_restart && prm->kkt_restart_freq > 0 && h.iter % prm->kkt_restart_freq == 0) {
+                        if (restart_check(h.inner, h.iter, false)) brk = true;
+                        else if (outer == 1) brk = true;
+                    }
+                    if (!brk && prm->use_duality_gap_restart && h.iter % prm->duality_gap_restart_freq == 0) {
+                        if (restart_check(h.inner, h.iter, restart_dg_flag)) brk = true;
+                        else if (outer == 1) brk = true;
+                    }
+                    if (rc) return rc;
+                    if (brk) break;
+                    // continue the epoch
+                    if ((rc = read_state())) return rc;
+                    h.reason = 0;
+                    if ((rc = write_state())) return rc;
+                }
+            }
+            if (done) break;
+            // xd/yd := epoch start - current
+            launch_sub<T>(s, I.xd, I.xd, it.xs, n);
+            launch_sub<T>(s, I.yd, I.yd, it.ys, m);
+            omega = primal_weight_update(omega);
+            if ((rc = read_state())) return rc;
+            h.omega = omega;
+            h.reason = 0;
+            if ((rc = write_state())) return rc;
+        }
+        if (!done) exit_code = 1;
+        return rc;
+    }
+    double eta_init = 1.0;
+};
+
+template <>
+std::vector<hipGraphExec_t> &Solver<double>::graph_cache() { return P->graphs64; }
+template <>
+std::vector<hipGraphExec_t> &Solver<float>::graph_cache() { return P->graphs32; }
+
+template <typename T>
+int init_iterates(pdcs_problem *P, const pdcs_params *prm, const double *x0, const double *y0) {
+    IterSet<T> &I = iters<T>(P);
+    hipStream_t s = P->ctx->stream;
+    const int64_t n = P->n, m = P->m;
+    std::vector<T> hx(n, (T)0), hy(m, (T)0);
+    if (prm->warm_start && x0 && y0) {
+        std::vector<double> Dr(n), Dl(m);
+        HIPCHK(P->ctx, hipMemcpy(Dr.data(), P->Dr, sizeof(double) * n, hipMemcpyDeviceToHost));
+        HIPCHK(P->ctx, hipMemcpy(Dl.data(), P->Dl, sizeof(double) * m, hipMemcpyDeviceToHost));
+        for (int64_t j = 0; j < n; ++j) hx[j] = (T)(x0[j] * Dr[j]);
+        for (int64_t i = 0; i < m; ++i) hy[i] = (T)(y0[i] * Dl[i]);
+    }
+    T *xs[] = {I.it.x, I.it.xs, I.it.xm, I.x0, I.x0l};
+    T *ys[] = {I.it.y, I.it.ys, I.it.ym, I.y0, I.y0l};
+    for (T *p : xs) HIPCHK(P->ctx, hipMemcpyAsync(p, hx.data(), sizeof(T) * n, hipMemcpyHostToDevice, s));
+    for (T *p : ys) HIPCHK(P->ctx, hipMemcpyAsync(p, hy.data(), sizeof(T) * m, hipMemcpyHostToDevice, s));
+    HIPCHK(P->ctx, hipStreamSynchronize(s));
+    return PDCS_OK;
+}
+
+// copy every iterate vector of the FP32 set into the FP64 set (polish start)
+int promote_iterates(pdcs_problem *P) {
+    hipStream_t s = P->ctx->stream;
+    IterSet<float> &A = P->f32;
+    IterSet<double> &B = P->f64;
+    const int64_t n = P->n, m = P->m;
+    std::pair<float *, double *> xv[] = {{A.it.x, B.it.x}, {A.it.xs, B.it.xs}, {A.it.xm, B.it.xm}, {A.x0, B.x0}, {A.x0l, B.x0l}};
+    std::pair<float *, double *> yv[] = {{A.it.y, B.it.y}, {A.it.ys, B.it.ys}, {A.it.ym, B.it.ym}, {A.y0, B.y0}, {A.y0l, B.y0l}};
+    for (auto &p : xv) launch_convert<double, float>(s, p.second, p.first, n);
+    for (auto &p : yv) launch_convert<double, float>(s, p.second, p.first, m);
+    HIPCHK(P->ctx, hipStreamSynchronize(s));
+    return PDCS_OK;
+}
+
+template <typename T>
+int store_solution(pdcs_problem *P, const T *x, const T *y) {
+    const int64_t n = P->n, m = P->m;
+    std::vector<T> hx(n), hy(m);
+    std::vector<double> Dr(n), Dl(m);
+    HIPCHK(P->ctx, hipMemcpy(hx.data(), x, sizeof(T) * n, hipMemcpyDeviceToHost));
+    HIPCHK(P->ctx, hipMemcpy(hy.data(), y, sizeof(T) * m, hipMemcpyDeviceToHost));
+    HIPCHK(P->ctx, hipMemcpy(Dr.data(), P->Dr, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPCHK(P->ctx, hipMemcpy(Dl.data(), P->Dl, sizeof(double) * m, hipMemcpyDeviceToHost));
+    P->sol_x.resize(n);
+    P->sol_y.resize(m);
+    for (int64_t j = 0; j < n; ++j) P->sol_x[j] = (double)hx[j] / Dr[j];
+    for (int64_t i = 0; i < m; ++i) P->sol_y[i] = (double)hy[i] / Dl[i];
+    P->solved = true;
+    return PDCS_OK;
+}
+
+double max_abs_vals(pdcs_problem *P) {
+    std::vector<double> v(P->nnz);
+    hipMemcpy(v.data(), P->d_vals, sizeof(double) * P->nnz, hipMemcpyDeviceToHost);
+    double g = 0.0;
+    for (double a : v) g = std::max(g, std::fabs(a));
+    return g;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pdcs_problem_create_host(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double *c, const int32_t *rowptr,
+                             const int32_t *colidx, const double *vals, const double *h, const double *bl,
+                             const double *bu, const pdcs_cones *cones, pdcs_problem **out) {
+    if (!ctx) return fail(nullptr, PDCS_ERR_ARG, "ctx is NULL");
+    int rc = build_problem(ctx, n, m, nb, c, rowptr, colidx, vals, h, bl, bu, cones, out);
+    if (rc) return rc;
+    pdcs_problem *P = *out;
+    if ((rc = scale_device(P)) || (rc = finish_scaling(P))) {
+        delete P;
+        *out = nullptr;
+        return rc;
+    }
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        delete P;
+        *out = nullptr;
+        return fail(ctx, PDCS_ERR_HIP, "scaling failed");
+    }
+    return PDCS_OK;
+}
+
+int pdcs_problem_create_device(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double *c, const int32_t *rowptr,
+                               const int32_t *colidx, const double *vals, const double *h, const double *bl,
+                               const double *bu, const pdcs_cones *cones, pdcs_problem **out) {
+    if (!ctx || !rowptr) return fail(ctx, PDCS_ERR_ARG, "NULL argument");
+    // stage to the host: the transpose and the cone layout are built there
+    std::vector<int32_t> hr(m + 1);
+    if (hipMemcpy(hr.data(), rowptr, sizeof(int32_t) * (m + 1), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(ctx, PDCS_ERR_ARG, "rowptr is not a device pointer");
+    const int64_t nnz = hr[m];
+    std::vector<int32_t> hc(nnz);
+    std::vector<double> hv(nnz), hcv(n), hh(m), hbl(nb), hbu(nb);
+    HIPCHK(ctx, hipMemcpy(hc.data(), colidx, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(hv.data(), vals, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(hcv.data(), c, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(hh.data(), h, sizeof(double) * m, hipMemcpyDeviceToHost));
+    if (nb > 0) {
+        HIPCHK(ctx, hipMemcpy(hbl.data(), bl, sizeof(double) * nb, hipMemcpyDeviceToHost));
+        HIPCHK(ctx, hipMemcpy(hbu.data(), bu, sizeof(double) * nb, hipMemcpyDeviceToHost));
+    }
+    return pdcs_problem_create_host(ctx, n, m, nb, hcv.data(), hr.data(), hc.data(), hv.data(), hh.data(), hbl.data(),
+                                    hbu.data(), cones, out);
+}
+
+int pdcs_problem_destroy(pdcs_problem *p) {
+    if (!p) return PDCS_OK;
+    hipSetDevice(p->ctx->device);
+    hipStreamSynchronize(p->ctx->stream);
+    delete p;
+    return PDCS_OK;
+}
+
+int pdcs_solve(pdcs_problem *P, const pdcs_params *prm_in, const double *x0, const double *y0, pdcs_result *res) {
+    if (!P || !res) return fail(P ? P->ctx : nullptr, PDCS_ERR_ARG, "NULL argument");
+    pdcs_params prm;
+    if (prm_in) prm = *prm_in; else pdcs_params_default(&prm);
+    if (prm.check_terminate_freq <= 0) return fail(P->ctx, PDCS_ERR_ARG, "check_terminate_freq must be positive");
+    if (prm.use_preconditioner != 1) return fail(P->ctx, PDCS_ERR_UNSUPPORTED, "only the preconditioned path is built");
+    pdcs_ctx *ctx = P->ctx;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    std::memset(res, 0, sizeof(*res));
+    const double t_setup = now_s();
+    int rc;
+    // attempt partial buffers (per-block sums of the attempt chain)
+    if (!P->apart) {
+        const int64_t bp = (P->n * P->lanesGt + kBlock - 1) / kBlock;
+        const int64_t bd = (P->m * P->lanesG + kBlock - 1) / kBlock +
+                           (P->K.nsoc_small * (int64_t)P->K.soc_small_group + kBlock - 1) / kBlock + P->K.nsoc_big + 8;
+        P->apart_stride = std::max(bp, bd) + 64;
+        P->apart = P->mem.alloc<double>(3 * P->apart_stride);
+        if (!P->apart) return fail(ctx, PDCS_ERR_HIP, "alloc failed");
+        P->graph_chunk = 64;
+    }
+    const bool mixed = prm.precision == PDCS_MIXED;
+    if ((rc = alloc_iters<double>(P))) return rc;
+    if (mixed && (rc = alloc_iters<float>(P))) return rc;
+    const double ginf = max_abs_vals(P);
+    res->time_setup = now_s() - t_setup;
+    const double t0 = now_s();
+    int64_t iters_total = 0;
+    const int64_t max_total = prm.max_outer_iter * prm.max_inner_iter;
+    Solver<double> S64(P, &prm);
+    S64.t_start = t0;
+    S64.time_limit = prm.time_limit;
+    S64.eta_init = 1.0 / ginf;
+    int64_t polish_base = 0;
+    if (mixed) {
+        if ((rc = init_iterates<float>(P, &prm, x0, y0))) return rc;
+        Solver<float> S32(P, &prm);
+        S32.t_start = t0;
+        S32.time_limit = prm.time_limit;
+        S32.eta_init = 1.0 / ginf;
+        if ((rc = conv_info<float>(P, P->f32.it.xs, P->f32.it.ys, S32.ci[0]))) return rc;
+        S32.omega = S32.init_primal_weight();
+        const int64_t cf = prm.mixed_check_freq > 0 ? prm.mixed_check_freq : prm.check_terminate_freq;
+        if ((rc = S32.run(0, max_total, prm.mixed_switch_tol, cf))) return rc;
+        iters_total = P->h_st->iter;
+        res->iter_fp32 = iters_total;
+        res->time_fp32 = now_s() - t0;
+        res->time_kkt += S32.t_kkt;
+        res->time_restart += S32.t_restart;
+        if (S32.exit_code != 10 || !prm.polish) {
+            // finished in FP32 (optimal or limit) or no polish requested
+            const ConvInfo &c = S32.ci[S32.exit_seq];
+            fill_result(c, res);
+            res->exit_code = S32.exit_code == 10 ? 0 : S32.exit_code;
+            res->sequence = S32.exit_seq;
+            res->iter = iters_total;
+            res->restarts = S32.restart_used;
+            res->omega = S32.omega;
+            res->time_solve = now_s() - t0;
+            store_solution<float>(P, S32.exit_seq ? P->f32.it.xm : P->f32.it.x, S32.exit_seq ? P->f32.it.ym : P->f32.it.y);
+            return PDCS_OK;
+        }
+        // FP64 polish: restart from the FP32 iterate
+        if ((rc = promote_iterates(P))) return rc;
+        S64.omega = S32.omega;
+        S64.ci[0] = S32.ci[0];
+        S64.ci[1] = S32.ci[1];
+        S64.restart_used = S32.restart_used;
+        S64.bs_t0 = S32.bs_t0;
+        S64.eta_init = P->h_st->eta;
+        S64.tau = S32.tau;
+        S64.sigma = S32.sigma;
+        // the polish starts a fresh epoch from the better of current / average
+        if (S32.exit_seq == 0) {
+            launch_copy<double>(ctx->stream, P->f64.it.xm, P->f64.it.x, P->n);
+            launch_copy<double>(ctx->stream, P->f64.it.ym, P->f64.it.y, P->m);
+        }
+        polish_base = iters_total;
+    } else {
+        if ((rc = init_iterates<double>(P, &prm, x0, y0))) return rc;
+        if ((rc = conv_info<double>(P, P->f64.it.xs, P->f64.it.ys, S64.ci[0]))) return rc;
+        S64.omega = S64.init_primal_weight();
+    }
+    const int64_t cf64 = (mixed && prm.polish_check_freq > 0) ? prm.polish_check_freq : prm.check_terminate_freq;
+    if ((rc = S64.run(polish_base, max_total, 0.0, cf64))) return rc;
+    const ConvInfo &c = S64.ci[S64.exit_seq];
+    fill_result(c, res);
+    res->exit_code = S64.exit_code;
+    res->sequence = S64.exit_seq;
+    res->iter = P->h_st->iter;
+    res->restarts = S64.restart_used;
+    res->omega = S64.omega;
+    res->time_solve = now_s() - t0;
+    res->time_kkt += S64.t_kkt;
+    res->time_restart += S64.t_restart;
+    store_solution<double>(P, S64.exit_seq ? P->f64.it.xm : P->f64.it.x, S64.exit_seq ? P->f64.it.ym : P->f64.it.y);
+    return PDCS_OK;
+}
+
+int pdcs_get_solution(pdcs_problem *P, double *x, double *y, double *s) {
+    if (!P) return fail(nullptr, PDCS_ERR_ARG, "problem is NULL");
+    if (!P->solved) return fail(P->ctx, PDCS_ERR_STATE, "no solution: call pdcs_solve first");
+    if (x) std::memcpy(x, P->sol_x.data(), sizeof(double) * P->n);
+    if (y) std::memcpy(y, P->sol_y.data(), sizeof(double) * P->m);
+    if (s) {
+        // s = c - G'y on the original data
+        for (int64_t j = 0; j < P->n; ++j) s[j] = P->c[j];
+        for (int64_t i = 0; i < P->m; ++i)
+            for (int32_t k = P->rowptr[i]; k < P->rowptr[i + 1]; ++k) s[P->col[k]] -= P->vals[k] * P->sol_y[i];
+    }
+    return PDCS_OK;
+}
+
+int pdcs_get_scaling(pdcs_problem *P, double *Dr, double *Dl) {
+    if (!P) return fail(nullptr, PDCS_ERR_ARG, "problem is NULL");
+    if (Dr) HIPCHK(P->ctx, hipMemcpy(Dr, P->Dr, sizeof(double) * P->n, hipMemcpyDeviceToHost));
+    if (Dl) HIPCHK(P->ctx, hipMemcpy(Dl, P->Dl, sizeof(double) * P->m, hipMemcpyDeviceToHost));
+    return PDCS_OK;
+}
+
+int pdcs_evaluate(pdcs_problem *P, const double *x, const double *y, pdcs_result *res) {
+    if (!P || !x || !y || !res) return fail(P ? P->ctx : nullptr, PDCS_ERR_ARG, "NULL argument");
+    int rc = alloc_iters<double>(P);
+    if (rc) return rc;
+    std::vector<double> Dr(P->n), Dl(P->m), xs(P->n), ys(P->m);
+    HIPCHK(P->ctx, hipMemcpy(Dr.data(), P->Dr, sizeof(double) * P->n, hipMemcpyDeviceToHost));
+    HIPCHK(P->ctx, hipMemcpy(Dl.data(), P->Dl, sizeof(double) * P->m, hipMemcpyDeviceToHost));
+    for (int64_t j = 0; j < P->n; ++j) xs[j] = x[j] * Dr[j];
+    for (int64_t i = 0; i < P->m; ++i) ys[i] = y[i] * Dl[i];
+    HIPCHK(P->ctx, hipMemcpy(P->f64.tn1, xs.data(), sizeof(double) * P->n, hipMemcpyHostToDevice));
+    HIPCHK(P->ctx, hipMemcpy(P->f64.tm1, ys.data(), sizeof(double) * P->m, hipMemcpyHostToDevice));
+    ConvInfo c{};
+    if ((rc = conv_info<double>(P, P->f64.tn1, P->f64.tm1, c))) return rc;
+    std::memset(res, 0, sizeof(*res));
+    fill_result(c, res);
+    return PDCS_OK;
+}
+
+int pdcs_spmv(pdcs_ctx *ctx, int32_t precision, int64_t m, int64_t n, const int32_t *rowptr, const int32_t *colidx,
+              const void *vals, const void *x, void *y) {
+    if (!ctx || !rowptr || !colidx || !vals || !x || !y) return fail(ctx, PDCS_ERR_ARG, "NULL argument");
+    int32_t nnz = 0;
+    HIPCHK(ctx, hipMemcpy(&nnz, rowptr + m, sizeof(int32_t), hipMemcpyDeviceToHost));
+    const int L = lanes_for(nnz, m);
+    if (precision == PDCS_FP64)
+        launch_spmv<double>(ctx->stream, Csr<double>{rowptr, colidx, (const double *)vals, m, L}, (const double *)x, (double *)y);
+    else
+        launch_spmv<float>(ctx->stream, Csr<float>{rowptr, colidx, (const float *)vals, m, L}, (const float *)x, (float *)y);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    (void)n;
+    return PDCS_OK;
+}
+
+int pdcs_project_dual(pdcs_ctx *ctx, int32_t precision, const pdcs_cones *cones, int64_t m, void *y, const double *d) {
+    if (!ctx || !cones || !y || !d) return fail(ctx, PDCS_ERR_ARG, "NULL argument");
+    int rc = validate_cones(ctx, 1, m, 0, cones);
+    if (rc) return rc;
+    DevBuf M;
+    std::vector<int32_t> ss_start, ss_dim, sb_start, sb_dim;
+    int64_t off = cones->mzero + cones->mnonneg;
+    int maxsmall = 0;
+    for (int64_t b = 0; b < cones->nsoc; ++b) {
+        const int64_t q = cones->soc[b];
+        if (q <= 1024) { ss_start.push_back((int32_t)off); ss_dim.push_back((int32_t)q); maxsmall = std::max<int>(maxsmall, (int)q); }
+        else { sb_start.push_back((int32_t)off); sb_dim.push_back((int32_t)q); }
+        off += q;
+    }
+    ConeLayout K{};
+    K.m = m; K.mzero = cones->mzero; K.mnonneg = cones->mnonneg; K.cone_begin = cones->mzero + cones->mnonneg;
+    K.nsoc_small = (int32_t)ss_start.size(); K.nsoc_big = (int32_t)sb_start.size();
+    int g = 8;
+    while (g < 64 && g < maxsmall - 1) g <<= 1;
+    K.soc_small_group = g;
+    int32_t *a = M.alloc<int32_t>((int64_t)ss_start.size()), *b = M.alloc<int32_t>((int64_t)ss_dim.size());
+    int32_t *c = M.alloc<int32_t>((int64_t)sb_start.size()), *e = M.alloc<int32_t>((int64_t)sb_dim.size());
+    double *d2 = M.alloc<double>(m);
+    if (!a || !b || !c || !e || !d2) return fail(ctx, PDCS_ERR_HIP, "alloc failed");
+    if (!ss_start.empty()) { HIPCHK(ctx, hipMemcpy(a, ss_start.data(), 4 * ss_start.size(), hipMemcpyHostToDevice)); HIPCHK(ctx, hipMemcpy(b, ss_dim.data(), 4 * ss_dim.size(), hipMemcpyHostToDevice)); }
+    if (!sb_start.empty()) { HIPCHK(ctx, hipMemcpy(c, sb_start.data(), 4 * sb_start.size(), hipMemcpyHostToDevice)); HIPCHK(ctx, hipMemcpy(e, sb_dim.data(), 4 * sb_dim.size(), hipMemcpyHostToDevice)); }
+    K.soc_small_start = a; K.soc_small_dim = b; K.soc_big_start = c; K.soc_big_dim = e;
+    std::vector<double> hd(m);
+    HIPCHK(ctx, hipMemcpy(hd.data(), d, sizeof(double) * m, hipMemcpyDeviceToHost));
+    for (auto &v : hd) v = v * v;
+    HIPCHK(ctx, hipMemcpy(d2, hd.data(), sizeof(double) * m, hipMemcpyHostToDevice));
+    if (precision == PDCS_FP64) launch_project_dual<double>(ctx->stream, K, d, d2, (double *)y);
+    else launch_project_dual<float>(ctx->stream, K, d, d2, (float *)y);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return PDCS_OK;
+}
+
+int pdcs_project_primal_cone(pdcs_ctx *ctx, const pdcs_cones *cones, int64_t m, double *r) {
+    if (!ctx || !cones || !r) return fail(ctx, PDCS_ERR_ARG, "NULL argument");
+    int rc = validate_cones(ctx, 1, m, 0, cones);
+    if (rc) return rc;
+    DevBuf M;
+    std::vector<int32_t> st, dm;
+    int64_t off = cones->mzero + cones->mnonneg;
+    for (int64_t b = 0; b < cones->nsoc; ++b) { st.push_back((int32_t)off); dm.push_back((int32_t)cones->soc[b]); off += cones->soc[b]; }
+    ConeLayout K{};
+    K.m = m; K.mzero = cones->mzero; K.mnonneg = cones->mnonneg; K.cone_begin = cones->mzero + cones->mnonneg;
+    K.nsoc_small = (int32_t)st.size();
+    int32_t *a = M.alloc<int32_t>((int64_t)st.size()), *b = M.alloc<int32_t>((int64_t)dm.size());
+    if (!a || !b) return fail(ctx, PDCS_ERR_HIP, "alloc failed");
+    if (!st.empty()) { HIPCHK(ctx, hipMemcpy(a, st.data(), 4 * st.size(), hipMemcpyHostToDevice)); HIPCHK(ctx, hipMemcpy(b, dm.data(), 4 * dm.size(), hipMemcpyHostToDevice)); }
+    K.soc_small_start = a; K.soc_small_dim = b;
+    launch_project_primal_cone(ctx->stream, K, r);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return PDCS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,182 @@
+"""Seeded synthetic conic programs for the five benchmark workloads.
+
+Every generator returns a :class:`ConicProblem` in the layout the PDCS solve
+entry points take (``rpdhg_gpu_solve``, src/pdcs_gpu/rpdhg_alg_gpu_gen.jl:1219):
+
+    min c'x  s.t.  G x - h in K,  bl <= x[:nb] <= bu
+
+with K = {0}^mzero x R+^mnonneg x SOC(soc[0]) x ... x Kexp^nexp x Kexp*^ndexp,
+rows in that order (recover_soc_cone_rotated_exp_cone_indices,
+rpdhg_alg_gpu_gen.jl:60-118).  G is CSR, 0-based, int32 indices, float64 values.
+
+Seeds follow the convention seed = 1000 + workload index; the random stream is
+numpy's PCG64 (``np.random.default_rng``).  The instance recipes follow the
+reference's own test scripts (a planted feasible point whose cone slack is
+strictly interior, test/test_soc.jl:18-26, test/test_exp.jl:18-28).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class ConicProblem:
+    n: int
+    m: int
+    nb: int
+    c: np.ndarray
+    rowptr: np.ndarray
+    colidx: np.ndarray
+    vals: np.ndarray
+    h: np.ndarray
+    bl: np.ndarray
+    bu: np.ndarray
+    mzero: int = 0
+    mnonneg: int = 0
+    soc: list = field(default_factory=list)
+    nexp: int = 0
+    ndexp: int = 0
+    name: str = ""
+
+    @property
+    def nnz(self) -> int:
+        return int(self.rowptr[-1])
+
+    def dense(self) -> np.ndarray:
+        A = np.zeros((self.m, self.n))
+        for i in range(self.m):
+            s, e = self.rowptr[i], self.rowptr[i + 1]
+            A[i, self.colidx[s:e]] += self.vals[s:e]
+        return A
+
+    def matvec(self, x: np.ndarray) -> np.ndarray:
+        rows = np.repeat(np.arange(self.m), np.diff(self.rowptr))
+        return np.bincount(rows, weights=self.vals * x[self.colidx], minlength=self.m)
+
+    def rmatvec(self, y: np.ndarray) -> np.ndarray:
+        rows = np.repeat(np.arange(self.m), np.diff(self.rowptr))
+        return np.bincount(self.colidx, weights=self.vals * y[rows], minlength=self.n)
+
+
+def csr_from_coo(m: int, rows: np.ndarray, cols: np.ndarray, vals: np.ndarray):
+    """Row-major CSR (sorted by row, then column; duplicates summed)."""
+    order = np.lexsort((cols, rows))
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    if len(rows) > 1:
+        dup = (rows[1:] == rows[:-1]) & (cols[1:] == cols[:-1])
+        if dup.any():
+            key = rows.astype(np.int64) * (int(cols.max()) + 1) + cols
+            uk, inv = np.unique(key, return_inverse=True)
+            v = np.zeros(len(uk))
+            np.add.at(v, inv, vals)
+            rows = (uk // (int(cols.max()) + 1)).astype(np.int64)
+            cols = (uk % (int(cols.max()) + 1)).astype(np.int64)
+            vals = v
+    rowptr = np.zeros(m + 1, dtype=np.int32)
+    np.add.at(rowptr, rows + 1, 1)
+    rowptr = np.cumsum(rowptr).astype(np.int32)
+    return rowptr, cols.astype(np.int32), np.ascontiguousarray(vals, dtype=np.float64)
+
+
+def random_pattern(rng, m: int, n: int, nnz: int):
+    lin = rng.choice(m * n, size=nnz, replace=False)
+    return (lin // n).astype(np.int64), (lin % n).astype(np.int64)
+
+
+def _soc_interior(rng, dims):
+    out = []
+    for q in dims:
+        tail = rng.standard_normal(q - 1)
+        head = np.linalg.norm(tail) + rng.uniform(0.1, 1.0)
+        out.append(np.concatenate([[head], tail]))
+    return np.concatenate(out)
+
+
+def soc_program(n: int, m: int, density: float, cone_dim: int, seed: int, box: float = 10.0,
+                name: str = "soc") -> ConicProblem:
+    """SOC program with a box-bounded primal (workload 0 and, per layer, workload 4).
+
+    G (m x n) has a uniform pattern of round(density*m*n) entries with N(0,1)
+    values; K_d is m/cone_dim SOC blocks.  x0 ~ U(-5,5); s0 strictly inside
+    K_d (tail ~ N(0,1), head = ||tail|| + U(0.1,1)); h = G x0 - s0; y0 in K_d*
+    built the same way; c = G' y0 + N(0, 0.1^2).
+    """
+    assert m % cone_dim == 0
+    rng = np.random.default_rng(seed)
+    nnz = int(round(density * m * n))
+    r, cidx = random_pattern(rng, m, n, nnz)
+    v = rng.standard_normal(nnz)
+    rowptr, colidx, vals = csr_from_coo(m, r, cidx, v)
+    x0 = rng.uniform(-5.0, 5.0, n)
+    dims = [cone_dim] * (m // cone_dim)
+    s0 = _soc_interior(rng, dims)
+    y0 = _soc_interior(rng, dims)
+    P = ConicProblem(n=n, m=m, nb=n, c=np.zeros(n), rowptr=rowptr, colidx=colidx, vals=vals,
+                     h=np.zeros(m), bl=np.full(n, -box), bu=np.full(n, box), soc=dims, name=name)
+    P.h = P.matvec(x0) - s0
+    P.c = P.rmatvec(y0) + rng.normal(0.0, 0.1, n)
+    return P
+
+
+def workload0(seed: int = 1000) -> ConicProblem:
+    """n=1000 box [-10,10]; G 2000x1000 at 1% density; 200 SOC(10) blocks."""
+    return soc_program(1000, 2000, 0.01, 10, seed, name="soc_n1000_m2000")
+
+
+def workload4_layer(k: int, seed: int = 1004) -> ConicProblem:
+    """Layer k of the 256-layer batch: n=200, m=400, 5% density, 40 SOC(10)."""
+    return soc_program(200, 400, 0.05, 10, seed * 1000 + k, name=f"soc_layer{k}")
+
+
+def workload2(seed: int = 1002, rows: int = 200000, cols: int = 50000,
+              density: float = 1e-4) -> ConicProblem:
+    """Large SOC robust regression: min t s.t. (t, A x - b) in SOC(rows+1), |x| <= B.
+
+    A (rows x cols) at `density`, N(0,1); w* 1%-sparse N(0,1);
+    b = A w* + N(0, 0.01^2); B = 10 ||w*||_inf.  Variables are (x, t) with t
+    last and free; row 0 of G picks t, rows 1.. are [A, 0] with h = [0; b].
+    """
+    rng = np.random.default_rng(seed)
+    nnz = int(round(density * rows * cols))
+    r, cidx = random_pattern(rng, rows, cols, nnz)
+    v = rng.standard_normal(nnz)
+    wstar = np.zeros(cols)
+    k = max(1, int(round(0.01 * cols)))
+    idx = rng.choice(cols, size=k, replace=False)
+    wstar[idx] = rng.standard_normal(k)
+    n, m = cols + 1, rows + 1
+    allr = np.concatenate([[0], r + 1])
+    allc = np.concatenate([[cols], cidx])
+    allv = np.concatenate([[1.0], v])
+    rowptr, colidx, vals = csr_from_coo(m, allr, allc, allv)
+    Aw = np.bincount(r, weights=v * wstar[cidx], minlength=rows)
+    b = Aw + rng.normal(0.0, 0.01, rows)
+    B = 10.0 * np.max(np.abs(wstar))
+    c = np.zeros(n)
+    c[cols] = 1.0
+    bl = np.concatenate([np.full(cols, -B), [-np.inf]])
+    bu = np.concatenate([np.full(cols, B), [np.inf]])
+    h = np.concatenate([[0.0], b])
+    return ConicProblem(n=n, m=m, nb=n, c=c, rowptr=rowptr, colidx=colidx, vals=vals, h=h,
+                        bl=bl, bu=bu, soc=[m], name=f"soc_regression_{rows}x{cols}")
+
+
+def lp_box(n: int, m_eq: int, m_in: int, seed: int) -> ConicProblem:
+    """Small feasible, bounded LP (zero + nonnegative cones, box primal) for linprog checks."""
+    rng = np.random.default_rng(seed)
+    m = m_eq + m_in
+    A = rng.standard_normal((m, n)) * (rng.uniform(size=(m, n)) < 0.4)
+    x0 = rng.uniform(-1, 1, n)
+    h = A @ x0
+    h[m_eq:] -= rng.uniform(0.1, 1.0, m_in)
+    rows, cols = np.nonzero(A)
+    rowptr, colidx, vals = csr_from_coo(m, rows, cols, A[rows, cols])
+    c = rng.standard_normal(n)
+    return ConicProblem(n=n, m=m, nb=n, c=c, rowptr=rowptr, colidx=colidx, vals=vals, h=h,
+                        bl=np.full(n, -2.0), bu=np.full(n, 2.0), mzero=m_eq, mnonneg=m_in,
+                        name=f"lp_{n}x{m}")
+
+
+WORKLOADS = {0: workload0, 2: workload2}

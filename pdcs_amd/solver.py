@@ -1,0 +1,207 @@
+"""Python host interface mirroring the reference's GPU solve entry points.
+
+``rpdhg_gpu_solve`` keeps the keyword names and defaults of
+``PDCS_GPU.rpdhg_gpu_solve`` (src/pdcs_gpu/rpdhg_alg_gpu_gen.jl:1219-1267) and
+returns a ``Solution`` exposing the fields the reference's MOI wrapper reads
+(src/pdcs_gpu/MOI_wrapper/MOI_wrapper.jl:648-659,700-710):
+``x.recovered_primal.primal_sol.x``, ``y.recovered_dual.dual_sol.y``,
+``y.slack.primal_sol.x`` and ``info.{exit_code, exit_status, pObj, dObj, time, iter}``.
+Every solve runs on the GPU through libpdcs_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from types import SimpleNamespace
+
+import numpy as np
+
+from . import _lib
+
+EXIT_STATUS = {0: "optimal", 1: "max_iter", 2: "primal_infeasible_low_acc",
+               3: "primal_infeasible_high_acc", 4: "dual_infeasible_low_acc",
+               5: "dual_infeasible_high_acc", 6: "time_limit", 7: "continue",
+               8: "numerical_error_nan", 9: "numerical_error_inf"}
+
+
+def _p(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class Context:
+    """A HIP stream on one device (one per host thread)."""
+
+    def __init__(self, device: int = 0):
+        self.L = _lib.load()
+        self.h = C.c_void_p()
+        _lib.check(self.L.pdcs_ctx_create(device, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            self.L.pdcs_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def make_cones(mzero=0, mnonneg=0, soc=(), rsoc=(), nexp=0, ndexp=0):
+    soc_a = np.ascontiguousarray(np.asarray(list(soc), dtype=np.int64).reshape(-1))
+    rsoc_a = np.ascontiguousarray(np.asarray(list(rsoc), dtype=np.int64).reshape(-1))
+    k = _lib.Cones(mzero, mnonneg, len(soc_a), _p(soc_a, C.c_int64) if len(soc_a) else None,
+                   len(rsoc_a), _p(rsoc_a, C.c_int64) if len(rsoc_a) else None, nexp, ndexp)
+    k._keep = (soc_a, rsoc_a)
+    return k
+
+
+class Problem:
+    """A conic program resident on the GPU (scaled once at creation)."""
+
+    def __init__(self, ctx: Context, n, m, nb, c, rowptr, colidx, vals, h, bl, bu, cones):
+        self.ctx = ctx
+        self.n, self.m, self.nb = int(n), int(m), int(nb)
+        self._arrs = [np.ascontiguousarray(c, dtype=np.float64),
+                      np.ascontiguousarray(rowptr, dtype=np.int32),
+                      np.ascontiguousarray(colidx, dtype=np.int32),
+                      np.ascontiguousarray(vals, dtype=np.float64),
+                      np.ascontiguousarray(h, dtype=np.float64),
+                      np.ascontiguousarray(bl, dtype=np.float64),
+                      np.ascontiguousarray(bu, dtype=np.float64)]
+        a = self._arrs
+        self.cones = cones
+        self.h = C.c_void_p()
+        _lib.check(ctx.L.pdcs_problem_create_host(ctx.h, self.n, self.m, self.nb, _p(a[0]), _p(a[1], C.c_int32),
+                                                  _p(a[2], C.c_int32), _p(a[3]), _p(a[4]), _p(a[5]), _p(a[6]),
+                                                  C.byref(cones), C.byref(self.h)), ctx.h)
+
+    @classmethod
+    def from_instance(cls, P, ctx: Context | None = None):
+        ctx = ctx or default_context()
+        cones = make_cones(P.mzero, P.mnonneg, P.soc, (), P.nexp, P.ndexp)
+        return cls(ctx, P.n, P.m, P.nb, P.c, P.rowptr, P.colidx, P.vals, P.h, P.bl, P.bu, cones)
+
+    def solve(self, params: _lib.Params | None = None, x0=None, y0=None, **kw):
+        prm = params or _lib.default_params()
+        for k, v in kw.items():
+            setattr(prm, k, v)
+        res = _lib.Result()
+        xw = _p(np.ascontiguousarray(x0, dtype=np.float64)) if x0 is not None else None
+        yw = _p(np.ascontiguousarray(y0, dtype=np.float64)) if y0 is not None else None
+        if x0 is not None and y0 is not None:
+            prm.warm_start = 1
+        _lib.check(self.ctx.L.pdcs_solve(self.h, C.byref(prm), xw, yw, C.byref(res)), self.ctx.h)
+        return res.as_dict()
+
+    def solution(self):
+        x = np.zeros(self.n)
+        y = np.zeros(self.m)
+        s = np.zeros(self.n)
+        _lib.check(self.ctx.L.pdcs_get_solution(self.h, _p(x), _p(y), _p(s)), self.ctx.h)
+        return x, y, s
+
+    def scaling(self):
+        Dr = np.zeros(self.n)
+        Dl = np.zeros(self.m)
+        _lib.check(self.ctx.L.pdcs_get_scaling(self.h, _p(Dr), _p(Dl)), self.ctx.h)
+        return Dr, Dl
+
+    def evaluate(self, x, y):
+        res = _lib.Result()
+        _lib.check(self.ctx.L.pdcs_evaluate(self.h, _p(np.ascontiguousarray(x, dtype=np.float64)),
+                                            _p(np.ascontiguousarray(y, dtype=np.float64)), C.byref(res)),
+                   self.ctx.h)
+        return res.as_dict()
+
+    def close(self):
+        if self.h:
+            self.ctx.L.pdcs_problem_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _solution_struct(x, y, s, res):
+    info = SimpleNamespace(exit_code=res["exit_code"], exit_status=EXIT_STATUS.get(res["exit_code"], "unknown"),
+                           pObj=res["pobj"], dObj=res["dobj"], time=res["time_solve"], iter=res["iter"],
+                           restart_used=res["restarts"], result=res)
+    return SimpleNamespace(
+        x=SimpleNamespace(recovered_primal=SimpleNamespace(primal_sol=SimpleNamespace(x=x))),
+        y=SimpleNamespace(recovered_dual=SimpleNamespace(dual_sol=SimpleNamespace(y=y)),
+                          slack=SimpleNamespace(primal_sol=SimpleNamespace(x=s))),
+        info=info)
+
+
+def rpdhg_gpu_solve(*, n, m, nb, c, G, h, mGzero, mGnonnegative, socG, rsocG=(), expG=0, dual_expG=0,
+                    bl, bu, soc_x=(), rsoc_x=(), exp_x=0, dual_exp_x=0, rescaling_method="ruiz_pock_chambolle",
+                    use_preconditioner=True, use_adaptive_restart=True, use_adaptive_step_size_weight=True,
+                    use_aggressive=True, use_accelerated=False, use_resolving=True, primal_sol=None,
+                    dual_sol=None, warm_start=False, max_outer_iter=10000, max_inner_iter=500000,
+                    abs_tol=1e-6, rel_tol=1e-6, eps_primal_infeasible_low_acc=1e-12,
+                    eps_dual_infeasible_low_acc=1e-12, eps_primal_infeasible_high_acc=1e-16,
+                    eps_dual_infeasible_high_acc=1e-16, print_freq=2000, check_terminate_freq=2000,
+                    verbose=1, time_limit=float("inf"), method="average", logfile_name=None,
+                    use_kkt_restart=False, kkt_restart_freq=2000, use_duality_gap_restart=True,
+                    duality_gap_restart_freq=2000, precision="fp64", ctx: Context | None = None):
+    """Mirror of ``PDCS_GPU.rpdhg_gpu_solve`` (rpdhg_alg_gpu_gen.jl:1219).
+
+    ``G`` is a ``(rowptr, colidx, vals)`` 0-based CSR triple or a scipy sparse
+    matrix.  Unsupported reference options raise ``ValueError`` instead of
+    silently changing the algorithm (primal-side cones, rotated SOC,
+    non-default rescaling methods, the non-aggressive main loops).
+    """
+    if len(soc_x) or len(rsoc_x) or exp_x or dual_exp_x:
+        raise ValueError("primal-side cones are not supported (the reference's are undefined, def_rpdhg_gen.jl:153)")
+    if len(rsocG):
+        raise ValueError("rotated SOC constraints are not supported")
+    if rescaling_method != "ruiz_pock_chambolle" or not use_preconditioner:
+        raise ValueError("only rescaling_method=:ruiz_pock_chambolle with the preconditioner is supported")
+    if not (use_adaptive_restart and use_adaptive_step_size_weight and use_resolving and use_aggressive
+            and not use_accelerated and method == "average"):
+        raise ValueError("only the aggressive restarted adaptive main loop (the reference GPU default) is supported")
+    if hasattr(G, "tocsr"):
+        Gc = G.tocsr()
+        Gc.sort_indices()
+        rowptr, colidx, vals = Gc.indptr, Gc.indices, Gc.data
+    else:
+        rowptr, colidx, vals = G
+    ctx = ctx or default_context()
+    cones = make_cones(mGzero, mGnonnegative, socG, (), expG, dual_expG)
+    prob = Problem(ctx, n, m, nb, c, rowptr, colidx, vals, h, bl, bu, cones)
+    prm = _lib.default_params()
+    prm.max_outer_iter = max_outer_iter
+    prm.max_inner_iter = max_inner_iter
+    prm.abs_tol = abs_tol
+    prm.rel_tol = rel_tol
+    prm.time_limit = time_limit
+    prm.check_terminate_freq = check_terminate_freq
+    prm.use_kkt_restart = int(use_kkt_restart)
+    prm.kkt_restart_freq = kkt_restart_freq
+    prm.use_duality_gap_restart = int(use_duality_gap_restart)
+    prm.duality_gap_restart_freq = duality_gap_restart_freq
+    prm.verbose = int(verbose > 1)
+    prm.precision = _lib.PDCS_MIXED if precision == "mixed" else _lib.PDCS_FP64
+    x0 = y0 = None
+    if warm_start:
+        if primal_sol is None or dual_sol is None or len(primal_sol) != n or len(dual_sol) != m:
+            raise ValueError("Warmstart doesn't match the problem size")
+        x0, y0 = primal_sol, dual_sol
+    res = prob.solve(prm, x0=x0, y0=y0)
+    x, y, s = prob.solution()
+    prob.close()
+    return _solution_struct(x, y, s, res)

@@ -1,0 +1,170 @@
+"""GPU parity: libpdcs_hip.so kernels and solves against the CPU FP64 oracle."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from pdcs_amd import _lib, instances
+from pdcs_amd.solver import Problem, make_cones
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def dev(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a)).to(device="cuda:0", dtype=dtype).contiguous()
+
+
+def vp(t):
+    return C.c_void_p(t.data_ptr())
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("shape", [(2000, 1000, 0.01), (3000, 200, 0.05), (500, 4000, 0.002)])
+def test_spmv_matches_oracle(ctx, prec, shape):
+    m, n, dens = shape
+    P = instances.soc_program(n, m, dens, 10, seed=5 + m)
+    x = np.random.default_rng(1).standard_normal(n)
+    ref = oracle.spmv(P.rowptr, P.colidx, P.vals, x)
+    dt = torch.float64 if prec == 0 else torch.float32
+    rp, ci = dev(P.rowptr, torch.int32), dev(P.colidx, torch.int32)
+    va, xx = dev(P.vals, dt), dev(x, dt)
+    y = torch.zeros(m, dtype=dt, device="cuda:0")
+    _lib.check(ctx.L.pdcs_spmv(ctx.h, prec, m, n, vp(rp), vp(ci), vp(va), vp(xx), vp(y)), ctx.h)
+    got = y.cpu().numpy().astype(np.float64)
+    tol = 1e-13 if prec == 0 else 2e-6
+    scale = np.abs(P.vals)[None, :].max() * np.abs(x).max() * 20
+    assert np.max(np.abs(got - ref)) <= tol * scale
+
+
+def _random_cones(rng, dims):
+    y = rng.standard_normal(sum(dims)) * rng.uniform(0.1, 10)
+    d = np.ones(sum(dims))
+    off = 0
+    for q in dims:
+        dl = np.exp(rng.uniform(-2, 2, q))
+        d[off:off + q] = dl[0] / dl
+        off += q
+    return y, d
+
+
+@pytest.mark.parametrize("dims", [[10] * 200, [3, 7, 64, 65, 200, 1024], [5000], [200001]])
+def test_project_dual_scaled_soc_matches_oracle(ctx, dims):
+    rng = np.random.default_rng(len(dims) + dims[0])
+    y, d = _random_cones(rng, dims)
+    # also place some heads so that interior / polar / boundary cases all occur
+    off = 0
+    ref = y.copy()
+    for b, q in enumerate(dims):
+        if b % 3 == 1:
+            y[off] = 3 * np.linalg.norm(d[off + 1:off + q] * y[off + 1:off + q])
+        elif b % 3 == 2:
+            y[off] = -3 * np.linalg.norm(y[off + 1:off + q] / d[off + 1:off + q])
+        off += q
+    ref = y.copy()
+    off = 0
+    for q in dims:
+        ref[off:off + q] = oracle.proj_soc_scaled(ref[off:off + q], d[off:off + q], rel_tol=1e-14, abs_tol=1e-18)
+        off += q
+    cones = make_cones(0, 0, dims)
+    t = dev(y)
+    dd = dev(d)
+    _lib.check(ctx.L.pdcs_project_dual(ctx.h, 0, C.byref(cones), len(y), vp(t), vp(dd)), ctx.h)
+    got = t.cpu().numpy()
+    off = 0
+    for q in dims:
+        a, b = got[off:off + q], ref[off:off + q]
+        assert np.linalg.norm(a - b) <= 1e-9 * (1 + np.linalg.norm(y[off:off + q])), (q, a[:3], b[:3])
+        off += q
+
+
+def test_project_primal_cone_plain_soc(ctx):
+    rng = np.random.default_rng(3)
+    dims = [4, 10, 100, 3000]
+    r = rng.standard_normal(sum(dims) + 7)
+    cones = make_cones(3, 4, dims)
+    ref = r.copy()
+    ref[:3] = 0
+    ref[3:7] = np.maximum(ref[3:7], 0)
+    off = 7
+    for q in dims:
+        ref[off:off + q] = oracle.proj_soc(ref[off:off + q])
+        off += q
+    t = dev(r)
+    _lib.check(ctx.L.pdcs_project_primal_cone(ctx.h, C.byref(cones), len(r), vp(t)), ctx.h)
+    assert np.allclose(t.cpu().numpy(), ref, rtol=1e-13, atol=1e-14)
+
+
+def test_scaling_matches_oracle(ctx):
+    P = instances.workload0()
+    prob = Problem.from_instance(P, ctx)
+    Dr, Dl = prob.scaling()
+    rDr, rDl, _ = oracle.scale(P)
+    assert np.allclose(Dr, rDr, rtol=1e-12)
+    assert np.allclose(Dl, rDl, rtol=1e-12)
+
+
+def test_evaluate_matches_oracle_kkt(ctx):
+    P = instances.workload0()
+    rng = np.random.default_rng(9)
+    x, y = rng.standard_normal(P.n), rng.standard_normal(P.m)
+    prob = Problem.from_instance(P, ctx)
+    got = prob.evaluate(x, y)
+    ref = oracle.kkt(P, x, y)
+    for k in ["pobj", "dobj", "rel_primal_res", "rel_dual_res", "rel_gap", "l2_rel_primal_res", "l2_rel_dual_res",
+              "rel_primal_res_code"]:
+        assert abs(got[k] - ref[k]) <= 1e-11 * (1 + abs(ref[k])), (k, got[k], ref[k])
+
+
+def _check_solution(P, res, x, y, ref, tol=1e-6):
+    assert res["exit_code"] == 0, res
+    assert max(res["rel_primal_res"], res["rel_dual_res"], res["rel_gap"]) < tol
+    for k in ("pobj", "dobj"):
+        assert abs(res[k] - ref[k]) <= tol * (1 + abs(ref[k])), (k, res[k], ref[k])
+    chk = oracle.kkt(P, x, y)   # independent re-evaluation of the returned iterate
+    assert max(chk["rel_primal_res"], chk["rel_dual_res"], chk["rel_gap"]) < tol, chk
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_solve_workload0_fp64(ctx, graph):
+    P = instances.workload0()
+    ref, _, _ = oracle.solve(P)
+    prob = Problem.from_instance(P, ctx)
+    res = prob.solve(use_graph=graph, time_limit=100.0)
+    x, y, _ = prob.solution()
+    _check_solution(P, res, x, y, ref)
+    assert res["iter"] == ref["iter"]
+
+
+def test_solve_workload0_mixed(ctx):
+    P = instances.workload0()
+    ref, _, _ = oracle.solve(P)
+    prob = Problem.from_instance(P, ctx)
+    res = prob.solve(precision=1, time_limit=100.0)
+    x, y, _ = prob.solution()
+    _check_solution(P, res, x, y, ref)
+
+
+def test_solve_lp(ctx):
+    scipy = pytest.importorskip("scipy.optimize")
+    P = instances.lp_box(60, 10, 30, seed=4)
+    A = P.dense()
+    lp = scipy.linprog(P.c, A_ub=-A[10:], b_ub=-P.h[10:], A_eq=A[:10], b_eq=P.h[:10],
+                       bounds=list(zip(P.bl, P.bu)), method="highs")
+    prob = Problem.from_instance(P, ctx)
+    res = prob.solve(time_limit=100.0)
+    x, y, _ = prob.solution()
+    assert res["exit_code"] == 0
+    assert abs(res["pobj"] - lp.fun) <= 1e-5 * (1 + abs(lp.fun))
+
+
+def test_solve_regression_small(ctx):
+    P = instances.workload2(rows=20000, cols=5000, density=1e-3)
+    ref, _, _ = oracle.solve(P)
+    prob = Problem.from_instance(P, ctx)
+    for prec in (0, 1):
+        res = prob.solve(precision=prec, time_limit=100.0)
+        x, y, _ = prob.solution()
+        _check_solution(P, res, x, y, ref)

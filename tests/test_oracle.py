@@ -1,0 +1,212 @@
+"""Pins of the CPU FP64 oracle (the checker of the HIP path).
+
+Each check is fixed by mathematics, not by retyping the oracle: closed forms,
+the optimality conditions that characterise a Euclidean projection onto a
+closed convex cone, brute-force minimisation, HiGHS on linear programs, weak
+duality through an independent README-residual evaluator, and the invariants
+of Ruiz / Pock-Chambolle equilibration.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from pdcs_amd import instances
+from tests.kkt_numpy import proj_soc, readme_residuals
+
+
+# ------------------------------------------------------------------ SOC --
+def test_soc_closed_form_moreau():
+    rng = np.random.default_rng(0)
+    for q in [2, 3, 10, 57]:
+        for _ in range(50):
+            v = rng.standard_normal(q) * 3
+            p = oracle.proj_soc(v)
+            pm = oracle.proj_soc(-v)
+            # Moreau for a self-dual cone: v = P(v) - P(-v), <P(v), P(-v)> = 0
+            assert np.allclose(v, p - pm, atol=1e-12)
+            assert abs(p @ pm) <= 1e-10 * (1 + v @ v)
+            assert np.linalg.norm(p[1:]) <= p[0] + 1e-12
+            assert np.allclose(oracle.proj_soc(p), p, atol=1e-13)   # idempotent
+
+
+def _scaled_soc_kkt(v, d, p):
+    """Optimality of p = argmin ||w - v|| over K_d = {(t,z): ||d z|| <= t}."""
+    t, z = p[0], p[1:]
+    feas = np.linalg.norm(d[1:] * z) - t                      # <= 0
+    w = v - p                                                  # must lie in the polar
+    polar = np.linalg.norm(w[1:] / d[1:]) + w[0]              # <= 0  (polar: ||b/d|| <= -a)
+    comp = abs(p @ w)
+    return feas, polar, comp
+
+
+@pytest.mark.parametrize("q", [2, 3, 5, 10, 40, 300])
+def test_scaled_soc_optimality_conditions(q):
+    rng = np.random.default_rng(q)
+    for trial in range(60):
+        v = rng.standard_normal(q) * rng.uniform(0.01, 100)
+        dl = np.exp(rng.uniform(-3, 3, q))
+        d = dl[0] / dl
+        if trial % 4 == 1:
+            v[0] = 2 * np.linalg.norm(d[1:] * v[1:])           # interior
+        if trial % 4 == 2:
+            v[0] = -2 * np.linalg.norm(v[1:] / d[1:])          # polar -> 0
+        if trial % 4 == 3:
+            v[0] = 0.0                                         # t == 0 branch
+        p = oracle.proj_soc_scaled(v, d, rel_tol=1e-14, abs_tol=1e-18)
+        feas, polar, comp = _scaled_soc_kkt(v, d, p)
+        s = 1 + np.linalg.norm(v)
+        assert feas <= 1e-9 * s and polar <= 1e-9 * s and comp <= 1e-9 * s * s, (trial, feas, polar, comp)
+
+
+def test_scaled_soc_reduces_to_closed_form():
+    rng = np.random.default_rng(11)
+    for q in [2, 4, 25]:
+        for _ in range(40):
+            v = rng.standard_normal(q) * 5
+            a = oracle.proj_soc_scaled(v, np.ones(q), rel_tol=1e-14, abs_tol=1e-18)
+            b = proj_soc(v)
+            assert np.allclose(a, b, atol=1e-9 * (1 + np.linalg.norm(v)))
+
+
+def test_scaled_soc_brute_force():
+    from scipy.optimize import minimize
+
+    rng = np.random.default_rng(5)
+    for _ in range(25):
+        q = int(rng.integers(2, 6))
+        v = rng.standard_normal(q) * 2
+        d = np.concatenate([[1.0], np.exp(rng.uniform(-1, 1, q - 1))])
+        p = oracle.proj_soc_scaled(v, d, rel_tol=1e-14, abs_tol=1e-18)
+        cons = [{"type": "ineq", "fun": lambda w: w[0]},
+                {"type": "ineq", "fun": lambda w: w[0] ** 2 - np.sum((d[1:] * w[1:]) ** 2)}]
+        best = np.inf
+        for x0 in (np.abs(v) + 1.0, np.concatenate([[np.linalg.norm(v) + 1], np.zeros(q - 1)]),
+                   np.concatenate([[np.linalg.norm(v) * 3 + 1], v[1:]])):
+            r = minimize(lambda w: 0.5 * np.sum((w - v) ** 2), x0, constraints=cons, method="SLSQP",
+                         options={"ftol": 1e-14, "maxiter": 1000})
+            w = r.x
+            if w[0] >= -1e-9 and np.linalg.norm(d[1:] * w[1:]) <= w[0] + 1e-7:
+                best = min(best, 0.5 * np.sum((w - v) ** 2))
+        assert np.isfinite(best)
+        # the oracle's point is feasible and no feasible point found by SLSQP is closer
+        assert np.linalg.norm(d[1:] * p[1:]) <= p[0] + 1e-9
+        assert 0.5 * np.sum((p - v) ** 2) <= best + 1e-7
+
+
+# ---------------------------------------------------------- exponential --
+def _kexp_generators():
+    """Unit generators of Kexp: boundary rays (rho, 1, e^rho) on a dense rho grid
+    and the extreme rays (-1, 0, 0), (0, 0, 1) of the s = 0 face."""
+    rho = np.concatenate([np.linspace(-60, 12, 20001), [-200, -100, 30]])
+    g = np.stack([rho, np.ones_like(rho), np.exp(rho)], axis=1)
+    g = np.concatenate([g, [[-1.0, 0.0, 0.0], [0.0, 0.0, 1.0]]])
+    return g / np.linalg.norm(g, axis=1, keepdims=True)
+
+
+_GEN = _kexp_generators()
+
+
+def _in_kexp(w, tol):
+    r, s, t = w
+    if s > 0:
+        return s * np.exp(r / s) <= t + tol and s >= -tol
+    return s >= -tol and r <= tol and t >= -tol
+
+
+def _polar_violation(u):
+    """max over generators q of <u, q>: <= 0 iff u in the polar of Kexp."""
+    return float(np.max(_GEN @ u))
+
+
+def test_exp_projection_optimality():
+    rng = np.random.default_rng(2)
+    for _ in range(400):
+        v = rng.standard_normal(3) * rng.uniform(0.1, 5)
+        p = oracle.proj_exp(v)
+        sc = 1 + np.linalg.norm(v)
+        assert _in_kexp(p, 1e-9 * sc), (v, p)
+        # v - p in the polar (variational inequality against every generator of Kexp)
+        assert _polar_violation(v - p) <= 1e-8 * sc, (v, p)
+        assert abs(p @ (v - p)) <= 1e-8 * sc * sc, (v, p)
+
+
+def test_exp_projection_special_cases():
+    # r, s <= 0 -> (r, 0, max(t, 0)) ; points inside Kexp are fixed
+    assert np.allclose(oracle.proj_exp([-1.0, -2.0, 3.0]), [-1.0, 0.0, 3.0])
+    inside = np.array([0.5, 1.0, 2.0])
+    assert np.allclose(oracle.proj_exp(inside), inside)
+    # points in the polar project to 0
+    assert np.allclose(oracle.proj_exp([1.0, -5.0, -np.e * 10]), 0.0, atol=1e-12)
+
+
+def test_dual_exp_moreau():
+    rng = np.random.default_rng(8)
+    for _ in range(200):
+        v = rng.standard_normal(3) * 3
+        pd = oracle.proj_dual_exp(v)
+        # pd in Kexp*: <pd, q> >= 0 for every generator q of Kexp
+        assert _polar_violation(-pd) <= 1e-8 * (1 + np.linalg.norm(v))
+        # v = P_{K*}(v) + P_{K°}(v), K° = -K: P_K(-v) = pd - v
+        assert np.allclose(oracle.proj_exp(-v), pd - v, atol=1e-12)
+
+
+# -------------------------------------------------------------- scaling --
+def test_scaling_invariants():
+    P = instances.workload0()
+    Dr, Dl, vals = oracle.scale(P)
+    A = P.dense()
+    S = np.zeros_like(A)
+    rows = np.repeat(np.arange(P.m), np.diff(P.rowptr))
+    S[rows, P.colidx] = vals
+    # the scaled values are exactly Dl^-1 G Dr^-1
+    assert np.allclose(S, A / Dl[:, None] / Dr[None, :], rtol=1e-12, atol=0)
+    # Pock-Chambolle (alpha = 1) makes the operator norm <= 1 (their Lemma 2)
+    assert np.linalg.norm(S, 2) <= 1.0 + 1e-12
+    # without the PC pass, 10 Ruiz l_inf passes equilibrate row/column maxima to ~1
+    Dr2, Dl2, v2 = oracle.scale(P, ruiz_iters=10, pock=0)
+    S2 = np.zeros_like(A)
+    S2[rows, P.colidx] = v2
+    rmax, cmax = np.abs(S2).max(axis=1), np.abs(S2).max(axis=0)
+    assert np.all(np.abs(rmax[rmax > 0] - 1) < 1e-2)   # empty rows/columns stay 0
+    assert np.all(np.abs(cmax[cmax > 0] - 1) < 1e-2)
+
+
+# ------------------------------------------------------------ whole solve --
+def test_kkt_matches_independent_evaluator():
+    P = instances.workload0()
+    rng = np.random.default_rng(3)
+    x, y = rng.standard_normal(P.n), rng.standard_normal(P.m)
+    a = oracle.kkt(P, x, y)
+    b = readme_residuals(P, x, y)
+    assert np.isclose(a["rel_primal_res"], b["primal"], rtol=1e-12)
+    assert np.isclose(a["rel_dual_res"], b["dual"], rtol=1e-12, atol=1e-300)
+    assert np.isclose(a["rel_gap"], b["gap"], rtol=1e-10)
+    assert np.isclose(a["pobj"], b["pobj"], rtol=1e-12)
+    assert np.isclose(a["dobj"], b["dobj"], rtol=1e-12)
+
+
+def test_oracle_lp_vs_highs():
+    from scipy.optimize import linprog
+
+    for seed in range(3):
+        P = instances.lp_box(40, 8, 25, seed=seed)
+        A = P.dense()
+        lp = linprog(P.c, A_ub=-A[8:], b_ub=-P.h[8:], A_eq=A[:8], b_eq=P.h[:8],
+                     bounds=list(zip(P.bl, P.bu)), method="highs")
+        assert lp.status == 0
+        res, x, y = oracle.solve(P, time_limit=60.0)
+        assert res["exit_code"] == 0
+        assert abs(res["pobj"] - lp.fun) <= 2e-6 * (1 + abs(lp.fun)), (res["pobj"], lp.fun)
+
+
+def test_oracle_workload0_weak_duality():
+    P = instances.workload0()
+    res, x, y = oracle.solve(P, time_limit=120.0)
+    assert res["exit_code"] == 0
+    r = readme_residuals(P, x, y)
+    assert max(r["primal"], r["dual"], r["gap"]) < 1e-6
+    # y must lie in K* (the dual objective is then a valid lower bound)
+    from tests.kkt_numpy import dual_cone_violation
+
+    assert dual_cone_violation(P, y) < 1e-9 * (1 + np.abs(y).max())
+    assert abs(r["pobj"] - res["pobj"]) < 1e-9 * (1 + abs(r["pobj"]))
