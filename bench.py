@@ -1,0 +1,118 @@
+#!/usr/bin/env python3
+"""Benchmark: time-to-1e-6 relative KKT of the PDCS PDHG hot path on one MI355X.
+
+Workload: the large SOC robust regression (A 200000 x 50000 at density 1e-4,
+one SOC of dimension 200001, 50000 box bounds), synthetic and seeded.  A step
+is one full solve from a cold start to the README l_inf tolerance 1e-6 with
+the problem already resident (uploaded and scaled) in HBM; `value` is the mean
+mixed-precision (FP32 iterates/SpMV, FP64 reductions and tests, FP64 polish)
+time-to-tolerance and the FP64 path is timed beside it.  The JSON line also
+carries the roofline of the dominant kernel (HIP-event timing inside the
+library's stream) and the CPU baseline (the FP64 oracle on this host, one
+thread, on a bounded sample of attempts).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "time-to-1e-6 rel. KKT & PDHG iters/s, FP32-mixed vs FP64 on one MI355X"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--rows", type=int, default=200000)
+    ap.add_argument("--cols", type=int, default=50000)
+    ap.add_argument("--density", type=float, default=1e-4)
+    ap.add_argument("--cpu-attempts", type=int, default=600)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--reps", type=int, default=50)
+    ap.add_argument("--no-graph", action="store_true", help="plain launches (for profilers that cannot follow graphs)")
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {a.gpus} is not supported (the solver runs on a single "
+              "MI355X; multi-GPU sharding is a design note, not a measured path).", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    from pdcs_amd import instances
+    from pdcs_amd.solver import Problem
+
+    t0 = time.time()
+    P = instances.workload2(rows=a.rows, cols=a.cols, density=a.density)
+    print(f"[bench] workload {P.name}: n={P.n} m={P.m} nnz={P.nnz} generated in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    prob = Problem.from_instance(P)
+
+    def run(precision, steps, warmup, tag):
+        out = []
+        for k in range(warmup + steps):
+            res = prob.solve(precision=precision, time_limit=600.0, use_graph=0 if a.no_graph else 1)
+            print(f"[bench] {tag} solve {k}: exit {res['exit_code']} iters {res['iter']} "
+                  f"time {res['time_solve']:.4f}s pobj {res['pobj']:.10e} "
+                  f"res {res['rel_primal_res']:.2e}/{res['rel_dual_res']:.2e}/{res['rel_gap']:.2e}",
+                  file=sys.stderr, flush=True)
+            if k >= warmup:
+                out.append(res)
+        return out
+
+    mixed = run(1, a.steps, a.warmup, "mixed")
+    kt_mixed = prob.time_kernels(precision=1, reps=a.reps)
+    fp64 = run(0, a.steps, a.warmup, "fp64")
+    kt_fp64 = prob.time_kernels(precision=0, reps=a.reps)
+    x, y, _ = prob.solution()
+
+    def summary(rs):
+        t = float(np.mean([r["time_solve"] for r in rs]))
+        it = float(np.mean([r["iter"] for r in rs]))
+        r = rs[-1]
+        return {"time_to_tol_s": t, "iters": it, "iters_per_s": it / t, "exit_code": r["exit_code"],
+                "pobj": r["pobj"], "dobj": r["dobj"], "rel_primal_res": r["rel_primal_res"],
+                "rel_dual_res": r["rel_dual_res"], "rel_gap": r["rel_gap"], "iter_fp32": r["iter_fp32"],
+                "time_kkt_s": r["time_kkt"], "time_restart_s": r["time_restart"]}
+
+    sm, s64 = summary(mixed), summary(fp64)
+    # dominant kernel of the mixed path
+    dom = max(kt_mixed, key=lambda k: kt_mixed[k]["ms"])
+    ach = kt_mixed[dom]["bytes"] / (kt_mixed[dom]["ms"] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernels_mixed": kt_mixed, "kernels_fp64": kt_fp64}
+    cpu = None
+    if not a.no_cpu:
+        os.environ.setdefault("OMP_NUM_THREADS", "1")
+        from oracle import oracle
+
+        t_cpu, done = oracle.time_attempts(P, a.cpu_attempts)
+        per_attempt = t_cpu / max(done, 1)
+        cpu = {"value": round(per_attempt * s64["iters"], 3), "unit": "s", "cores": 1, "kind": "port",
+               "iters_per_s": round(1.0 / per_attempt, 2),
+               "sample": f"FP64 oracle, first {done} PDHG attempts of this workload timed on one host thread "
+                         f"({t_cpu:.1f}s), scaled to the {s64['iters']:.0f} attempts the FP64 GPU solve needed"}
+    line = {"metric": METRIC, "value": round(sm["time_to_tol_s"], 5), "unit": "s", "n_gpus": 1, "steps": a.steps,
+            "warmup": a.warmup, "ms_per_step": round(sm["time_to_tol_s"] * 1e3, 3), "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32 iterates+SpMV / fp64 reductions+polish",
+            "data": "synthetic (seeded PCG64, planted sparse regression)",
+            "config": {"workload": f"large SOC robust regression {a.rows}x{a.cols} density {a.density:g}",
+                       "n": P.n, "m": P.m, "nnz": P.nnz, "cones": f"1 SOC({P.m})", "tol": 1e-6},
+            "mixed": sm, "fp64": s64, "speedup_mixed_vs_fp64": round(s64["time_to_tol_s"] / sm["time_to_tol_s"], 3),
+            "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -69,9 +69,10 @@ typedef struct {
     int32_t bisect_intent;            /* 0: duality-gap bisection as the reference parses it */
     int32_t code_primal_denominator;  /* 0: README primal residual denominator */
     int32_t use_graph;                /* 1: replay attempt chunks as hipGraphs */
-    int64_t mixed_check_freq;         /* MIXED: FP32 phase check interval (0 = check_terminate_freq) */
-    int64_t polish_check_freq;        /* FP64 polish check interval (0 = check_terminate_freq) */
-    double mixed_switch_tol;          /* MIXED: FP32 phase stops below this README KKT (1e-5) */
+    int64_t mixed_check_freq;         /* MIXED: FP32 phase check interval (200; 0 = check_terminate_freq) */
+    int64_t polish_check_freq;        /* MIXED: FP64 polish check interval (200; 0 = check_terminate_freq) */
+    double mixed_switch_tol;          /* MIXED: the FP32 phase hands over to the FP64 polish below this
+                                         README KKT (1e-5) or when a check fails to halve it */
 } pdcs_params;
 
 /* Result of a solve; exit codes as the reference's check_termination_criteria
@@ -135,6 +136,16 @@ int pdcs_get_scaling(pdcs_problem *p, double *Dr, double *Dl);
 /* README residuals of host (x, y) against the problem (converge_info_calculation_diagonal!,
  * rpdhg_alg_gpu_gen_scaling.jl:126-183), computed on the device. */
 int pdcs_evaluate(pdcs_problem *p, const double *x, const double *y, pdcs_result *res);
+
+/* Per-kernel timing of one PDHG attempt on the context's stream with HIP events:
+ * ms[k] = mean milliseconds of kernel k over `reps` back-to-back launches and
+ * bytes[k] = its algorithmic HBM bytes per launch, for k = 0 primal step
+ * (G'y + update + box), 1 dual step (G xbar + update), 2 cone projections,
+ * 3 G'dy dot, 4 step-size control, 5 reflection/average.  The analogue of the
+ * reference's wall-clock accumulators time_proj / time_iterative
+ * (rpdhg_alg_gpu_gen.jl:458-463).  Needs a previous pdcs_solve in the same
+ * precision (it runs on that solve's iterates and leaves them modified). */
+int pdcs_time_kernels(pdcs_problem *p, int32_t precision, int32_t reps, double *ms, double *bytes);
 
 /* ---- kernel-level entry points (device pointers, run on the context's stream,
  * synchronous on return) ---- */

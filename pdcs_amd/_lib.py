@@ -54,7 +54,7 @@ class Result(C.Structure):
 EXPORTS = ["pdcs_params_default", "pdcs_ctx_create", "pdcs_ctx_destroy", "pdcs_last_error",
            "pdcs_problem_create_host", "pdcs_problem_create_device", "pdcs_problem_destroy",
            "pdcs_solve", "pdcs_get_solution", "pdcs_get_scaling", "pdcs_evaluate", "pdcs_spmv",
-           "pdcs_project_dual", "pdcs_project_primal_cone"]
+           "pdcs_project_dual", "pdcs_project_primal_cone", "pdcs_time_kernels"]
 
 _lib = None
 
@@ -90,6 +90,7 @@ def load():
     L.pdcs_spmv.argtypes = [vp, c_i32, c_i64, c_i64, vp, vp, vp, vp, vp]
     L.pdcs_project_dual.argtypes = [vp, c_i32, C.POINTER(Cones), c_i64, vp, vp]
     L.pdcs_project_primal_cone.argtypes = [vp, C.POINTER(Cones), c_i64, vp]
+    L.pdcs_time_kernels.argtypes = [vp, c_i32, c_i32, c_dp, c_dp]
     _lib = L
     return L
 

@@ -21,6 +21,22 @@ struct Csr {
     int32_t lanes;   // lanes per row for the wave-split SpMV (power of two, 1..64)
 };
 
+// Per-cone scalar state of the multi-workgroup scaled-SOC root find.
+struct BigSocState {
+    double s;          // head of the point being projected
+    double t, tL, tR;  // Newton iterate and bracket of the head t
+    double lam;        // multiplier of the last projection (warm start)
+    double t_closed;   // head for the s == 0 closed form
+    int32_t cs;        // case: 0 interior, 1 polar, 2 s == 0, 3 Newton
+    int32_t done;
+    int32_t iters;
+    int32_t pad;
+};
+constexpr int kBigElems = 4;                       // tail elements per thread in a pass
+constexpr int kBigBlockElems = kBigElems * 256;    // tail elements per workgroup
+constexpr int kBigSums = 6;
+constexpr int kBigNewtonPasses = 4;
+
 // Cone layout of the dual rows on the device.
 struct ConeLayout {
     int64_t m;
@@ -29,8 +45,15 @@ struct ConeLayout {
     int32_t nsoc_small;          // SOCs handled by lane groups (dim <= 64 * kSocRegs)
     const int32_t *soc_small_start, *soc_small_dim;
     int32_t soc_small_group;     // lanes per cone for the small-SOC kernel
-    int32_t nsoc_big;            // SOCs handled by one workgroup each
+    int32_t nsoc_big;            // SOCs split over several workgroups each
     const int32_t *soc_big_start, *soc_big_dim;
+    int32_t nbig_blocks;         // workgroups of one pass over all big SOCs
+    const int32_t *big_blk_cone; // cone of each workgroup
+    const int32_t *big_blk_first;// first tail element of each workgroup
+    const int32_t *big_cone_blk0, *big_cone_nblk;
+    BigSocState *big_state;      // per big cone
+    double *big_part;            // kBigSums partials per workgroup
+    unsigned int *big_count;     // arrival counters per big cone
     int32_t nexp;                // exponential cones (code 12: y in Kexp*)
     int32_t ndexp;               // dual exponential cones (code 15: y in Kexp)
     int64_t exp_begin;           // first row of the exponential section
@@ -87,6 +110,16 @@ template <typename T>
 void launch_attempt(hipStream_t s, int slot, const Csr<T> &G, const Csr<T> &Gt,
                     const ProbVec<T> &P, const ConeLayout &K, const double *d2,
                     Iter<T> it, StepState *st, double *part, int64_t part_stride);
+
+// the kernels of one attempt, individually: 0 primal (G'y + update), 1 dual (G xbar +
+// update), 2 cone projections, 3 G'dy dot, 4 step control, 5 reflection
+constexpr int kAttemptKernels = 6;
+constexpr unsigned kAttemptMaxGrid = 2048;   // workgroups of the fused SpMV kernels (grid-stride beyond)
+unsigned attempt_grid(int64_t rows, int lanes);
+template <typename T>
+void launch_attempt_part(int part_id, hipStream_t s, int slot, const Csr<T> &G, const Csr<T> &Gt,
+                         const ProbVec<T> &P, const ConeLayout &K, const double *d2, Iter<T> it,
+                         StepState *st, double *part, int64_t part_stride);
 
 // continuation of an attempt whose decision was deferred to the host (exit check)
 void launch_continue(hipStream_t s, int slot, StepState *st);

@@ -101,12 +101,12 @@ __global__ __launch_bounds__(kBlock) void k_primal(Csr<T> Gt, ProbVec<T> P, cons
     __shared__ double lds[kBlock / 64];
     if (halted(st, slot)) return;
     const double ps = st->eta / st->omega * st->dec;   // primal_step_size, scaling.jl:687-689
-    const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int64_t row = gid / L;
-    const int lane = (int)(gid % L);
-    const double acc = row_dot<T, L>(Gt, row, lane, ys);
+    const int lane = (int)(threadIdx.x % L);
     double d2 = 0.0;
-    if (row < P.n && lane == 0) {
+    for (int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / L; row < P.n;
+         row += (int64_t)gridDim.x * (kBlock / L)) {
+    const double acc = row_dot<T, L>(Gt, row, lane, ys);
+    if (lane == 0) {
         const double xl = (double)xs[row];
         double v = xl + ps * (acc - (double)P.c[row]);            // primal_update, utils.cu:8
         if (row < P.nb) v = fmin(fmax(v, (double)P.bl[row]), (double)P.bu[row]);  // code 18
@@ -114,7 +114,8 @@ __global__ __launch_bounds__(kBlock) void k_primal(Csr<T> Gt, ProbVec<T> P, cons
         x[row] = xv;
         xbar[row] = (T)(2.0 * (double)xv - xl);                   // extrapolation_update, utils.cu:18
         const double d = (double)xv - xl;
-        d2 = d * d;
+        d2 += d * d;
+    }
     }
     d2 = block_sum(d2, lds);
     if (threadIdx.x == 0) part[blockIdx.x] = d2;
@@ -128,12 +129,12 @@ __global__ __launch_bounds__(kBlock) void k_dual(Csr<T> G, ProbVec<T> P, ConeLay
     __shared__ double lds[kBlock / 64];
     if (halted(st, slot)) return;
     const double ds = st->eta * st->omega * st->dec;  // dual_step_size, scaling.jl:698-700
-    const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int64_t row = gid / L;
-    const int lane = (int)(gid % L);
-    const double acc = row_dot<T, L>(G, row, lane, xbar);
+    const int lane = (int)(threadIdx.x % L);
     double d2 = 0.0;
-    if (row < P.m && lane == 0) {
+    for (int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / L; row < P.m;
+         row += (int64_t)gridDim.x * (kBlock / L)) {
+    const double acc = row_dot<T, L>(G, row, lane, xbar);
+    if (lane == 0) {
         const double yl = (double)ys[row];
         double v = yl - ds * (acc - (double)P.h[row]);             // dual_update, utils.cu:28
         if (row < K.cone_begin) {
@@ -142,10 +143,11 @@ __global__ __launch_bounds__(kBlock) void k_dual(Csr<T> G, ProbVec<T> P, ConeLay
             y[row] = yv;
             const double d = (double)yv - yl;
             dy[row] = (T)d;
-            d2 = d * d;
+            d2 += d * d;
         } else {
             y[row] = (T)v;                                         // projected by the cone kernels
         }
+    }
     }
     d2 = block_sum(d2, lds);
     if (threadIdx.x == 0) part[blockIdx.x] = d2;
@@ -188,16 +190,21 @@ __device__ __forceinline__ int soc_case(double s, const SocSums &z) {
     return 3;
 }
 
-// one Newton update given S, S1 at t; returns new t and convergence flag
+// One safeguarded Newton update of phi(t) = S^(-1/2) - 1 given S and S1 = sum a (1+d^2)/w^3
+// at t (phi' = S1 / S^(3/2)).  Convergence: phi == 0 keeps t; a Newton step
+// shorter than 1e-8 t leaves an error ~ (1e-8)^2 t by quadratic convergence, so
+// its result is final; a bracket narrower than 4 ulps ends a bisection tail.
 __device__ __forceinline__ double soc_newton_step(double t, double S, double S1, double &tL,
                                                   double &tR, bool &done) {
     const double isq = 1.0 / sqrt(S);
     const double phi = isq - 1.0;
+    if (phi == 0.0) { done = true; return t; }
     if (phi < 0.0) tL = fmax(tL, t); else tR = fmin(tR, t);
-    const double dphi = S1 * isq * isq * isq;           // phi'(t) = S1 / S^(3/2)
+    const double dphi = S1 * isq * isq * isq;
     double tn = t - phi / dphi;
-    if (!(tn > tL && tn < tR)) tn = 0.5 * (tL + tR);
-    done = fabs(tn - t) <= 4e-16 * fabs(tn) || phi == 0.0 || tR - tL <= 4e-16 * tR;
+    bool newton = true;
+    if (!(tn >= tL && tn <= tR)) { tn = 0.5 * (tL + tR); newton = false; }
+    done = (newton && fabs(tn - t) <= 1e-8 * fabs(tn)) || (tR - tL) <= 4.5e-16 * fabs(tR);
     return tn;
 }
 
@@ -274,74 +281,214 @@ __global__ __launch_bounds__(kBlock) void k_soc_group(ConeLayout K, const double
     }
 }
 
-// one workgroup per cone (large cones)
+// ------------------------------------------------ large scaled SOCs --
+// A cone longer than 1024 rows is split over workgroups of kBigBlockElems
+// tail elements.  Each pass writes per-workgroup partial sums; the last
+// workgroup to arrive (device-scope counter) reduces them in fixed order and
+// runs the scalar Newton update of phi(t) (see the small-cone kernel), so a
+// pass never waits on another workgroup.  Pass sequence per projection:
+//   init   case sums (||d x||, ||x/d||, ||x||, s==0 closed form) and, from the
+//          previous projection's multiplier lambda, S and S' at the warm start
+//          t0 = s / (1 - lambda) with one Newton step;
+//   newton kBigNewtonPasses passes, each one Newton step (no-ops once done);
+//          the final pass lets its last workgroup finish any remaining steps;
+//   apply  z = x / (1 + lambda d^2), head t, dy and sum dy^2.
+// Last-arrival detection.  Thread 0 wrote this workgroup's partials; its
+// agent-scope acq_rel increment publishes them (release) and, in the last
+// workgroup, makes every other workgroup's partials visible (acquire; the
+// barrier then extends that to the whole workgroup).  Measured on MI355X this
+// costs ~4 us per pass against ~13 us for __threadfence(), which writes back L2.
+__device__ __forceinline__ bool big_last_arrival(unsigned int *count, unsigned int nblk, bool *lds_flag) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned int prev = __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        *lds_flag = (prev == nblk - 1);
+    }
+    __syncthreads();
+    return *lds_flag;
+}
+
+__device__ __forceinline__ void big_newton_update(BigSocState &S, double Ssum, double S1sum) {
+    bool done = false;
+    S.t = soc_newton_step(S.t, Ssum, S1sum, S.tL, S.tR, done);
+    S.iters += 1;
+    S.done = done ? 1 : 0;
+}
+
 template <typename T>
-__global__ __launch_bounds__(1024) void k_soc_block(ConeLayout K, const double *__restrict__ d2,
-                                                    T *__restrict__ y, const T *__restrict__ ys,
-                                                    T *__restrict__ dy, StepState *st, int slot,
-                                                    double *__restrict__ part) {
-    __shared__ double lds[1024 / 64];
+__global__ __launch_bounds__(kBlock) void k_bigsoc_init(ConeLayout K, const double *__restrict__ d2,
+                                                        const T *__restrict__ y, StepState *st, int slot) {
+    __shared__ double lds[kBlock / 64];
+    __shared__ bool flag;
     if (st != nullptr && halted(st, slot)) return;
-    const int64_t cone = blockIdx.x;
+    const int b = blockIdx.x;
+    const int cone = K.big_blk_cone[b];
     const int64_t st0 = K.soc_big_start[cone];
     const int k = K.soc_big_dim[cone] - 1;
-    const T *yt = y + st0 + 1;
-    const double *dt = d2 + st0 + 1;
+    BigSocState *S = K.big_state + cone;
     const double s = (double)y[st0];
-    SocSums z = {0, 0, 0};
-    for (int e = threadIdx.x; e < k; e += blockDim.x) {
-        const double xv = (double)yt[e], dd = dt[e];
-        z.nd2 += dd * xv * xv;
-        z.ni2 += xv * xv / dd;
-        z.nx2 += xv * xv;
+    const double lw = S->lam;
+    double t0 = NAN;
+    if (isfinite(lw) && lw > 0.0 && s != 0.0 && ((s > 0.0 && lw < 1.0) || (s < 0.0 && lw > 1.0))) t0 = s / (1.0 - lw);
+    double v[kBigSums] = {0, 0, 0, 0, 0, 0};
+    const int e0 = K.big_blk_first[b];
+    for (int q = 0; q < kBigElems; ++q) {
+        const int e = e0 + q * kBlock + threadIdx.x;
+        if (e < k) {
+            const double xv = (double)y[st0 + 1 + e], dd = d2[st0 + 1 + e];
+            v[0] += dd * xv * xv;
+            v[1] += xv * xv / dd;
+            v[2] += xv * xv;
+            if (t0 == t0) soc_newton_terms(t0, s, xv, dd, v[3], v[4]);
+            const double qv = xv / (1.0 + dd);
+            v[5] += dd * qv * qv;
+        }
     }
-    z.nd2 = block_sum_all(z.nd2, lds);
-    z.ni2 = block_sum_all(z.ni2, lds);
-    z.nx2 = block_sum_all(z.nx2, lds);
-    const int cs = soc_case(s, z);
-    double t = s, lam = 0.0;
-    if (cs == 2) {
-        double S = 0.0;
-        for (int e = threadIdx.x; e < k; e += blockDim.x) {
-            const double xv = (double)yt[e], dd = dt[e];
-            const double q = xv / (1.0 + dd);
-            S += dd * q * q;
+    double *part = K.big_part + (int64_t)b * kBigSums;
+    for (int q = 0; q < kBigSums; ++q) {
+        const double r = block_sum(v[q], lds);
+        if (threadIdx.x == 0) part[q] = r;
+    }
+    if (!big_last_arrival(K.big_count + cone, (unsigned)K.big_cone_nblk[cone], &flag)) return;
+    double tot[kBigSums];
+    {
+        const int b0 = K.big_cone_blk0[cone], nb = K.big_cone_nblk[cone];
+        for (int q = 0; q < kBigSums; ++q) {
+            double acc = 0.0;
+            for (int j = threadIdx.x; j < nb; j += kBlock)
+                acc += __builtin_nontemporal_load(&K.big_part[(int64_t)(b0 + j) * kBigSums + q]);
+            tot[q] = block_sum(acc, lds);
         }
-        t = sqrt(block_sum_all(S, lds));
-        lam = 1.0;
-    } else if (cs == 3) {
-        double tL = fmax(s, 0.0), tR = sqrt(s * s + z.nx2);
-        t = tL;
-        bool done = false;
-        for (int it = 0; it < 100 && !done; ++it) {
-            double S = 0.0, S1 = 0.0;
-            for (int e = threadIdx.x; e < k; e += blockDim.x) soc_newton_terms(t, s, (double)yt[e], dt[e], S, S1);
-            S = block_sum_all(S, lds);
-            S1 = block_sum_all(S1, lds);
-            t = soc_newton_step(t, S, S1, tL, tR, done);
+    }
+    if (threadIdx.x != 0) return;
+    K.big_count[cone] = 0u;
+    BigSocState L = *S;
+    L.s = s;
+    L.iters = 0;
+    L.done = 1;
+    const SocSums z = {tot[0], tot[1], tot[2]};
+    L.cs = soc_case(s, z);
+    if (L.cs == 2) L.t_closed = sqrt(tot[5]);
+    if (L.cs == 3) {
+        L.tL = fmax(s, 0.0);
+        L.tR = sqrt(s * s + tot[2]);
+        L.done = 0;
+        if (t0 == t0 && t0 > L.tL && t0 < L.tR && tot[3] > 0.0) {
+            L.t = t0;
+            big_newton_update(L, tot[3], tot[4]);
+        } else {
+            L.t = L.tL;
         }
-        lam = 1.0 - s / t;
+    }
+    *S = L;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bigsoc_newton(ConeLayout K, const double *__restrict__ d2,
+                                                          const T *__restrict__ y, StepState *st, int slot,
+                                                          int finish) {
+    __shared__ double lds[kBlock / 64];
+    __shared__ bool flag;
+    if (st != nullptr && halted(st, slot)) return;
+    const int b = blockIdx.x;
+    const int cone = K.big_blk_cone[b];
+    BigSocState *S = K.big_state + cone;
+    if (S->done) return;
+    const int64_t st0 = K.soc_big_start[cone];
+    const int k = K.soc_big_dim[cone] - 1;
+    const double s = S->s, t = S->t;
+    double a = 0.0, c = 0.0;
+    const int e0 = K.big_blk_first[b];
+    for (int q = 0; q < kBigElems; ++q) {
+        const int e = e0 + q * kBlock + threadIdx.x;
+        if (e < k) soc_newton_terms(t, s, (double)y[st0 + 1 + e], d2[st0 + 1 + e], a, c);
+    }
+    a = block_sum(a, lds);
+    c = block_sum(c, lds);
+    double *part = K.big_part + (int64_t)b * kBigSums;
+    if (threadIdx.x == 0) { part[0] = a; part[1] = c; }
+    if (!big_last_arrival(K.big_count + cone, (unsigned)K.big_cone_nblk[cone], &flag)) return;
+    __shared__ BigSocState L;
+    {
+        const int b0 = K.big_cone_blk0[cone], nb = K.big_cone_nblk[cone];
+        double A = 0.0, C = 0.0;
+        for (int j = threadIdx.x; j < nb; j += kBlock) {
+            A += __builtin_nontemporal_load(&K.big_part[(int64_t)(b0 + j) * kBigSums + 0]);
+            C += __builtin_nontemporal_load(&K.big_part[(int64_t)(b0 + j) * kBigSums + 1]);
+        }
+        A = block_sum(A, lds);
+        C = block_sum(C, lds);
+        if (threadIdx.x == 0) {
+            K.big_count[cone] = 0u;
+            L = *S;
+            big_newton_update(L, A, C);
+        }
     }
     __syncthreads();
-    T *yh = y + st0;
-    if (cs == 1) {
-        for (int e = threadIdx.x; e <= k; e += blockDim.x) yh[e] = (T)0.0;
-    } else if (cs >= 2) {
-        for (int e = threadIdx.x; e < k; e += blockDim.x) yh[1 + e] = (T)((double)yh[1 + e] / (1.0 + lam * dt[e]));
-        if (threadIdx.x == 0) yh[0] = (T)t;
+    // final pass: this workgroup completes any remaining Newton steps on its own
+    if (finish) {
+        for (int it = 0; it < 200; ++it) {
+            if (L.done) break;
+            double A = 0.0, C = 0.0;
+            const double tt = L.t;
+            for (int e = threadIdx.x; e < k; e += kBlock) soc_newton_terms(tt, s, (double)y[st0 + 1 + e], d2[st0 + 1 + e], A, C);
+            A = block_sum(A, lds);
+            C = block_sum(C, lds);
+            if (threadIdx.x == 0) big_newton_update(L, A, C);
+            __syncthreads();
+        }
     }
-    __syncthreads();
+    if (threadIdx.x == 0) *S = L;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bigsoc_apply(ConeLayout K, const double *__restrict__ d2,
+                                                         T *__restrict__ y, const T *__restrict__ ys,
+                                                         T *__restrict__ dy, StepState *st, int slot,
+                                                         double *__restrict__ part) {
+    __shared__ double lds[kBlock / 64];
+    if (st != nullptr && halted(st, slot)) return;
+    const int b = blockIdx.x;
+    const int cone = K.big_blk_cone[b];
+    BigSocState *S = K.big_state + cone;
+    const int64_t st0 = K.soc_big_start[cone];
+    const int k = K.soc_big_dim[cone] - 1;
+    const int cs = S->cs;
+    double lam = 0.0, th = S->s;
+    if (cs == 2) { lam = 1.0; th = S->t_closed; }
+    if (cs == 3) { lam = 1.0 - S->s / S->t; th = S->t; }
     double my2 = 0.0;
-    if (dy != nullptr) {
-        for (int e = threadIdx.x; e <= k; e += blockDim.x) {
-            const double d = (double)yh[e] - (double)ys[st0 + e];
-            dy[st0 + e] = (T)d;
+    const int e0 = K.big_blk_first[b];
+    for (int q = 0; q < kBigElems; ++q) {
+        const int e = e0 + q * kBlock + threadIdx.x;
+        if (e < k) {
+            const int64_t i = st0 + 1 + e;
+            double v = (double)y[i];
+            if (cs == 1) v = 0.0;
+            else if (cs >= 2) v = v / (1.0 + lam * d2[i]);
+            const T vt = (T)v;
+            y[i] = vt;
+            if (dy != nullptr) {
+                const double d = (double)vt - (double)ys[i];
+                dy[i] = (T)d;
+                my2 += d * d;
+            }
+        }
+    }
+    if (b == K.big_cone_blk0[cone] && threadIdx.x == 0) {
+        const double hv = cs == 1 ? 0.0 : th;
+        const T ht = (T)hv;
+        y[st0] = ht;
+        if (dy != nullptr) {
+            const double d = (double)ht - (double)ys[st0];
+            dy[st0] = (T)d;
             my2 += d * d;
         }
+        if (cs == 3) S->lam = lam;
     }
     if (part != nullptr) {
         my2 = block_sum(my2, lds);
-        if (threadIdx.x == 0) part[blockIdx.x] = my2;
+        if (threadIdx.x == 0) part[b] = my2;
     }
 }
 
@@ -351,12 +498,13 @@ __global__ __launch_bounds__(kBlock) void k_atdy(Csr<T> Gt, int64_t n, const T *
                                                  StepState *st, int slot, double *__restrict__ part) {
     __shared__ double lds[kBlock / 64];
     if (halted(st, slot)) return;
-    const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int64_t row = gid / L;
-    const int lane = (int)(gid % L);
-    const double acc = row_dot<T, L>(Gt, row, lane, dy);
+    const int lane = (int)(threadIdx.x % L);
     double v = 0.0;
-    if (row < n && lane == 0) v = ((double)x[row] - (double)xs[row]) * acc;
+    for (int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / L; row < n;
+         row += (int64_t)gridDim.x * (kBlock / L)) {
+        const double acc = row_dot<T, L>(Gt, row, lane, dy);
+        if (lane == 0) v += ((double)x[row] - (double)xs[row]) * acc;
+    }
     v = block_sum(v, lds);
     if (threadIdx.x == 0) part[blockIdx.x] = v;
 }
@@ -391,14 +539,14 @@ __device__ void step_continue(StepState *st, int slot) {
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_step(StepState *st, int slot, const double *__restrict__ part,
-                                                 int64_t stride, int n1, int n2, int n3) {
-    __shared__ double lds[kBlock / 64];
+__global__ __launch_bounds__(1024) void k_step(StepState *st, int slot, const double *__restrict__ part,
+                                               int64_t stride, int n1, int n2, int n3) {
+    __shared__ double lds[1024 / 64];
     if (halted(st, slot)) return;
     double a = 0.0, b = 0.0, c = 0.0;
-    for (int i = threadIdx.x; i < n1; i += kBlock) a += part[i];
-    for (int i = threadIdx.x; i < n2; i += kBlock) b += part[stride + i];
-    for (int i = threadIdx.x; i < n3; i += kBlock) c += part[2 * stride + i];
+    for (int i = threadIdx.x; i < n1; i += 1024) a += part[i];
+    for (int i = threadIdx.x; i < n2; i += 1024) b += part[stride + i];
+    for (int i = threadIdx.x; i < n3; i += 1024) c += part[2 * stride + i];
     a = block_sum(a, lds);
     b = block_sum(b, lds);
     c = block_sum(c, lds);
@@ -457,6 +605,10 @@ __global__ __launch_bounds__(kBlock) void k_reflect(Iter<T> it, int64_t n, int64
 }
 
 static inline unsigned nblocks(int64_t work) { return (unsigned)((work + kBlock - 1) / kBlock); }
+unsigned attempt_grid(int64_t rows, int lanes) {
+    const unsigned b = nblocks(rows * lanes);
+    return b < kAttemptMaxGrid ? b : kAttemptMaxGrid;
+}
 
 #define PDCS_LANES(L, CALL) \
     switch (L) {            \
@@ -487,30 +639,63 @@ static void launch_soc(hipStream_t s, const ConeLayout &K, const double *d2, T *
         }
 #undef SOCG
     }
-    if (K.nsoc_big > 0)
-        k_soc_block<T><<<K.nsoc_big, 1024, 0, s>>>(K, d2, y, ys, dy, st, slot,
-                                                   part ? part + (K.nsoc_small > 0 ? soc_blocks(K) : 0) : nullptr);
+    if (K.nsoc_big > 0) {
+        const unsigned nb = (unsigned)K.nbig_blocks;
+        k_bigsoc_init<T><<<nb, kBlock, 0, s>>>(K, d2, y, st, slot);
+        for (int p = 0; p < kBigNewtonPasses; ++p)
+            k_bigsoc_newton<T><<<nb, kBlock, 0, s>>>(K, d2, y, st, slot, p == kBigNewtonPasses - 1);
+        k_bigsoc_apply<T><<<nb, kBlock, 0, s>>>(K, d2, y, ys, dy, st, slot,
+                                                part ? part + (K.nsoc_small > 0 ? soc_blocks(K) : 0) : nullptr);
+    }
 }
+
+template <typename T>
+void launch_attempt_part(int part_id, hipStream_t s, int slot, const Csr<T> &G, const Csr<T> &Gt,
+                         const ProbVec<T> &P, const ConeLayout &K, const double *d2, Iter<T> it, StepState *st,
+                         double *part, int64_t stride) {
+    const unsigned bp = attempt_grid(P.n, Gt.lanes);
+    const unsigned bd = attempt_grid(P.m, G.lanes);
+    switch (part_id) {
+    case 0:
+#define KP(L) k_primal<T, L><<<bp, kBlock, 0, s>>>(Gt, P, it.ys, it.xs, it.x, it.xbar, st, slot, part)
+        PDCS_LANES(Gt.lanes, KP)
+#undef KP
+        break;
+    case 1:
+#define KD(L) k_dual<T, L><<<bd, kBlock, 0, s>>>(G, P, K, it.xbar, it.ys, it.y, it.dy, st, slot, part + stride)
+        PDCS_LANES(G.lanes, KD)
+#undef KD
+        break;
+    case 2:
+        launch_soc<T>(s, K, d2, it.y, it.ys, it.dy, st, slot, part + stride + bd);
+        break;
+    case 3:
+#define KA(L) k_atdy<T, L><<<bp, kBlock, 0, s>>>(Gt, P.n, it.dy, it.x, it.xs, st, slot, part + 2 * stride)
+        PDCS_LANES(Gt.lanes, KA)
+#undef KA
+        break;
+    case 4: {
+        const int ncone = soc_blocks(K) + (K.nsoc_big > 0 ? K.nbig_blocks : 0);
+        k_step<<<1, 1024, 0, s>>>(st, slot, part, stride, (int)bp, (int)(bd + ncone), (int)bp);
+        break;
+    }
+    default:
+        k_reflect<T><<<nblocks(P.n > P.m ? P.n : P.m), kBlock, 0, s>>>(it, P.n, P.m, st, slot);
+        break;
+    }
+}
+template void launch_attempt_part<double>(int, hipStream_t, int, const Csr<double> &, const Csr<double> &,
+                                          const ProbVec<double> &, const ConeLayout &, const double *,
+                                          Iter<double>, StepState *, double *, int64_t);
+template void launch_attempt_part<float>(int, hipStream_t, int, const Csr<float> &, const Csr<float> &,
+                                         const ProbVec<float> &, const ConeLayout &, const double *,
+                                         Iter<float>, StepState *, double *, int64_t);
 
 template <typename T>
 void launch_attempt(hipStream_t s, int slot, const Csr<T> &G, const Csr<T> &Gt, const ProbVec<T> &P,
                     const ConeLayout &K, const double *d2, Iter<T> it, StepState *st, double *part,
                     int64_t stride) {
-    const unsigned bp = nblocks(P.n * Gt.lanes);
-    const unsigned bd = nblocks(P.m * G.lanes);
-#define KP(L) k_primal<T, L><<<bp, kBlock, 0, s>>>(Gt, P, it.ys, it.xs, it.x, it.xbar, st, slot, part)
-    PDCS_LANES(Gt.lanes, KP)
-#undef KP
-#define KD(L) k_dual<T, L><<<bd, kBlock, 0, s>>>(G, P, K, it.xbar, it.ys, it.y, it.dy, st, slot, part + stride)
-    PDCS_LANES(G.lanes, KD)
-#undef KD
-    launch_soc<T>(s, K, d2, it.y, it.ys, it.dy, st, slot, part + stride + bd);
-#define KA(L) k_atdy<T, L><<<bp, kBlock, 0, s>>>(Gt, P.n, it.dy, it.x, it.xs, st, slot, part + 2 * stride)
-    PDCS_LANES(Gt.lanes, KA)
-#undef KA
-    const int ncone = soc_blocks(K) + K.nsoc_big;
-    k_step<<<1, kBlock, 0, s>>>(st, slot, part, stride, (int)bp, (int)(bd + ncone), (int)bp);
-    k_reflect<T><<<nblocks(P.n > P.m ? P.n : P.m), kBlock, 0, s>>>(it, P.n, P.m, st, slot);
+    for (int k = 0; k < kAttemptKernels; ++k) launch_attempt_part<T>(k, s, slot, G, Gt, P, K, d2, it, st, part, stride);
 }
 template void launch_attempt<double>(hipStream_t, int, const Csr<double> &, const Csr<double> &,
                                      const ProbVec<double> &, const ConeLayout &, const double *,

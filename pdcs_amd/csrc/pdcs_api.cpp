@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -52,6 +53,14 @@ int fail(pdcs_ctx *ctx, int code, const char *fmt, ...) {
                         hipGetErrorString(e_), __FILE__, __LINE__);                         \
     } while (0)
 
+// Stream-ordered synchronous copy on the context's stream (the stream is
+// non-blocking, so plain hipMemcpy would not wait for its kernels).
+hipError_t copy_sync(pdcs_ctx *ctx, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, ctx->stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(ctx->stream);
+}
+
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -69,7 +78,11 @@ struct DevBuf {
     T *alloc(int64_t n) {
         void *p = nullptr;
         if (hipMalloc(&p, sizeof(T) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess) return nullptr;
-        hipMemset(p, 0, sizeof(T) * (size_t)std::max<int64_t>(n, 1));
+        if (hipMemset(p, 0, sizeof(T) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess ||
+            hipDeviceSynchronize() != hipSuccess) {
+            hipFree(p);
+            return nullptr;
+        }
         ptrs.push_back(p);
         return (T *)p;
     }
@@ -117,7 +130,6 @@ struct pdcs_problem {
     KktOut *kout = nullptr;
     StepState *st = nullptr;
     ConeLayout K{};
-    int32_t *ss_start = nullptr, *ss_dim = nullptr, *sb_start = nullptr, *sb_dim = nullptr;
     int lanesG = 1, lanesGt = 1;
     IterSet<double> f64;
     IterSet<float> f32;
@@ -164,8 +176,8 @@ void pdcs_params_default(pdcs_params *p) {
     p->bisect_intent = 0;
     p->code_primal_denominator = 0;
     p->use_graph = 1;
-    p->mixed_check_freq = 0;
-    p->polish_check_freq = 0;
+    p->mixed_check_freq = 200;
+    p->polish_check_freq = 200;
     p->mixed_switch_tol = 1e-5;
 }
 
@@ -217,7 +229,75 @@ int validate_cones(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const pdcs_c
 
 template <typename T>
 int upload(pdcs_ctx *ctx, T *dst, const T *src, int64_t n) {
+    if (n <= 0) return PDCS_OK;
     HIPCHK(ctx, hipMemcpyAsync(dst, src, sizeof(T) * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+    return PDCS_OK;
+}
+
+// Device cone layout: SOCs of dimension <= 1024 go to the lane-group kernel,
+// longer ones are split over workgroups of kBigBlockElems tail rows.
+int build_cone_layout(pdcs_ctx *ctx, DevBuf &M, int64_t m, int64_t mzero, int64_t mnonneg,
+                      const std::vector<int64_t> &soc, ConeLayout &K) {
+    std::vector<int32_t> ss_start, ss_dim, sb_start, sb_dim, blk_cone, blk_first, cone_blk0, cone_nblk;
+    int64_t off = mzero + mnonneg;
+    int maxsmall = 0;
+    for (int64_t q : soc) {
+        if (q <= 1024) {
+            ss_start.push_back((int32_t)off);
+            ss_dim.push_back((int32_t)q);
+            maxsmall = std::max<int>(maxsmall, (int)q);
+        } else {
+            const int c = (int)sb_start.size();
+            sb_start.push_back((int32_t)off);
+            sb_dim.push_back((int32_t)q);
+            const int64_t nb = (q - 1 + kBigBlockElems - 1) / kBigBlockElems;
+            cone_blk0.push_back((int32_t)blk_cone.size());
+            cone_nblk.push_back((int32_t)nb);
+            for (int64_t b = 0; b < nb; ++b) {
+                blk_cone.push_back(c);
+                blk_first.push_back((int32_t)(b * kBigBlockElems));
+            }
+        }
+        off += q;
+    }
+    K = ConeLayout{};
+    K.m = m;
+    K.mzero = mzero;
+    K.mnonneg = mnonneg;
+    K.cone_begin = mzero + mnonneg;
+    K.nsoc_small = (int32_t)ss_start.size();
+    K.nsoc_big = (int32_t)sb_start.size();
+    int g = 8;
+    while (g < 64 && g < maxsmall - 1) g <<= 1;
+    K.soc_small_group = g;
+    K.exp_begin = off;
+    auto up32 = [&](const std::vector<int32_t> &v) -> const int32_t * {
+        if (v.empty()) return nullptr;
+        int32_t *d = M.alloc<int32_t>((int64_t)v.size());
+        if (d) upload(ctx, d, v.data(), (int64_t)v.size());
+        return d;
+    };
+    K.soc_small_start = up32(ss_start);
+    K.soc_small_dim = up32(ss_dim);
+    K.soc_big_start = up32(sb_start);
+    K.soc_big_dim = up32(sb_dim);
+    K.nbig_blocks = (int32_t)blk_cone.size();
+    K.big_blk_cone = up32(blk_cone);
+    K.big_blk_first = up32(blk_first);
+    K.big_cone_blk0 = up32(cone_blk0);
+    K.big_cone_nblk = up32(cone_nblk);
+    if (K.nsoc_big > 0) {
+        K.big_state = M.alloc<BigSocState>(K.nsoc_big);
+        K.big_part = M.alloc<double>((int64_t)K.nbig_blocks * kBigSums);
+        K.big_count = M.alloc<unsigned int>(K.nsoc_big);
+        if (!K.big_state || !K.big_part || !K.big_count) return fail(ctx, PDCS_ERR_HIP, "alloc failed");
+        std::vector<BigSocState> init(K.nsoc_big);
+        for (auto &b : init) { std::memset(&b, 0, sizeof(b)); b.lam = NAN; b.done = 1; }
+        upload(ctx, K.big_state, init.data(), K.nsoc_big);
+    }
+    if ((K.nsoc_small && (!K.soc_small_start || !K.soc_small_dim)) || (K.nsoc_big && (!K.soc_big_start || !K.big_blk_cone)))
+        return fail(ctx, PDCS_ERR_HIP, "alloc failed");
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return PDCS_OK;
 }
 
@@ -276,17 +356,17 @@ int alloc_iters(pdcs_problem *P) {
     if (!I.it.x || !I.h2 || !I.bu || !I.yd) return fail(P->ctx, PDCS_ERR_HIP, "iterate allocation failed");
     // scaled data in precision T: c/Dr, h/Dl, bl*Dr, bu*Dr
     std::vector<double> hDr(P->n), hDl(P->m);
-    HIPCHK(P->ctx, hipMemcpy(hDr.data(), P->Dr, sizeof(double) * P->n, hipMemcpyDeviceToHost));
-    HIPCHK(P->ctx, hipMemcpy(hDl.data(), P->Dl, sizeof(double) * P->m, hipMemcpyDeviceToHost));
+    HIPCHK(P->ctx, copy_sync(P->ctx, hDr.data(), P->Dr, sizeof(double) * P->n, hipMemcpyDeviceToHost));
+    HIPCHK(P->ctx, copy_sync(P->ctx, hDl.data(), P->Dl, sizeof(double) * P->m, hipMemcpyDeviceToHost));
     std::vector<T> c(n), h(m), bl(P->nb), bu(P->nb);
     for (int64_t j = 0; j < n; ++j) c[j] = (T)(P->c[j] / hDr[j]);
     for (int64_t i = 0; i < m; ++i) h[i] = (T)(P->h[i] / hDl[i]);
     for (int64_t j = 0; j < P->nb; ++j) { bl[j] = (T)(P->bl[j] * hDr[j]); bu[j] = (T)(P->bu[j] * hDr[j]); }
-    HIPCHK(P->ctx, hipMemcpy(I.c, c.data(), sizeof(T) * n, hipMemcpyHostToDevice));
-    HIPCHK(P->ctx, hipMemcpy(I.h, h.data(), sizeof(T) * m, hipMemcpyHostToDevice));
+    HIPCHK(P->ctx, copy_sync(P->ctx, I.c, c.data(), sizeof(T) * n, hipMemcpyHostToDevice));
+    HIPCHK(P->ctx, copy_sync(P->ctx, I.h, h.data(), sizeof(T) * m, hipMemcpyHostToDevice));
     if (P->nb > 0) {
-        HIPCHK(P->ctx, hipMemcpy(I.bl, bl.data(), sizeof(T) * P->nb, hipMemcpyHostToDevice));
-        HIPCHK(P->ctx, hipMemcpy(I.bu, bu.data(), sizeof(T) * P->nb, hipMemcpyHostToDevice));
+        HIPCHK(P->ctx, copy_sync(P->ctx, I.bl, bl.data(), sizeof(T) * P->nb, hipMemcpyHostToDevice));
+        HIPCHK(P->ctx, copy_sync(P->ctx, I.bu, bu.data(), sizeof(T) * P->nb, hipMemcpyHostToDevice));
     }
     I.ready = true;
     return PDCS_OK;
@@ -367,38 +447,10 @@ int build_problem(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double 
             return fail(ctx, PDCS_ERR_UNSUPPORTED, "problem too large for the residual reduction layout");
         }
     }
-    // cone layout: small SOCs (dim <= 1024) in lane groups, larger ones a workgroup each
-    std::vector<int32_t> ss_start, ss_dim, sb_start, sb_dim;
-    int64_t off = cones->mzero + cones->mnonneg;
-    int maxsmall = 0;
-    for (int64_t b = 0; b < cones->nsoc; ++b) {
-        const int64_t q = cones->soc[b];
-        if (q <= 1024) { ss_start.push_back((int32_t)off); ss_dim.push_back((int32_t)q); maxsmall = std::max<int>(maxsmall, (int)q); }
-        else { sb_start.push_back((int32_t)off); sb_dim.push_back((int32_t)q); }
-        off += q;
+    if ((rc = build_cone_layout(ctx, M, m, cones->mzero, cones->mnonneg, P->soc, P->K))) {
+        delete P;
+        return rc;
     }
-    P->K.m = m;
-    P->K.mzero = cones->mzero;
-    P->K.mnonneg = cones->mnonneg;
-    P->K.cone_begin = cones->mzero + cones->mnonneg;
-    P->K.nsoc_small = (int32_t)ss_start.size();
-    P->K.nsoc_big = (int32_t)sb_start.size();
-    int g = 8;
-    while (g < 64 && g < maxsmall - 1) g <<= 1;
-    P->K.soc_small_group = g;
-    P->K.nexp = 0; P->K.ndexp = 0; P->K.exp_begin = off;
-    if (!ss_start.empty()) {
-        P->ss_start = M.alloc<int32_t>((int64_t)ss_start.size()); P->ss_dim = M.alloc<int32_t>((int64_t)ss_dim.size());
-        upload(ctx, P->ss_start, ss_start.data(), (int64_t)ss_start.size());
-        upload(ctx, P->ss_dim, ss_dim.data(), (int64_t)ss_dim.size());
-    }
-    if (!sb_start.empty()) {
-        P->sb_start = M.alloc<int32_t>((int64_t)sb_start.size()); P->sb_dim = M.alloc<int32_t>((int64_t)sb_dim.size());
-        upload(ctx, P->sb_start, sb_start.data(), (int64_t)sb_start.size());
-        upload(ctx, P->sb_dim, sb_dim.data(), (int64_t)sb_dim.size());
-    }
-    P->K.soc_small_start = P->ss_start; P->K.soc_small_dim = P->ss_dim;
-    P->K.soc_big_start = P->sb_start; P->K.soc_big_dim = P->sb_dim;
     // uploads
     upload(ctx, P->d_rowptr, rowptr, m + 1);
     upload(ctx, P->d_col, colidx, nnz);
@@ -433,7 +485,7 @@ int build_problem(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double 
 // scaling vectors of the dual cone projections (scale_preconditioner!, preprocess.jl:173-297)
 int finish_scaling(pdcs_problem *P) {
     std::vector<double> Dl(P->m), d(P->m, 1.0), d2(P->m);
-    HIPCHK(P->ctx, hipMemcpy(Dl.data(), P->Dl, sizeof(double) * P->m, hipMemcpyDeviceToHost));
+    HIPCHK(P->ctx, copy_sync(P->ctx, Dl.data(), P->Dl, sizeof(double) * P->m, hipMemcpyDeviceToHost));
     int64_t off = P->mzero + P->mnonneg;
     for (int64_t q : P->soc) {
         for (int64_t k = 0; k < q; ++k) d[off + k] = Dl[off] / Dl[off + k];
@@ -441,8 +493,8 @@ int finish_scaling(pdcs_problem *P) {
     }
     for (int64_t k = off; k < P->m; ++k) d[k] = 1.0 / Dl[k];
     for (int64_t i = 0; i < P->m; ++i) d2[i] = d[i] * d[i];
-    HIPCHK(P->ctx, hipMemcpy(P->dsc, d.data(), sizeof(double) * P->m, hipMemcpyHostToDevice));
-    HIPCHK(P->ctx, hipMemcpy(P->dsc2, d2.data(), sizeof(double) * P->m, hipMemcpyHostToDevice));
+    HIPCHK(P->ctx, copy_sync(P->ctx, P->dsc, d.data(), sizeof(double) * P->m, hipMemcpyHostToDevice));
+    HIPCHK(P->ctx, copy_sync(P->ctx, P->dsc2, d2.data(), sizeof(double) * P->m, hipMemcpyHostToDevice));
     return PDCS_OK;
 }
 
@@ -624,6 +676,9 @@ struct Solver {
         return std::sqrt(om * om * c.l2_abs_p * c.l2_abs_p + c.l2_abs_d * c.l2_abs_d / (om * om) + c.abs_gap * c.abs_gap);
     }
     static double max_l2(const ConvInfo &c) { return std::max(std::max(c.l2_rel_p, c.l2_rel_d), c.rel_gap); }
+    static double kkt_readme(const ConvInfo &c) { return std::max(std::max(c.linf_rel_p, c.linf_rel_d), c.rel_gap); }
+    double best_kkt = INFINITY;
+    int n_checks = 0;
 
     // exit_condition_check_diagonal! + check_termination_criteria
     bool exit_check(const T *x, const T *y, int64_t iter) {
@@ -632,6 +687,12 @@ struct Solver {
         if (!rc) rc = conv_info<T>(P, I.it.xm, I.it.ym, ci[1]);
         t_kkt += now_s() - t0;
         if (rc) return true;
+        if (prm->verbose && P->K.nsoc_big > 0 && std::getenv("PDCS_TRACE")) {
+            BigSocState b{};
+            copy_sync(P->ctx, &b, P->K.big_state, sizeof(b), hipMemcpyDeviceToHost);
+            fprintf(stderr, "[pdcs trace] big cone 0: case %d done %d newton iters %d t %.17g lam %.17g s %.17g\n", b.cs,
+                    b.done, b.iters, b.t, b.lam, b.s);
+        }
         if (prm->verbose)
             fprintf(stderr, "[pdcs %s] iter %ld relp %.3e reld %.3e gap %.3e pobj %.10e dobj %.10e restarts %ld\n",
                     sizeof(T) == 4 ? "fp32" : "fp64", (long)iter, ci[0].linf_rel_p, ci[0].linf_rel_d, ci[0].rel_gap,
@@ -820,10 +881,18 @@ struct Solver {
                     if (rc) return rc;
                     max_kkt = max_l2(ci[0]);
                     h.dec_next = std::min(0.9 + 0.1 * std::log10(max_kkt * 1e5), 1.0);
-                    if (stop_tol > 0 && std::max(ci[0].linf_rel_p, std::max(ci[0].linf_rel_d, ci[0].rel_gap)) < stop_tol) {
-                        exit_code = 10;   // phase switch (mixed precision)
-                        done = true;
-                        break;
+                    if (stop_tol > 0) {
+                        // FP32 phase: hand over to the FP64 polish once the README KKT is
+                        // below stop_tol or stops improving (the FP32 rounding floor)
+                        const double kk = std::min(kkt_readme(ci[0]), kkt_readme(ci[1]));
+                        const bool stalled = n_checks >= 2 && kk > 0.5 * best_kkt;
+                        best_kkt = std::min(best_kkt, kk);
+                        n_checks++;
+                        if (kk < stop_tol || stalled) {
+                            exit_code = 10;   // phase switch
+                            done = true;
+                            break;
+                        }
                     }
                     if (h.iter >= max_iter_total) { exit_code = 1; done = true; break; }
                     if ((rc = write_state())) return rc;
@@ -881,8 +950,8 @@ int init_iterates(pdcs_problem *P, const pdcs_params *prm, const double *x0, con
     std::vector<T> hx(n, (T)0), hy(m, (T)0);
     if (prm->warm_start && x0 && y0) {
         std::vector<double> Dr(n), Dl(m);
-        HIPCHK(P->ctx, hipMemcpy(Dr.data(), P->Dr, sizeof(double) * n, hipMemcpyDeviceToHost));
-        HIPCHK(P->ctx, hipMemcpy(Dl.data(), P->Dl, sizeof(double) * m, hipMemcpyDeviceToHost));
+        HIPCHK(P->ctx, copy_sync(P->ctx, Dr.data(), P->Dr, sizeof(double) * n, hipMemcpyDeviceToHost));
+        HIPCHK(P->ctx, copy_sync(P->ctx, Dl.data(), P->Dl, sizeof(double) * m, hipMemcpyDeviceToHost));
         for (int64_t j = 0; j < n; ++j) hx[j] = (T)(x0[j] * Dr[j]);
         for (int64_t i = 0; i < m; ++i) hy[i] = (T)(y0[i] * Dl[i]);
     }
@@ -913,10 +982,10 @@ int store_solution(pdcs_problem *P, const T *x, const T *y) {
     const int64_t n = P->n, m = P->m;
     std::vector<T> hx(n), hy(m);
     std::vector<double> Dr(n), Dl(m);
-    HIPCHK(P->ctx, hipMemcpy(hx.data(), x, sizeof(T) * n, hipMemcpyDeviceToHost));
-    HIPCHK(P->ctx, hipMemcpy(hy.data(), y, sizeof(T) * m, hipMemcpyDeviceToHost));
-    HIPCHK(P->ctx, hipMemcpy(Dr.data(), P->Dr, sizeof(double) * n, hipMemcpyDeviceToHost));
-    HIPCHK(P->ctx, hipMemcpy(Dl.data(), P->Dl, sizeof(double) * m, hipMemcpyDeviceToHost));
+    HIPCHK(P->ctx, copy_sync(P->ctx, hx.data(), x, sizeof(T) * n, hipMemcpyDeviceToHost));
+    HIPCHK(P->ctx, copy_sync(P->ctx, hy.data(), y, sizeof(T) * m, hipMemcpyDeviceToHost));
+    HIPCHK(P->ctx, copy_sync(P->ctx, Dr.data(), P->Dr, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPCHK(P->ctx, copy_sync(P->ctx, Dl.data(), P->Dl, sizeof(double) * m, hipMemcpyDeviceToHost));
     P->sol_x.resize(n);
     P->sol_y.resize(m);
     for (int64_t j = 0; j < n; ++j) P->sol_x[j] = (double)hx[j] / Dr[j];
@@ -927,7 +996,7 @@ int store_solution(pdcs_problem *P, const T *x, const T *y) {
 
 double max_abs_vals(pdcs_problem *P) {
     std::vector<double> v(P->nnz);
-    hipMemcpy(v.data(), P->d_vals, sizeof(double) * P->nnz, hipMemcpyDeviceToHost);
+    copy_sync(P->ctx, v.data(), P->d_vals, sizeof(double) * P->nnz, hipMemcpyDeviceToHost);
     double g = 0.0;
     for (double a : v) g = std::max(g, std::fabs(a));
     return g;
@@ -963,18 +1032,18 @@ int pdcs_problem_create_device(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, 
     if (!ctx || !rowptr) return fail(ctx, PDCS_ERR_ARG, "NULL argument");
     // stage to the host: the transpose and the cone layout are built there
     std::vector<int32_t> hr(m + 1);
-    if (hipMemcpy(hr.data(), rowptr, sizeof(int32_t) * (m + 1), hipMemcpyDeviceToHost) != hipSuccess)
+    if (copy_sync(ctx, hr.data(), rowptr, sizeof(int32_t) * (m + 1), hipMemcpyDeviceToHost) != hipSuccess)
         return fail(ctx, PDCS_ERR_ARG, "rowptr is not a device pointer");
     const int64_t nnz = hr[m];
     std::vector<int32_t> hc(nnz);
     std::vector<double> hv(nnz), hcv(n), hh(m), hbl(nb), hbu(nb);
-    HIPCHK(ctx, hipMemcpy(hc.data(), colidx, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
-    HIPCHK(ctx, hipMemcpy(hv.data(), vals, sizeof(double) * nnz, hipMemcpyDeviceToHost));
-    HIPCHK(ctx, hipMemcpy(hcv.data(), c, sizeof(double) * n, hipMemcpyDeviceToHost));
-    HIPCHK(ctx, hipMemcpy(hh.data(), h, sizeof(double) * m, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, copy_sync(ctx, hc.data(), colidx, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, copy_sync(ctx, hv.data(), vals, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, copy_sync(ctx, hcv.data(), c, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, copy_sync(ctx, hh.data(), h, sizeof(double) * m, hipMemcpyDeviceToHost));
     if (nb > 0) {
-        HIPCHK(ctx, hipMemcpy(hbl.data(), bl, sizeof(double) * nb, hipMemcpyDeviceToHost));
-        HIPCHK(ctx, hipMemcpy(hbu.data(), bu, sizeof(double) * nb, hipMemcpyDeviceToHost));
+        HIPCHK(ctx, copy_sync(ctx, hbl.data(), bl, sizeof(double) * nb, hipMemcpyDeviceToHost));
+        HIPCHK(ctx, copy_sync(ctx, hbu.data(), bu, sizeof(double) * nb, hipMemcpyDeviceToHost));
     }
     return pdcs_problem_create_host(ctx, n, m, nb, hcv.data(), hr.data(), hc.data(), hv.data(), hh.data(), hbl.data(),
                                     hbu.data(), cones, out);
@@ -1003,13 +1072,19 @@ int pdcs_solve(pdcs_problem *P, const pdcs_params *prm_in, const double *x0, con
     if (!P->apart) {
         const int64_t bp = (P->n * P->lanesGt + kBlock - 1) / kBlock;
         const int64_t bd = (P->m * P->lanesG + kBlock - 1) / kBlock +
-                           (P->K.nsoc_small * (int64_t)P->K.soc_small_group + kBlock - 1) / kBlock + P->K.nsoc_big + 8;
+                           (P->K.nsoc_small * (int64_t)P->K.soc_small_group + kBlock - 1) / kBlock + P->K.nbig_blocks + 8;
         P->apart_stride = std::max(bp, bd) + 64;
         P->apart = P->mem.alloc<double>(3 * P->apart_stride);
         if (!P->apart) return fail(ctx, PDCS_ERR_HIP, "alloc failed");
         P->graph_chunk = 64;
     }
     const bool mixed = prm.precision == PDCS_MIXED;
+    // every solve starts from the same root-finder state (no warm start carried over)
+    if (P->K.nsoc_big > 0) {
+        std::vector<BigSocState> init(P->K.nsoc_big);
+        for (auto &b : init) { std::memset(&b, 0, sizeof(b)); b.lam = NAN; b.done = 1; }
+        HIPCHK(ctx, copy_sync(ctx, P->K.big_state, init.data(), sizeof(BigSocState) * init.size(), hipMemcpyHostToDevice));
+    }
     if ((rc = alloc_iters<double>(P))) return rc;
     if (mixed && (rc = alloc_iters<float>(P))) return rc;
     const double ginf = max_abs_vals(P);
@@ -1087,6 +1162,65 @@ int pdcs_solve(pdcs_problem *P, const pdcs_params *prm_in, const double *x0, con
     return PDCS_OK;
 }
 
+}  // extern "C"
+
+namespace {
+template <typename T>
+int time_kernels(pdcs_problem *P, int reps, double *ms, double *bytes) {
+    pdcs_ctx *ctx = P->ctx;
+    hipStream_t s = ctx->stream;
+    IterSet<T> &I = iters<T>(P);
+    if (!I.ready || !P->apart) return fail(ctx, PDCS_ERR_STATE, "run pdcs_solve in this precision first");
+    Csr<T> G{P->d_rowptr, P->d_col, I.vals, P->m, P->lanesG};
+    Csr<T> Gt{P->d_trowptr, P->d_tcol, I.tvals, P->n, P->lanesGt};
+    ProbVec<T> PV{I.c, I.h, I.bl, I.bu, P->n, P->m, P->nb};
+    const double sv = sizeof(T), n = (double)P->n, m = (double)P->m, nnz = (double)P->nnz, nb = (double)P->nb;
+    const double melem = (double)(P->mzero + P->mnonneg), mc = m - melem;
+    const double b[kAttemptKernels] = {
+        nnz * (sv + 4) + 4 * (n + 1) + sv * (m + 4 * n + 2 * nb),
+        nnz * (sv + 4) + 4 * (m + 1) + sv * (n + 3 * m + melem),
+        mc * (4 * sv + 8) + 8.0 * (double)(P->K.nsoc_small + P->K.nsoc_big),
+        nnz * (sv + 4) + 4 * (n + 1) + sv * (m + 2 * n),
+        8.0 * 3 * (double)P->apart_stride,
+        sv * 5 * (n + m)};
+    hipEvent_t e0, e1;
+    HIPCHK(ctx, hipEventCreate(&e0));
+    HIPCHK(ctx, hipEventCreate(&e1));
+    for (int k = 0; k < kAttemptKernels; ++k) {
+        StepState &h = *P->h_st;
+        HIPCHK(ctx, hipMemcpyAsync(&h, P->st, sizeof(StepState), hipMemcpyDeviceToHost, s));
+        HIPCHK(ctx, hipStreamSynchronize(s));
+        h.stop_at = (int64_t)1 << 62;
+        h.reflect_slot = 0;
+        h.check_freq = (int64_t)1 << 40;
+        h.restart_freq = 0;
+        h.kkt_restart_freq = 0;
+        HIPCHK(ctx, hipMemcpyAsync(P->st, &h, sizeof(StepState), hipMemcpyHostToDevice, s));
+        launch_attempt_part<T>(k, s, 0, G, Gt, PV, P->K, P->dsc2, I.it, P->st, P->apart, P->apart_stride);  // warm
+        HIPCHK(ctx, hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r)
+            launch_attempt_part<T>(k, s, 0, G, Gt, PV, P->K, P->dsc2, I.it, P->st, P->apart, P->apart_stride);
+        HIPCHK(ctx, hipEventRecord(e1, s));
+        HIPCHK(ctx, hipEventSynchronize(e1));
+        float el = 0.f;
+        HIPCHK(ctx, hipEventElapsedTime(&el, e0, e1));
+        if (ms) ms[k] = (double)el / reps;
+        if (bytes) bytes[k] = b[k];
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    HIPCHK(ctx, hipGetLastError());
+    return PDCS_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int pdcs_time_kernels(pdcs_problem *P, int32_t precision, int32_t reps, double *ms, double *bytes) {
+    if (!P || reps <= 0) return fail(P ? P->ctx : nullptr, PDCS_ERR_ARG, "bad argument");
+    return precision == PDCS_FP64 ? time_kernels<double>(P, reps, ms, bytes) : time_kernels<float>(P, reps, ms, bytes);
+}
+
 int pdcs_get_solution(pdcs_problem *P, double *x, double *y, double *s) {
     if (!P) return fail(nullptr, PDCS_ERR_ARG, "problem is NULL");
     if (!P->solved) return fail(P->ctx, PDCS_ERR_STATE, "no solution: call pdcs_solve first");
@@ -1103,8 +1237,8 @@ int pdcs_get_solution(pdcs_problem *P, double *x, double *y, double *s) {
 
 int pdcs_get_scaling(pdcs_problem *P, double *Dr, double *Dl) {
     if (!P) return fail(nullptr, PDCS_ERR_ARG, "problem is NULL");
-    if (Dr) HIPCHK(P->ctx, hipMemcpy(Dr, P->Dr, sizeof(double) * P->n, hipMemcpyDeviceToHost));
-    if (Dl) HIPCHK(P->ctx, hipMemcpy(Dl, P->Dl, sizeof(double) * P->m, hipMemcpyDeviceToHost));
+    if (Dr) HIPCHK(P->ctx, copy_sync(P->ctx, Dr, P->Dr, sizeof(double) * P->n, hipMemcpyDeviceToHost));
+    if (Dl) HIPCHK(P->ctx, copy_sync(P->ctx, Dl, P->Dl, sizeof(double) * P->m, hipMemcpyDeviceToHost));
     return PDCS_OK;
 }
 
@@ -1113,12 +1247,12 @@ int pdcs_evaluate(pdcs_problem *P, const double *x, const double *y, pdcs_result
     int rc = alloc_iters<double>(P);
     if (rc) return rc;
     std::vector<double> Dr(P->n), Dl(P->m), xs(P->n), ys(P->m);
-    HIPCHK(P->ctx, hipMemcpy(Dr.data(), P->Dr, sizeof(double) * P->n, hipMemcpyDeviceToHost));
-    HIPCHK(P->ctx, hipMemcpy(Dl.data(), P->Dl, sizeof(double) * P->m, hipMemcpyDeviceToHost));
+    HIPCHK(P->ctx, copy_sync(P->ctx, Dr.data(), P->Dr, sizeof(double) * P->n, hipMemcpyDeviceToHost));
+    HIPCHK(P->ctx, copy_sync(P->ctx, Dl.data(), P->Dl, sizeof(double) * P->m, hipMemcpyDeviceToHost));
     for (int64_t j = 0; j < P->n; ++j) xs[j] = x[j] * Dr[j];
     for (int64_t i = 0; i < P->m; ++i) ys[i] = y[i] * Dl[i];
-    HIPCHK(P->ctx, hipMemcpy(P->f64.tn1, xs.data(), sizeof(double) * P->n, hipMemcpyHostToDevice));
-    HIPCHK(P->ctx, hipMemcpy(P->f64.tm1, ys.data(), sizeof(double) * P->m, hipMemcpyHostToDevice));
+    HIPCHK(P->ctx, copy_sync(P->ctx, P->f64.tn1, xs.data(), sizeof(double) * P->n, hipMemcpyHostToDevice));
+    HIPCHK(P->ctx, copy_sync(P->ctx, P->f64.tm1, ys.data(), sizeof(double) * P->m, hipMemcpyHostToDevice));
     ConvInfo c{};
     if ((rc = conv_info<double>(P, P->f64.tn1, P->f64.tm1, c))) return rc;
     std::memset(res, 0, sizeof(*res));
@@ -1130,7 +1264,7 @@ int pdcs_spmv(pdcs_ctx *ctx, int32_t precision, int64_t m, int64_t n, const int3
               const void *vals, const void *x, void *y) {
     if (!ctx || !rowptr || !colidx || !vals || !x || !y) return fail(ctx, PDCS_ERR_ARG, "NULL argument");
     int32_t nnz = 0;
-    HIPCHK(ctx, hipMemcpy(&nnz, rowptr + m, sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHK(ctx, copy_sync(ctx, &nnz, rowptr + m, sizeof(int32_t), hipMemcpyDeviceToHost));
     const int L = lanes_for(nnz, m);
     if (precision == PDCS_FP64)
         launch_spmv<double>(ctx->stream, Csr<double>{rowptr, colidx, (const double *)vals, m, L}, (const double *)x, (double *)y);
@@ -1147,32 +1281,15 @@ int pdcs_project_dual(pdcs_ctx *ctx, int32_t precision, const pdcs_cones *cones,
     int rc = validate_cones(ctx, 1, m, 0, cones);
     if (rc) return rc;
     DevBuf M;
-    std::vector<int32_t> ss_start, ss_dim, sb_start, sb_dim;
-    int64_t off = cones->mzero + cones->mnonneg;
-    int maxsmall = 0;
-    for (int64_t b = 0; b < cones->nsoc; ++b) {
-        const int64_t q = cones->soc[b];
-        if (q <= 1024) { ss_start.push_back((int32_t)off); ss_dim.push_back((int32_t)q); maxsmall = std::max<int>(maxsmall, (int)q); }
-        else { sb_start.push_back((int32_t)off); sb_dim.push_back((int32_t)q); }
-        off += q;
-    }
     ConeLayout K{};
-    K.m = m; K.mzero = cones->mzero; K.mnonneg = cones->mnonneg; K.cone_begin = cones->mzero + cones->mnonneg;
-    K.nsoc_small = (int32_t)ss_start.size(); K.nsoc_big = (int32_t)sb_start.size();
-    int g = 8;
-    while (g < 64 && g < maxsmall - 1) g <<= 1;
-    K.soc_small_group = g;
-    int32_t *a = M.alloc<int32_t>((int64_t)ss_start.size()), *b = M.alloc<int32_t>((int64_t)ss_dim.size());
-    int32_t *c = M.alloc<int32_t>((int64_t)sb_start.size()), *e = M.alloc<int32_t>((int64_t)sb_dim.size());
+    std::vector<int64_t> soc(cones->soc, cones->soc + cones->nsoc);
+    if ((rc = build_cone_layout(ctx, M, m, cones->mzero, cones->mnonneg, soc, K))) return rc;
     double *d2 = M.alloc<double>(m);
-    if (!a || !b || !c || !e || !d2) return fail(ctx, PDCS_ERR_HIP, "alloc failed");
-    if (!ss_start.empty()) { HIPCHK(ctx, hipMemcpy(a, ss_start.data(), 4 * ss_start.size(), hipMemcpyHostToDevice)); HIPCHK(ctx, hipMemcpy(b, ss_dim.data(), 4 * ss_dim.size(), hipMemcpyHostToDevice)); }
-    if (!sb_start.empty()) { HIPCHK(ctx, hipMemcpy(c, sb_start.data(), 4 * sb_start.size(), hipMemcpyHostToDevice)); HIPCHK(ctx, hipMemcpy(e, sb_dim.data(), 4 * sb_dim.size(), hipMemcpyHostToDevice)); }
-    K.soc_small_start = a; K.soc_small_dim = b; K.soc_big_start = c; K.soc_big_dim = e;
+    if (!d2) return fail(ctx, PDCS_ERR_HIP, "alloc failed");
     std::vector<double> hd(m);
-    HIPCHK(ctx, hipMemcpy(hd.data(), d, sizeof(double) * m, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, copy_sync(ctx, hd.data(), d, sizeof(double) * m, hipMemcpyDeviceToHost));
     for (auto &v : hd) v = v * v;
-    HIPCHK(ctx, hipMemcpy(d2, hd.data(), sizeof(double) * m, hipMemcpyHostToDevice));
+    HIPCHK(ctx, copy_sync(ctx, d2, hd.data(), sizeof(double) * m, hipMemcpyHostToDevice));
     if (precision == PDCS_FP64) launch_project_dual<double>(ctx->stream, K, d, d2, (double *)y);
     else launch_project_dual<float>(ctx->stream, K, d, d2, (float *)y);
     HIPCHK(ctx, hipGetLastError());
@@ -1185,16 +1302,9 @@ int pdcs_project_primal_cone(pdcs_ctx *ctx, const pdcs_cones *cones, int64_t m, 
     int rc = validate_cones(ctx, 1, m, 0, cones);
     if (rc) return rc;
     DevBuf M;
-    std::vector<int32_t> st, dm;
-    int64_t off = cones->mzero + cones->mnonneg;
-    for (int64_t b = 0; b < cones->nsoc; ++b) { st.push_back((int32_t)off); dm.push_back((int32_t)cones->soc[b]); off += cones->soc[b]; }
     ConeLayout K{};
-    K.m = m; K.mzero = cones->mzero; K.mnonneg = cones->mnonneg; K.cone_begin = cones->mzero + cones->mnonneg;
-    K.nsoc_small = (int32_t)st.size();
-    int32_t *a = M.alloc<int32_t>((int64_t)st.size()), *b = M.alloc<int32_t>((int64_t)dm.size());
-    if (!a || !b) return fail(ctx, PDCS_ERR_HIP, "alloc failed");
-    if (!st.empty()) { HIPCHK(ctx, hipMemcpy(a, st.data(), 4 * st.size(), hipMemcpyHostToDevice)); HIPCHK(ctx, hipMemcpy(b, dm.data(), 4 * dm.size(), hipMemcpyHostToDevice)); }
-    K.soc_small_start = a; K.soc_small_dim = b;
+    std::vector<int64_t> soc(cones->soc, cones->soc + cones->nsoc);
+    if ((rc = build_cone_layout(ctx, M, m, cones->mzero, cones->mnonneg, soc, K))) return rc;
     launch_project_primal_cone(ctx->stream, K, r);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));

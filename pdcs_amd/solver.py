@@ -10,7 +10,9 @@ Every solve runs on the GPU through libpdcs_hip.so.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 from types import SimpleNamespace
 
 import numpy as np
@@ -27,6 +29,19 @@ def _p(a, t=C.c_double):
     return a.ctypes.data_as(C.POINTER(t))
 
 
+_live_problems: "weakref.WeakSet" = weakref.WeakSet()
+_live_contexts: "weakref.WeakSet" = weakref.WeakSet()
+
+
+@atexit.register
+def _release_all():
+    """Free device objects before the HIP runtime is torn down at exit."""
+    for p in list(_live_problems):
+        p.close()
+    for c in list(_live_contexts):
+        c.close()
+
+
 class Context:
     """A HIP stream on one device (one per host thread)."""
 
@@ -34,8 +49,12 @@ class Context:
         self.L = _lib.load()
         self.h = C.c_void_p()
         _lib.check(self.L.pdcs_ctx_create(device, C.byref(self.h)))
+        _live_contexts.add(self)
 
     def close(self):
+        for p in list(_live_problems):
+            if p.ctx is self:
+                p.close()
         if self.h:
             self.L.pdcs_ctx_destroy(self.h)
             self.h = C.c_void_p()
@@ -85,6 +104,7 @@ class Problem:
         _lib.check(ctx.L.pdcs_problem_create_host(ctx.h, self.n, self.m, self.nb, _p(a[0]), _p(a[1], C.c_int32),
                                                   _p(a[2], C.c_int32), _p(a[3]), _p(a[4]), _p(a[5]), _p(a[6]),
                                                   C.byref(cones), C.byref(self.h)), ctx.h)
+        _live_problems.add(self)
 
     @classmethod
     def from_instance(cls, P, ctx: Context | None = None):
@@ -124,10 +144,19 @@ class Problem:
                    self.ctx.h)
         return res.as_dict()
 
+    KERNELS = ["primal_step", "dual_step", "cone_projection", "atdy_dot", "step_control", "reflection"]
+
+    def time_kernels(self, precision: int = 0, reps: int = 50):
+        """Mean ms and algorithmic bytes per launch of each attempt kernel (after a solve)."""
+        ms = np.zeros(6)
+        by = np.zeros(6)
+        _lib.check(self.ctx.L.pdcs_time_kernels(self.h, precision, reps, _p(ms), _p(by)), self.ctx.h)
+        return {k: {"ms": float(ms[i]), "bytes": float(by[i])} for i, k in enumerate(self.KERNELS)}
+
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             self.ctx.L.pdcs_problem_destroy(self.h)
-            self.h = C.c_void_p()
+        self.h = C.c_void_p()
 
     def __del__(self):
         try:

@@ -50,7 +50,7 @@ def _random_cones(rng, dims):
     return y, d
 
 
-@pytest.mark.parametrize("dims", [[10] * 200, [3, 7, 64, 65, 200, 1024], [5000], [200001]])
+@pytest.mark.parametrize("dims", [[10] * 200, [3, 7, 64, 65, 200, 1024], [1025, 2049, 5000, 10], [200001]])
 def test_project_dual_scaled_soc_matches_oracle(ctx, dims):
     rng = np.random.default_rng(len(dims) + dims[0])
     y, d = _random_cones(rng, dims)
@@ -75,8 +75,14 @@ def test_project_dual_scaled_soc_matches_oracle(ctx, dims):
     got = t.cpu().numpy()
     off = 0
     for q in dims:
-        a, b = got[off:off + q], ref[off:off + q]
-        assert np.linalg.norm(a - b) <= 1e-9 * (1 + np.linalg.norm(y[off:off + q])), (q, a[:3], b[:3])
+        a, b, v, dd = got[off:off + q], ref[off:off + q], y[off:off + q], d[off:off + q]
+        sc = 1 + np.linalg.norm(v)
+        assert np.linalg.norm(a - b) <= 1e-9 * sc, (q, off, a[:3], b[:3])
+        # projection optimality of the GPU result itself: a in K_d, v - a in the polar, complementarity
+        w = v - a
+        assert np.linalg.norm(dd[1:] * a[1:]) - a[0] <= 1e-12 * sc
+        assert np.linalg.norm(w[1:] / dd[1:]) + w[0] <= 1e-12 * sc
+        assert abs(a @ w) <= 1e-12 * sc * sc
         off += q
 
 
@@ -136,6 +142,20 @@ def test_solve_workload0_fp64(ctx, graph):
     x, y, _ = prob.solution()
     _check_solution(P, res, x, y, ref)
     assert res["iter"] == ref["iter"]
+
+
+def test_trajectory_tracks_oracle(ctx):
+    """The first PDHG attempts match the oracle iterate for iterate (same
+    scaling, step-size, extrapolation and projection arithmetic in FP64)."""
+    P = instances.workload0()
+    prob = Problem.from_instance(P, ctx)
+    for K, tol in [(1, 1e-8), (2, 1e-8), (5, 1e-8), (10, 1e-8), (100, 1e-5)]:
+        r1, x1, y1 = oracle.solve(P, check_terminate_freq=K, max_outer_iter=1, max_inner_iter=K)
+        r2 = prob.solve(check_terminate_freq=K, max_outer_iter=1, max_inner_iter=K, use_graph=0)
+        x2, y2, _ = prob.solution()
+        assert r1["iter"] == r2["iter"] == K
+        assert np.abs(x1 - x2).max() <= tol * (1 + np.abs(x1).max()), K
+        assert np.abs(y1 - y2).max() <= tol * (1 + np.abs(y1).max()), K
 
 
 def test_solve_workload0_mixed(ctx):
