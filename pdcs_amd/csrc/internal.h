@@ -24,18 +24,18 @@ struct Csr {
 // Per-cone scalar state of the multi-workgroup scaled-SOC root find.
 struct BigSocState {
     double s;          // head of the point being projected
-    double t, tL, tR;  // Newton iterate and bracket of the head t
-    double lam;        // multiplier of the last projection (warm start)
+    double t, tL, tR;  // root-find iterate and bracket of the projected head t
+    double t_prev;     // projected head of the previous projection (warm start), NaN if none
     double t_closed;   // head for the s == 0 closed form
-    int32_t cs;        // case: 0 interior, 1 polar, 2 s == 0, 3 Newton
+    int32_t cs;        // case: 0 interior, 1 polar, 2 s == 0, 3 root find
     int32_t done;
-    int32_t iters;
-    int32_t pad;
+    int32_t iters;     // exact Newton steps taken by the fallback
+    int32_t series;    // 1 if the series solve was accepted
 };
 constexpr int kBigElems = 4;                       // tail elements per thread in a pass
 constexpr int kBigBlockElems = kBigElems * 256;    // tail elements per workgroup
-constexpr int kBigSums = 6;
-constexpr int kBigNewtonPasses = 4;
+constexpr int kBigMoments = 12;                    // series terms M_0..M_{K}, K = kBigMoments - 1
+constexpr int kBigSums = 4 + kBigMoments + 1;      // case sums, moments, max e_i
 
 // Cone layout of the dual rows on the device.
 struct ConeLayout {
@@ -164,7 +164,8 @@ struct KktData {
     const double *c_raw, *h_raw, *blf, *buf, *laml, *lamu, *Dr, *Dl;
     int64_t n, m, nb;
     ConeLayout K;
-    double *rtmp;   // m scratch
+    double *rtmp;          // m scratch
+    double *scratch_big;   // nbig_blocks scratch (large-cone norms)
 };
 struct KktOut {
     double pobj, dobj, atyInf, atyN1, sInf, sN1, d2, dInf, axInf, axN1, p2, pInf, projInf;

@@ -77,6 +77,26 @@ __device__ __forceinline__ double block_sum_all(double v, double *lds) {
     return r;
 }
 
+// Block-wide reduction of N values in one LDS round (kBlock threads): entries
+// with ismax[q] take the maximum, the others the sum; results broadcast to all.
+template <int N>
+__device__ __forceinline__ void block_reduce_multi(double (&v)[N], const bool (&ismax)[N], double *lds /* 4*N */) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) v[q] = ismax[q] ? wave_max(v[q]) : wave_sum(v[q]);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < N; ++q) lds[w * N + q] = v[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        double r = lds[q];
+        for (int i = 1; i < kBlock / 64; ++i) r = ismax[q] ? fmax(r, lds[i * N + q]) : r + lds[i * N + q];
+        v[q] = r;
+    }
+    __syncthreads();
+}
+
 template <typename T, int L>
 __device__ __forceinline__ double row_dot(const Csr<T> &A, int64_t row, int lane, const T *x) {
     double acc = 0.0;
@@ -283,16 +303,19 @@ __global__ __launch_bounds__(kBlock) void k_soc_group(ConeLayout K, const double
 
 // ------------------------------------------------ large scaled SOCs --
 // A cone longer than 1024 rows is split over workgroups of kBigBlockElems
-// tail elements.  Each pass writes per-workgroup partial sums; the last
-// workgroup to arrive (device-scope counter) reduces them in fixed order and
-// runs the scalar Newton update of phi(t) (see the small-cone kernel), so a
-// pass never waits on another workgroup.  Pass sequence per projection:
-//   init   case sums (||d x||, ||x/d||, ||x||, s==0 closed form) and, from the
-//          previous projection's multiplier lambda, S and S' at the warm start
-//          t0 = s / (1 - lambda) with one Newton step;
-//   newton kBigNewtonPasses passes, each one Newton step (no-ops once done);
-//          the final pass lets its last workgroup finish any remaining steps;
-//   apply  z = x / (1 + lambda d^2), head t, dy and sum dy^2.
+// tail rows.  One pass (k_bigsoc_init) computes, per workgroup, the case sums
+// (||d x||^2, ||x/d||^2, ||x||^2, the s == 0 closed form) and the moments of
+// S around the previous projection's head t0:
+//   w_i(t0 + h) = w_i0 (1 + h e_i),  w_i0 = t0 (1 + d_i^2) - s d_i^2,  e_i = (1 + d_i^2) / w_i0
+//   S(t0 + h) = sum_k (k+1) (-h)^k M_k,   M_k = sum_i a_i e_i^k / w_i0^2,
+// and max_i e_i.  The last workgroup to arrive reduces the partials in fixed
+// order and solves phi = 0 on the truncated series (safeguarded Newton, all
+// scalar).  With K = kBigMoments terms the relative truncation error of S is
+// at most (K+1) r^K / (1-r)^2, r = |h| max e_i; below 1e-16 the root is accepted.  Otherwise (no warm
+// start, or a far move such as the first attempt after a restart) the
+// single-workgroup k_bigsoc_finish runs exact Newton steps to convergence.
+// k_bigsoc_apply then writes z = x / (1 + lambda d^2) and the head.
+
 // Last-arrival detection.  Thread 0 wrote this workgroup's partials; its
 // agent-scope acq_rel increment publishes them (release) and, in the last
 // workgroup, makes every other workgroup's partials visible (acquire; the
@@ -315,128 +338,169 @@ __device__ __forceinline__ void big_newton_update(BigSocState &S, double Ssum, d
     S.done = done ? 1 : 0;
 }
 
+// Series solve of phi(t0 + h) = 0 from the moments; returns false if the
+// truncation bound is not met (the caller then falls back to exact Newton).
+__device__ bool big_series_solve(BigSocState &L, double t0, const double *M, double emax) {
+    double c[kBigMoments];
+    for (int k = 0; k < kBigMoments; ++k) c[k] = (double)(k + 1) * ((k & 1) ? -M[k] : M[k]);
+    const double rad = 0.5 / emax;
+    double hL = fmax(L.tL - t0, -rad), hR = fmin(L.tR - t0, rad);
+    if (!(hL < hR)) return false;
+    double h = fmin(fmax(0.0, hL), hR);
+    bool ok = false;
+    for (int it = 0; it < 60; ++it) {
+        double S = c[kBigMoments - 1], dS = 0.0;
+        for (int k = kBigMoments - 2; k >= 0; --k) { dS = dS * h + S; S = S * h + c[k]; }
+        if (!(S > 0.0)) break;
+        const double isq = 1.0 / sqrt(S);
+        const double phi = isq - 1.0;
+        if (phi == 0.0) { ok = true; break; }
+        if (phi < 0.0) hL = fmax(hL, h); else hR = fmin(hR, h);
+        const double dphi = -0.5 * dS * isq * isq * isq;
+        double hn = h - phi / dphi;
+        bool newton = true;
+        if (!(dphi > 0.0) || !(hn >= hL && hn <= hR)) { hn = 0.5 * (hL + hR); newton = false; }
+        const bool conv = (newton && fabs(hn - h) <= 1e-15 * fabs(t0 + hn)) || (hR - hL) <= 4.5e-16 * fabs(t0 + hR);
+        h = hn;
+        if (conv) { ok = true; break; }
+    }
+    // the series root (clipped to the bracket and the convergence radius) is the
+    // expansion point of the next pass even when it is not accepted
+    L.t = fmin(fmax(t0 + h, L.tL), L.tR);
+    if (!ok) return false;
+    const double r = fabs(h) * emax;
+    if (!(r < 0.5)) return false;
+    // remainder of the dropped terms k >= kBigMoments: sum (k+1) r^k <= (kBigMoments+1) r^kBigMoments / (1-r)^2
+    double rk = 1.0;
+    for (int k = 0; k < kBigMoments; ++k) rk *= r;
+    if ((double)(kBigMoments + 1) * rk / ((1.0 - r) * (1.0 - r)) > 1e-16) return false;
+    L.t = t0 + h;
+    return true;
+}
+
+// One moment pass.  refine == 0: first pass of a projection (case sums and
+// moments at the previous head t_prev); refine == 1: moments at the current
+// iterate L.t of a projection whose series solve was not accepted (no-op
+// once done).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bigsoc_init(ConeLayout K, const double *__restrict__ d2,
-                                                        const T *__restrict__ y, StepState *st, int slot) {
-    __shared__ double lds[kBlock / 64];
+__global__ __launch_bounds__(kBlock) void k_bigsoc_pass(ConeLayout K, const double *__restrict__ d2,
+                                                        const T *__restrict__ y, StepState *st, int slot, int refine) {
+    __shared__ double lds[(kBlock / 64) * kBigSums];
     __shared__ bool flag;
     if (st != nullptr && halted(st, slot)) return;
     const int b = blockIdx.x;
     const int cone = K.big_blk_cone[b];
+    BigSocState *S = K.big_state + cone;
+    if (refine && S->done) return;
     const int64_t st0 = K.soc_big_start[cone];
     const int k = K.soc_big_dim[cone] - 1;
-    BigSocState *S = K.big_state + cone;
-    const double s = (double)y[st0];
-    const double lw = S->lam;
-    double t0 = NAN;
-    if (isfinite(lw) && lw > 0.0 && s != 0.0 && ((s > 0.0 && lw < 1.0) || (s < 0.0 && lw > 1.0))) t0 = s / (1.0 - lw);
-    double v[kBigSums] = {0, 0, 0, 0, 0, 0};
+    const double s = refine ? S->s : (double)y[st0];
+    double t0;
+    if (refine) {
+        t0 = S->t;
+    } else {
+        const double tp = S->t_prev;
+        t0 = (isfinite(tp) && tp > fmax(s, 0.0)) ? tp : NAN;
+    }
+    double v[kBigSums];
+    bool ismax[kBigSums];
+#pragma unroll
+    for (int q = 0; q < kBigSums; ++q) { v[q] = 0.0; ismax[q] = (q == kBigSums - 1); }
     const int e0 = K.big_blk_first[b];
     for (int q = 0; q < kBigElems; ++q) {
         const int e = e0 + q * kBlock + threadIdx.x;
         if (e < k) {
             const double xv = (double)y[st0 + 1 + e], dd = d2[st0 + 1 + e];
-            v[0] += dd * xv * xv;
-            v[1] += xv * xv / dd;
-            v[2] += xv * xv;
-            if (t0 == t0) soc_newton_terms(t0, s, xv, dd, v[3], v[4]);
-            const double qv = xv / (1.0 + dd);
-            v[5] += dd * qv * qv;
+            const double a = dd * xv * xv;
+            if (!refine) {
+                v[0] += a;
+                v[1] += xv * xv / dd;
+                v[2] += xv * xv;
+                const double qv = xv / (1.0 + dd);
+                v[3] += dd * qv * qv;
+            }
+            if (t0 == t0) {
+                const double bb = 1.0 + dd;
+                const double iw = 1.0 / (t0 * bb - s * dd);
+                const double ei = bb * iw;
+                double term = a * iw * iw;
+#pragma unroll
+                for (int m = 0; m < kBigMoments; ++m) { v[4 + m] += term; term *= ei; }
+                v[4 + kBigMoments] = fmax(v[4 + kBigMoments], ei);
+            }
         }
     }
+    block_reduce_multi<kBigSums>(v, ismax, lds);
     double *part = K.big_part + (int64_t)b * kBigSums;
-    for (int q = 0; q < kBigSums; ++q) {
-        const double r = block_sum(v[q], lds);
-        if (threadIdx.x == 0) part[q] = r;
-    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < kBigSums; ++q) part[q] = v[q];
     if (!big_last_arrival(K.big_count + cone, (unsigned)K.big_cone_nblk[cone], &flag)) return;
     double tot[kBigSums];
     {
         const int b0 = K.big_cone_blk0[cone], nb = K.big_cone_nblk[cone];
-        for (int q = 0; q < kBigSums; ++q) {
-            double acc = 0.0;
-            for (int j = threadIdx.x; j < nb; j += kBlock)
-                acc += __builtin_nontemporal_load(&K.big_part[(int64_t)(b0 + j) * kBigSums + q]);
-            tot[q] = block_sum(acc, lds);
+#pragma unroll
+        for (int q = 0; q < kBigSums; ++q) tot[q] = 0.0;
+        for (int j = threadIdx.x; j < nb; j += kBlock) {
+#pragma unroll
+            for (int q = 0; q < kBigSums; ++q) {
+                const double pv = __builtin_nontemporal_load(&K.big_part[(int64_t)(b0 + j) * kBigSums + q]);
+                tot[q] = ismax[q] ? fmax(tot[q], pv) : tot[q] + pv;
+            }
         }
+        block_reduce_multi<kBigSums>(tot, ismax, lds);
     }
     if (threadIdx.x != 0) return;
     K.big_count[cone] = 0u;
     BigSocState L = *S;
-    L.s = s;
-    L.iters = 0;
-    L.done = 1;
-    const SocSums z = {tot[0], tot[1], tot[2]};
-    L.cs = soc_case(s, z);
-    if (L.cs == 2) L.t_closed = sqrt(tot[5]);
-    if (L.cs == 3) {
-        L.tL = fmax(s, 0.0);
-        L.tR = sqrt(s * s + tot[2]);
-        L.done = 0;
-        if (t0 == t0 && t0 > L.tL && t0 < L.tR && tot[3] > 0.0) {
-            L.t = t0;
-            big_newton_update(L, tot[3], tot[4]);
-        } else {
+    if (!refine) {
+        L.s = s;
+        L.iters = 0;
+        L.series = 0;
+        L.done = 1;
+        const SocSums z = {tot[0], tot[1], tot[2]};
+        L.cs = soc_case(s, z);
+        if (L.cs == 2) L.t_closed = sqrt(tot[3]);
+        if (L.cs == 3) {
+            L.tL = fmax(s, 0.0);
+            L.tR = sqrt(s * s + tot[2]);
+            L.done = 0;
             L.t = L.tL;
+        }
+    }
+    if (L.cs == 3 && !L.done && t0 == t0 && t0 < L.tR && tot[4] > 0.0) {
+        if (big_series_solve(L, t0, tot + 4, tot[4 + kBigMoments])) {
+            L.done = 1;
+            L.series = 1;
         }
     }
     *S = L;
 }
 
+// Fallback: one workgroup finishes the exact Newton iteration (no-op when done).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bigsoc_newton(ConeLayout K, const double *__restrict__ d2,
-                                                          const T *__restrict__ y, StepState *st, int slot,
-                                                          int finish) {
-    __shared__ double lds[kBlock / 64];
-    __shared__ bool flag;
+__global__ __launch_bounds__(1024) void k_bigsoc_finish(ConeLayout K, const double *__restrict__ d2,
+                                                        const T *__restrict__ y, StepState *st, int slot) {
+    __shared__ double lds[1024 / 64];
+    __shared__ BigSocState L;
     if (st != nullptr && halted(st, slot)) return;
-    const int b = blockIdx.x;
-    const int cone = K.big_blk_cone[b];
+    const int cone = blockIdx.x;
     BigSocState *S = K.big_state + cone;
     if (S->done) return;
     const int64_t st0 = K.soc_big_start[cone];
     const int k = K.soc_big_dim[cone] - 1;
-    const double s = S->s, t = S->t;
-    double a = 0.0, c = 0.0;
-    const int e0 = K.big_blk_first[b];
-    for (int q = 0; q < kBigElems; ++q) {
-        const int e = e0 + q * kBlock + threadIdx.x;
-        if (e < k) soc_newton_terms(t, s, (double)y[st0 + 1 + e], d2[st0 + 1 + e], a, c);
-    }
-    a = block_sum(a, lds);
-    c = block_sum(c, lds);
-    double *part = K.big_part + (int64_t)b * kBigSums;
-    if (threadIdx.x == 0) { part[0] = a; part[1] = c; }
-    if (!big_last_arrival(K.big_count + cone, (unsigned)K.big_cone_nblk[cone], &flag)) return;
-    __shared__ BigSocState L;
-    {
-        const int b0 = K.big_cone_blk0[cone], nb = K.big_cone_nblk[cone];
+    if (threadIdx.x == 0) L = *S;
+    __syncthreads();
+    const double s = L.s;
+    for (int it = 0; it < 200; ++it) {
+        if (L.done) break;
         double A = 0.0, C = 0.0;
-        for (int j = threadIdx.x; j < nb; j += kBlock) {
-            A += __builtin_nontemporal_load(&K.big_part[(int64_t)(b0 + j) * kBigSums + 0]);
-            C += __builtin_nontemporal_load(&K.big_part[(int64_t)(b0 + j) * kBigSums + 1]);
-        }
+        const double tt = L.t;
+        for (int e = threadIdx.x; e < k; e += 1024) soc_newton_terms(tt, s, (double)y[st0 + 1 + e], d2[st0 + 1 + e], A, C);
         A = block_sum(A, lds);
         C = block_sum(C, lds);
-        if (threadIdx.x == 0) {
-            K.big_count[cone] = 0u;
-            L = *S;
-            big_newton_update(L, A, C);
-        }
-    }
-    __syncthreads();
-    // final pass: this workgroup completes any remaining Newton steps on its own
-    if (finish) {
-        for (int it = 0; it < 200; ++it) {
-            if (L.done) break;
-            double A = 0.0, C = 0.0;
-            const double tt = L.t;
-            for (int e = threadIdx.x; e < k; e += kBlock) soc_newton_terms(tt, s, (double)y[st0 + 1 + e], d2[st0 + 1 + e], A, C);
-            A = block_sum(A, lds);
-            C = block_sum(C, lds);
-            if (threadIdx.x == 0) big_newton_update(L, A, C);
-            __syncthreads();
-        }
+        if (threadIdx.x == 0) big_newton_update(L, A, C);
+        __syncthreads();
     }
     if (threadIdx.x == 0) *S = L;
 }
@@ -484,7 +548,7 @@ __global__ __launch_bounds__(kBlock) void k_bigsoc_apply(ConeLayout K, const dou
             dy[st0] = (T)d;
             my2 += d * d;
         }
-        if (cs == 3) S->lam = lam;
+        S->t_prev = (cs == 1) ? NAN : th;
     }
     if (part != nullptr) {
         my2 = block_sum(my2, lds);
@@ -641,9 +705,9 @@ static void launch_soc(hipStream_t s, const ConeLayout &K, const double *d2, T *
     }
     if (K.nsoc_big > 0) {
         const unsigned nb = (unsigned)K.nbig_blocks;
-        k_bigsoc_init<T><<<nb, kBlock, 0, s>>>(K, d2, y, st, slot);
-        for (int p = 0; p < kBigNewtonPasses; ++p)
-            k_bigsoc_newton<T><<<nb, kBlock, 0, s>>>(K, d2, y, st, slot, p == kBigNewtonPasses - 1);
+        k_bigsoc_pass<T><<<nb, kBlock, 0, s>>>(K, d2, y, st, slot, 0);
+        k_bigsoc_pass<T><<<nb, kBlock, 0, s>>>(K, d2, y, st, slot, 1);
+        k_bigsoc_finish<T><<<K.nsoc_big, 1024, 0, s>>>(K, d2, y, st, slot);
         k_bigsoc_apply<T><<<nb, kBlock, 0, s>>>(K, d2, y, ys, dy, st, slot,
                                                 part ? part + (K.nsoc_small > 0 ? soc_blocks(K) : 0) : nullptr);
     }
@@ -1031,6 +1095,70 @@ __global__ __launch_bounds__(kBlock) void k_kkt_soc(const int32_t *start, const 
 struct KktCounts {
     int c[Q_N];
 };
+// Large cones: multi-workgroup plain SOC projection residual.  Pass 1 writes
+// per-workgroup partial sums of ||tail||^2; pass 2 lets every workgroup reduce
+// its cone's partials (a few hundred values), then scores its slice.
+__global__ __launch_bounds__(kBlock) void k_kkt_big_norm(ConeLayout K, const double *__restrict__ r,
+                                                         double *__restrict__ scratch) {
+    __shared__ double lds[kBlock / 64];
+    const int b = blockIdx.x;
+    const int cone = K.big_blk_cone[b];
+    const int64_t st0 = K.soc_big_start[cone];
+    const int k = K.soc_big_dim[cone] - 1;
+    double acc = 0.0;
+    const int e0 = K.big_blk_first[b];
+    for (int q = 0; q < kBigElems; ++q) {
+        const int e = e0 + q * kBlock + threadIdx.x;
+        if (e < k) { const double v = r[st0 + 1 + e]; acc += v * v; }
+    }
+    acc = block_sum(acc, lds);
+    if (threadIdx.x == 0) scratch[b] = acc;
+}
+__global__ __launch_bounds__(kBlock) void k_kkt_big_res(ConeLayout K, const double *__restrict__ r,
+                                                        const double *__restrict__ scratch, double *__restrict__ part,
+                                                        int64_t bstride) {
+    __shared__ double lds[kBlock / 64];
+    const int b = blockIdx.x;
+    const int cone = K.big_blk_cone[b];
+    const int64_t st0 = K.soc_big_start[cone];
+    const int k = K.soc_big_dim[cone] - 1;
+    double nx = 0.0;
+    {
+        const int b0 = K.big_cone_blk0[cone], nb = K.big_cone_nblk[cone];
+        for (int j = threadIdx.x; j < nb; j += kBlock) nx += scratch[b0 + j];
+        nx = sqrt(block_sum_all(nx, lds));
+    }
+    const double t = r[st0];
+    double c = 1.0, th = t;
+    if (nx <= -t) { c = 0.0; th = 0.0; }
+    else if (nx > t) { c = (1.0 + t / nx) / 2.0; th = nx * c; }
+    double p2 = 0.0, pinf = 0.0, projinf = 0.0;
+    const int e0 = K.big_blk_first[b];
+    for (int q = 0; q < kBigElems; ++q) {
+        const int e = e0 + q * kBlock + threadIdx.x;
+        if (e < k) {
+            const double v = r[st0 + 1 + e], p = v * c, pr = p - v;
+            p2 += pr * pr;
+            pinf = fmax(pinf, fabs(pr));
+            projinf = fmax(projinf, fabs(p));
+        }
+    }
+    if (b == K.big_cone_blk0[cone] && threadIdx.x == 0) {
+        const double pr = th - t;
+        p2 += pr * pr;
+        pinf = fmax(pinf, fabs(pr));
+        projinf = fmax(projinf, fabs(th));
+    }
+    p2 = block_sum(p2, lds);
+    pinf = block_max(pinf, lds);
+    projinf = block_max(projinf, lds);
+    if (threadIdx.x == 0) {
+        part[Q_P2 * kKktStride + bstride + b] = p2;
+        part[Q_PINF * kKktStride + bstride + b] = pinf;
+        part[Q_PROJINF * kKktStride + bstride + b] = projinf;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_kkt_final(const double *part, KktCounts counts, KktOut *out) {
     __shared__ double lds[kBlock / 64];
     const bool ismax[Q_N] = {false, false, false, false, false, true, true, true, false, false, true, true, true};
@@ -1063,7 +1191,7 @@ void launch_kkt(hipStream_t s, const KktData &D, const T *x, const T *y, double 
     const unsigned bn = nblocks(D.n * D.Gt.lanes);
     const unsigned bm = nblocks(D.m * D.G.lanes);
     const unsigned bs1 = D.K.nsoc_small > 0 ? nblocks((int64_t)D.K.nsoc_small * 64) : 0;
-    const unsigned bs2 = D.K.nsoc_big > 0 ? nblocks((int64_t)D.K.nsoc_big * 64) : 0;
+    const unsigned bs2 = D.K.nsoc_big > 0 ? (unsigned)D.K.nbig_blocks : 0;
     // counts per quantity; the n-side and m-side share Q_DOBJ (n side first)
     KktCounts kc;
     int *counts = kc.c;
@@ -1081,7 +1209,10 @@ void launch_kkt(hipStream_t s, const KktData &D, const T *x, const T *y, double 
     PDCS_LANES(D.G.lanes, KM)
 #undef KM
     if (bs1) k_kkt_soc<<<bs1, kBlock, 0, s>>>(D.K.soc_small_start, D.K.soc_small_dim, D.K.nsoc_small, D.rtmp, part, bm);
-    if (bs2) k_kkt_soc<<<bs2, kBlock, 0, s>>>(D.K.soc_big_start, D.K.soc_big_dim, D.K.nsoc_big, D.rtmp, part, bm + bs1);
+    if (bs2) {
+        k_kkt_big_norm<<<bs2, kBlock, 0, s>>>(D.K, D.rtmp, D.scratch_big);
+        k_kkt_big_res<<<bs2, kBlock, 0, s>>>(D.K, D.rtmp, D.scratch_big, part, bm + bs1);
+    }
     k_kkt_final<<<1, kBlock, 0, s>>>(part, kc, out);
 }
 template void launch_kkt<double>(hipStream_t, const KktData &, const double *, const double *, double *, KktOut *);

@@ -125,11 +125,13 @@ struct pdcs_problem {
     double *d_vals = nullptr, *d_tvals = nullptr;     // scaled FP64
     double *c_raw = nullptr, *h_raw = nullptr, *blf = nullptr, *buf = nullptr, *laml = nullptr, *lamu = nullptr;
     double *Dr = nullptr, *Dl = nullptr, *dsc = nullptr, *dsc2 = nullptr;
+    double *kkt_big = nullptr;   // large-cone norm partials of the residual check
     double *rtmp = nullptr, *kpart = nullptr, *dpart = nullptr, *dout = nullptr, *apart = nullptr;
     int64_t apart_stride = 0;
     KktOut *kout = nullptr;
     StepState *st = nullptr;
     ConeLayout K{};
+    ConeLayout Kaux{};   // same layout, own root-finder state: host-driven projections (restart search)
     int lanesG = 1, lanesGt = 1;
     IterSet<double> f64;
     IterSet<float> f32;
@@ -292,7 +294,7 @@ int build_cone_layout(pdcs_ctx *ctx, DevBuf &M, int64_t m, int64_t mzero, int64_
         K.big_count = M.alloc<unsigned int>(K.nsoc_big);
         if (!K.big_state || !K.big_part || !K.big_count) return fail(ctx, PDCS_ERR_HIP, "alloc failed");
         std::vector<BigSocState> init(K.nsoc_big);
-        for (auto &b : init) { std::memset(&b, 0, sizeof(b)); b.lam = NAN; b.done = 1; }
+        for (auto &b : init) { std::memset(&b, 0, sizeof(b)); b.t_prev = NAN; b.done = 1; }
         upload(ctx, K.big_state, init.data(), K.nsoc_big);
     }
     if ((K.nsoc_small && (!K.soc_small_start || !K.soc_small_dim)) || (K.nsoc_big && (!K.soc_big_start || !K.big_blk_cone)))
@@ -451,6 +453,17 @@ int build_problem(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double 
         delete P;
         return rc;
     }
+    P->Kaux = P->K;
+    P->kkt_big = M.alloc<double>(P->K.nbig_blocks + 1);
+    if (P->K.nsoc_big > 0) {
+        P->Kaux.big_state = M.alloc<BigSocState>(P->K.nsoc_big);
+        P->Kaux.big_part = M.alloc<double>((int64_t)P->K.nbig_blocks * kBigSums);
+        P->Kaux.big_count = M.alloc<unsigned int>(P->K.nsoc_big);
+        if (!P->Kaux.big_state || !P->Kaux.big_part || !P->Kaux.big_count) {
+            delete P;
+            return fail(ctx, PDCS_ERR_HIP, "alloc failed");
+        }
+    }
     // uploads
     upload(ctx, P->d_rowptr, rowptr, m + 1);
     upload(ctx, P->d_col, colidx, nnz);
@@ -507,6 +520,7 @@ KktData kkt_data(pdcs_problem *P) {
     D.n = P->n; D.m = P->m; D.nb = P->nb;
     D.K = P->K;
     D.rtmp = P->rtmp;
+    D.scratch_big = P->kkt_big;
     return D;
 }
 
@@ -601,7 +615,7 @@ struct Solver {
         dots<1>(A, A, len, o);
         return std::sqrt(o[0]);
     }
-    void proj_dual(T *y) { launch_project_dual<T>(s, P->K, P->dsc, P->dsc2, y); }
+    void proj_dual(T *y) { launch_project_dual<T>(s, P->Kaux, P->dsc, P->dsc2, y); }
 
     // Mnorm, rpdhg_alg_gpu_gen.jl:133-138
     double Mnorm2(const T *x, const T *y, double tau_, double sigma_) {
@@ -690,8 +704,8 @@ struct Solver {
         if (prm->verbose && P->K.nsoc_big > 0 && std::getenv("PDCS_TRACE")) {
             BigSocState b{};
             copy_sync(P->ctx, &b, P->K.big_state, sizeof(b), hipMemcpyDeviceToHost);
-            fprintf(stderr, "[pdcs trace] big cone 0: case %d done %d newton iters %d t %.17g lam %.17g s %.17g\n", b.cs,
-                    b.done, b.iters, b.t, b.lam, b.s);
+            fprintf(stderr, "[pdcs trace] big cone 0: case %d done %d series %d newton iters %d t %.17g s %.17g\n", b.cs,
+                    b.done, b.series, b.iters, b.t, b.s);
         }
         if (prm->verbose)
             fprintf(stderr, "[pdcs %s] iter %ld relp %.3e reld %.3e gap %.3e pobj %.10e dobj %.10e restarts %ld\n",
@@ -1082,8 +1096,9 @@ int pdcs_solve(pdcs_problem *P, const pdcs_params *prm_in, const double *x0, con
     // every solve starts from the same root-finder state (no warm start carried over)
     if (P->K.nsoc_big > 0) {
         std::vector<BigSocState> init(P->K.nsoc_big);
-        for (auto &b : init) { std::memset(&b, 0, sizeof(b)); b.lam = NAN; b.done = 1; }
+        for (auto &b : init) { std::memset(&b, 0, sizeof(b)); b.t_prev = NAN; b.done = 1; }
         HIPCHK(ctx, copy_sync(ctx, P->K.big_state, init.data(), sizeof(BigSocState) * init.size(), hipMemcpyHostToDevice));
+        HIPCHK(ctx, copy_sync(ctx, P->Kaux.big_state, init.data(), sizeof(BigSocState) * init.size(), hipMemcpyHostToDevice));
     }
     if ((rc = alloc_iters<double>(P))) return rc;
     if (mixed && (rc = alloc_iters<float>(P))) return rc;

@@ -112,8 +112,9 @@ def test_scaling_matches_oracle(ctx):
     assert np.allclose(Dl, rDl, rtol=1e-12)
 
 
-def test_evaluate_matches_oracle_kkt(ctx):
-    P = instances.workload0()
+@pytest.mark.parametrize("which", ["workload0", "regression"])
+def test_evaluate_matches_oracle_kkt(ctx, which):
+    P = instances.workload0() if which == "workload0" else instances.workload2(rows=20000, cols=5000, density=1e-3)
     rng = np.random.default_rng(9)
     x, y = rng.standard_normal(P.n), rng.standard_normal(P.m)
     prob = Problem.from_instance(P, ctx)
