@@ -416,68 +416,6 @@ void oracle_proj_dual_exp(double *v) {
     oracle_proj_exp(w);
     v[0] += w[0]; v[1] += w[1]; v[2] += w[2];
 }
-/* Scaled exponential cone {v: D v in Kexp}.  The reference's
- * exponent_proj_diagonal! (exp_proj.jl:666-720) is restated through the
- * boundary parametrisation v(rho, s) = s (rho/dr, 1/ds, e^rho/dt): the
- * projection is the best of the heuristic points and the boundary point whose
- * rho solves <v0,u'>||u||^2 = <v0,u><u,u'> (u = dv/ds), found here by
- * bracketed bisection over rho.  Parity unpinned against the reference's own
- * case analysis (rho_bound_diagonal, exp_proj.jl:268-466); pinned instead by
- * the projection optimality conditions in tests. */
-static double sexp_F(const double *v0, const double *d, double rho) {
-    double dr = d[0], ds = d[1], dt = d[2], e = exp(rho);
-    double u0 = rho / dr, u1 = 1.0 / ds, u2 = e / dt;
-    double p0 = 1.0 / dr, p2 = e / dt;
-    double uu = u0 * u0 + u1 * u1 + u2 * u2, up = u0 * p0 + u2 * p2;
-    double vu = v0[0] * u0 + v0[1] * u1 + v0[2] * u2, vp = v0[0] * p0 + v0[2] * p2;
-    return (vp * uu - vu * up) / (e * uu);
-}
-static int in_sexp(const double *v, const double *d, double tol) {
-    double r = d[0] * v[0], s = d[1] * v[1], t = d[2] * v[2];
-    if (s > 0) return s * exp(r / s) <= t + tol * (1 + fabs(t));
-    return (s >= -tol) && r <= tol && t >= -tol;
-}
-void oracle_proj_exp_scaled(double *v, const double *d) {
-    double v0[3] = {v[0], v[1], v[2]};
-    double best[3], bestd;
-    if (in_sexp(v0, d, 0.0)) return;
-    /* candidate 1: origin / r<=0,s=0 face: (min(r,0), 0, max(t,0)) is in the cone */
-    best[0] = dmin(v0[0], 0); best[1] = 0; best[2] = dmax(v0[2], 0);
-    bestd = sqrt((best[0] - v0[0]) * (best[0] - v0[0]) + v0[1] * v0[1] + (best[2] - v0[2]) * (best[2] - v0[2]));
-    /* candidate 2: boundary ray point; scan rho for sign changes of F */
-    double prev = NAN, prho = 0;
-    for (int k = 0; k <= 800; ++k) {
-        double rho = -40.0 + 0.1 * k;
-        double f = sexp_F(v0, d, rho);
-        if (k > 0 && isfinite(f) && isfinite(prev) && (f == 0 || (f > 0) != (prev > 0))) {
-            double lo = prho, hi = rho, flo = prev;
-            for (int it = 0; it < 200; ++it) {
-                double mid = 0.5 * (lo + hi), fm = sexp_F(v0, d, mid);
-                if ((fm > 0) == (flo > 0)) { lo = mid; flo = fm; } else hi = mid;
-            }
-            double r = 0.5 * (lo + hi), e = exp(r);
-            double u[3] = {r / d[0], 1.0 / d[1], e / d[2]};
-            double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-            double s = (v0[0] * u[0] + v0[1] * u[1] + v0[2] * u[2]) / uu;
-            if (s > 0) {
-                double c[3] = {s * u[0], s * u[1], s * u[2]};
-                double dd = sqrt((c[0] - v0[0]) * (c[0] - v0[0]) + (c[1] - v0[1]) * (c[1] - v0[1]) + (c[2] - v0[2]) * (c[2] - v0[2]));
-                if (dd < bestd) { bestd = dd; best[0] = c[0]; best[1] = c[1]; best[2] = c[2]; }
-            }
-        }
-        prev = f;
-        prho = rho;
-    }
-    v[0] = best[0]; v[1] = best[1]; v[2] = best[2];
-}
-/* dualExponent_proj_diagonal!, exp_proj.jl:722-730.  With C = {w: D w in Kexp},
- * C* = {y: D^-1 y in Kexp*} and Moreau gives P_{C*}(v) = v + P_C(-v). */
-void oracle_proj_dual_exp_scaled(double *v, const double *d) {
-    double w[3] = {-v[0], -v[1], -v[2]};
-    oracle_proj_exp_scaled(w, d);
-    v[0] += w[0]; v[1] += w[1]; v[2] += w[2];
-}
-
 /* dual-side projection in scaled space: codes 1 (free), 4 (nonneg), 6 (scaled SOC),
  * 12 (Kexp* scaled), 15 (Kexp scaled); massive_block_proj.cu:1081-1194 dispatch */
 void oracle_proj_dual(const oracle_problem *P, double *y, const double *dsc,
@@ -494,15 +432,12 @@ void oracle_proj_dual(const oracle_problem *P, double *y, const double *dsc,
         off += P->soc[b];
     }
     free(tmp);
-    for (int64_t b = 0; b < P->nexp; ++b) { /* code 12: y in (scaled) Kexp* */
-        double dd[3] = {1.0 / dsc[off], 1.0 / dsc[off + 1], 1.0 / dsc[off + 2]};
-        oracle_proj_dual_exp_scaled(y + off, dd);
-        off += 3;
-    }
-    for (int64_t b = 0; b < P->ndexp; ++b) { /* code 15: y/Dl in Kexp, i.e. dsc .* y in Kexp */
-        oracle_proj_exp_scaled(y + off, dsc + off);
-        off += 3;
-    }
+    /* exponential blocks carry one row scale per block (block-constant Dl, see
+     * oracle_scale), so the scaled cones are Kexp* / Kexp themselves: code 12
+     * (dualExponent_proj_diagonal!) reduces to dualExponent_proj!
+     * (exp_proj.jl:741-750) and code 15 to exponent_proj! (:635-664) */
+    for (int64_t b = 0; b < P->nexp; ++b) { oracle_proj_dual_exp(y + off); off += 3; }
+    for (int64_t b = 0; b < P->ndexp; ++b) { oracle_proj_exp(y + off); off += 3; }
 }
 
 /* con_proj!: projection onto K in the original space (codes 2/3/5/13/16) */
@@ -527,6 +462,20 @@ static void scale_pass(const oracle_problem *P, double *vals, const double *Dr,
     for (int64_t j = 0; j < P->n; ++j) Drp[j] *= Dr[j];
     for (int64_t i = 0; i < P->m; ++i) Dlp[i] *= Dl[i];
 }
+/* Row scales of an exponential (or dual exponential) block are replaced by
+ * their maximum, so every block keeps a single scale: the exponential cone is
+ * invariant under a positive multiple but not under a non-uniform diagonal.
+ * (The reference instead scales per row and projects onto the diagonally
+ * scaled cone, exp_proj.jl:666-730.) */
+static void exp_block_max(const oracle_problem *P, double *Dl) {
+    int64_t off = P->mzero + P->mnonneg;
+    for (int64_t b = 0; b < P->nsoc; ++b) off += P->soc[b];
+    for (int64_t b = 0; b < P->nexp + P->ndexp; ++b, off += 3) {
+        const double v = dmax(Dl[off], dmax(Dl[off + 1], Dl[off + 2]));
+        Dl[off] = Dl[off + 1] = Dl[off + 2] = v;
+    }
+}
+
 void oracle_scale(const oracle_problem *P, int ruiz_iters, int pock, double *Drp,
                   double *Dlp, double *vals) {
     int64_t nnz = P->rowptr[P->m];
@@ -548,6 +497,7 @@ void oracle_scale(const oracle_problem *P, int ruiz_iters, int pock, double *Drp
         for (int64_t j = 0; j < P->n; ++j) Dr[j] = sqrt(Dr[j]);
         for (int64_t j = 0; j < P->n; ++j) if (Dr[j] == 0.0) Dr[j] = 1.0;
         for (int64_t i = 0; i < P->m; ++i) if (Dl[i] == 0.0) Dl[i] = 1.0;
+        exp_block_max(P, Dl);
         scale_pass(P, vals, Dr, Dl, Drp, Dlp);
     }
     if (pock) {
@@ -564,6 +514,7 @@ void oracle_scale(const oracle_problem *P, int ruiz_iters, int pock, double *Drp
         for (int64_t j = 0; j < P->n; ++j) Dr[j] = sqrt(Dr[j]);
         for (int64_t j = 0; j < P->n; ++j) if (Dr[j] == 0.0) Dr[j] = 1.0;
         for (int64_t i = 0; i < P->m; ++i) if (Dl[i] == 0.0) Dl[i] = 1.0;
+        exp_block_max(P, Dl);
         scale_pass(P, vals, Dr, Dl, Drp, Dlp);
     }
     free(Dr);

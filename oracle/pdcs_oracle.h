@@ -103,9 +103,6 @@ void oracle_proj_primal_cone(const oracle_problem *prob, double *r);
 /* exponential cone K_exp = cl{(r,s,t): s>0, s exp(r/s) <= t} and its dual */
 void oracle_proj_exp(double *v);
 void oracle_proj_dual_exp(double *v);
-/* scaled variants: {v : D v in Kexp} with D = diag(d) */
-void oracle_proj_exp_scaled(double *v, const double *d);
-void oracle_proj_dual_exp_scaled(double *v, const double *d);
 
 /* Ruiz(10) + Pock-Chambolle(alpha=1) scaling; outputs Dr (n), Dl (m) products
  * and the scaled CSR values (nnz) */

@@ -1,0 +1,123 @@
+// exp_cone.h -- Euclidean projections onto the exponential cone and its dual.
+// Host-and-device inline functions: the kernels call them per lane, and a
+// host build of the same code can be checked against the oracle on a CPU.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace pdcs {
+
+// Projection onto Kexp = cl{(r, s, t) : s > 0, s exp(r/s) <= t} (rows in the
+// order of exponent_proj!, src/pdcs_gpu/exp_proj.jl:635-664).  One lane per
+// cone, FP64 whatever the iterate precision.
+//
+// Cheap cases first: v in Kexp, v in the polar cone -Kexp* (projection 0),
+// r <= 0 and s <= 0 (projection (r, 0, max(t, 0))).  Otherwise a primal point
+// vp in Kexp and a polar point vd are formed from the two one-sided heuristics
+// (lift t, or drop to the s = 0 face); when vp + vd = v with <vp, vd> = 0 the
+// Moreau decomposition certifies vp as the projection.  Else the projection
+// lies on the boundary ray s (rho, 1, e^rho) with rho the root of
+//   F(rho) = ((rho-1) r + s) e^rho - (r - rho s) e^-rho - (rho^2 - rho + 1) t
+// (Friberg, "Projection onto the exponential cone: a univariate root-finding
+// problem", Optim. Methods Softw. 38 (2023)), bracketed by 1 - s/r and r/s
+// and solved by Newton kept inside the bracket (bisection when a step leaves
+// it or fails to halve F's bracket).  F is evaluated times e^-|rho| so no
+// exponential overflows.  The boundary point replaces the heuristic point
+// when it is not farther from v.
+__host__ __device__ inline bool exp_in_cone(double r, double s, double t) {
+    if (s > 0.0) return t > 0.0 && r <= s * log(t / s);
+    return s == 0.0 && r <= 0.0 && t >= 0.0;
+}
+__host__ __device__ inline bool exp_in_polar(double r, double s, double t) {
+    if (r > 0.0) return t < 0.0 && s / r <= 1.0 + log(-t / r);
+    return r == 0.0 && s <= 0.0 && t <= 0.0;
+}
+__host__ __device__ inline void exp_root_fn(double r, double s, double t, double rho, double &f, double &df) {
+    const double k = fabs(rho);
+    const double ep = exp(rho - k), en = exp(-rho - k), pk = exp(-k);
+    f = ((rho - 1.0) * r + s) * ep - (r - rho * s) * en - (rho * (rho - 1.0) + 1.0) * t * pk;
+    df = (rho * r + s) * ep + (r - (rho - 1.0) * s) * en - (2.0 * rho - 1.0) * t * pk;
+}
+__host__ __device__ inline double dist3(double a, double b, double c) { return sqrt(a * a + b * b + c * c); }
+
+__host__ __device__ inline void exp_project(double &r, double &s, double &t) {
+    const double r0 = r, s0 = s, t0 = t;
+    if (exp_in_cone(r0, s0, t0)) return;
+    if (exp_in_polar(r0, s0, t0)) { r = s = t = 0.0; return; }
+    if (r0 <= 0.0 && s0 <= 0.0) { s = 0.0; t = fmax(t0, 0.0); return; }
+    // primal heuristic point
+    double br = fmin(r0, 0.0), bs = 0.0, bt = fmax(t0, 0.0);
+    double bd = dist3(br - r0, s0, bt - t0);
+    if (s0 > 0.0) {
+        const double tp = fmax(t0, s0 * exp(fmin(r0 / s0, 700.0)));
+        if (tp - t0 < bd) { br = r0; bs = s0; bt = tp; bd = tp - t0; }
+    }
+    // polar heuristic point
+    double qr = 0.0, qs = fmin(s0, 0.0), qt = fmin(t0, 0.0);
+    double qd = dist3(r0, qs - s0, qt - t0);
+    if (r0 > 0.0) {
+        const double td = fmin(t0, -r0 * exp(fmin(s0 / r0 - 1.0, 700.0)));
+        if (t0 - td < qd) { qr = r0; qs = s0; qt = td; }
+    }
+    // Moreau certificate: vp in Kexp, vd in the polar, vp + vd = v, <vp, vd> = 0
+    const bool sum_ok = fabs(br + qr - r0) <= 1e-14 * (1.0 + fabs(r0)) &&
+                        fabs(bs + qs - s0) <= 1e-14 * (1.0 + fabs(s0)) &&
+                        fabs(bt + qt - t0) <= 1e-14 * (1.0 + fabs(t0));
+    const double ip = fabs(br * qr + bs * qs + bt * qt);
+    if (sum_ok && ip <= 1e-14 * dist3(br, bs, bt) * dist3(qr, qs, qt)) {
+        r = br; s = bs; t = bt;
+        return;
+    }
+    double lo = r0 > 0.0 ? 1.0 - s0 / r0 : -INFINITY;
+    double hi = s0 > 0.0 ? r0 / s0 : INFINITY;
+    if (lo > hi) { const double q = lo; lo = hi; hi = q; }
+    double f, df;
+    if (!(lo > -INFINITY)) {
+        double w = 1.0;
+        lo = hi - w;
+        exp_root_fn(r0, s0, t0, lo, f, df);
+        while (f > 0.0 && w < 1e5) { w *= 2.0; lo = hi - w; exp_root_fn(r0, s0, t0, lo, f, df); }
+    }
+    if (!(hi < INFINITY)) {
+        double w = 1.0;
+        hi = lo + w;
+        exp_root_fn(r0, s0, t0, hi, f, df);
+        while (f < 0.0 && w < 1e5) { w *= 2.0; hi = lo + w; exp_root_fn(r0, s0, t0, hi, f, df); }
+    }
+    double flo, fhi;
+    exp_root_fn(r0, s0, t0, lo, flo, df);
+    exp_root_fn(r0, s0, t0, hi, fhi, df);
+    if (flo <= 0.0 && fhi >= 0.0) {
+        double rho = 0.5 * (lo + hi), dxo = hi - lo, dx = dxo;
+        for (int it = 0; it < 100; ++it) {
+            exp_root_fn(r0, s0, t0, rho, f, df);
+            if (f == 0.0) break;
+            if (f < 0.0) lo = rho; else hi = rho;
+            if (((rho - hi) * df - f) * ((rho - lo) * df - f) > 0.0 || fabs(2.0 * f) > fabs(dxo * df)) {
+                dxo = dx;
+                dx = 0.5 * (hi - lo);
+                rho = lo + dx;
+            } else {
+                dxo = dx;
+                dx = f / df;
+                rho -= dx;
+            }
+            const double sr = fmax(1.0, fabs(rho));
+            if (fabs(dx) <= 2e-16 * sr || hi - lo <= 4e-16 * sr) break;
+        }
+        const double ss = ((rho - 1.0) * r0 + s0) / (rho * (rho - 1.0) + 1.0);
+        if (ss > 0.0) {
+            const double cr = rho * ss, ct = exp(rho + log(ss));
+            if (dist3(cr - r0, ss - s0, ct - t0) <= bd) { br = cr; bs = ss; bt = ct; }
+        }
+    }
+    r = br; s = bs; t = bt;
+}
+// projection onto Kexp* by Moreau: P_{K*}(v) = v + P_K(-v)
+__host__ __device__ inline void exp_project_dual(double &r, double &s, double &t) {
+    double a = -r, b = -s, c = -t;
+    exp_project(a, b, c);
+    r += a; s += b; t += c;
+}
+
+}  // namespace pdcs

@@ -137,7 +137,7 @@ void launch_axpy_box(hipStream_t s, T *out, const T *a, double alpha, const T *b
                      const T *bu, int64_t n, int64_t nb);
 template <typename T>
 void launch_axpy(hipStream_t s, T *out, const T *a, double alpha, const T *b, int64_t n);
-// in-place projection onto the dual cone (scaled): nonneg + SOC + exp
+// in-place projection onto the dual cone (scaled): nonneg + SOC + exp (codes 4/6/12/15)
 template <typename T>
 void launch_project_dual(hipStream_t s, const ConeLayout &K, const double *d, const double *d2,
                          T *y);
@@ -180,6 +180,8 @@ void launch_row_colmax(hipStream_t s, const Csr<double> &G, const Csr<double> &G
 void launch_scale_vals(hipStream_t s, double *vals, const int32_t *rowptr, const int32_t *col,
                        int64_t m, const double *Dl, const double *Dr);
 void launch_sqrt_fix(hipStream_t s, double *v, int64_t n); // v = sqrt(v); 0 -> 1
+// rows [begin, begin + 3 nblk) of Dl: each block of three takes its maximum
+void launch_exp_block_max(hipStream_t s, double *Dl, int64_t begin, int64_t nblk);
 void launch_mul(hipStream_t s, double *a, const double *b, int64_t n);
 void launch_gather(hipStream_t s, double *dst, const double *src, const int32_t *map, int64_t n);
 

@@ -20,6 +20,7 @@
 // are accumulated in double.  Reductions are two-stage and in fixed order, so
 // results are run-to-run deterministic.
 #include "internal.h"
+#include "exp_cone.h"
 
 #include <math.h>
 
@@ -556,6 +557,39 @@ __global__ __launch_bounds__(kBlock) void k_bigsoc_apply(ConeLayout K, const dou
     }
 }
 
+// dual-side projection of the exponential section: the first nexp blocks
+// (expG rows, y in Kexp*, code 12) and then ndexp blocks (dual_expG rows,
+// y in Kexp, code 15); dy and sum dy^2 when `dy` is given.  Row scales are
+// constant on each block, so the scaled cones are the cones themselves.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_exp(ConeLayout K, T *__restrict__ y, const T *__restrict__ ys,
+                                                T *__restrict__ dy, StepState *st, int slot,
+                                                double *__restrict__ part) {
+    __shared__ double lds[kBlock / 64];
+    if (st != nullptr && halted(st, slot)) return;
+    const int64_t b = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    double my2 = 0.0;
+    if (b < (int64_t)K.nexp + K.ndexp) {
+        const int64_t i = K.exp_begin + 3 * b;
+        double v0 = (double)y[i], v1 = (double)y[i + 1], v2 = (double)y[i + 2];
+        if (b < K.nexp) exp_project_dual(v0, v1, v2);
+        else exp_project(v0, v1, v2);
+        const T w[3] = {(T)v0, (T)v1, (T)v2};
+        for (int q = 0; q < 3; ++q) {
+            y[i + q] = w[q];
+            if (dy != nullptr) {
+                const double d = (double)w[q] - (double)ys[i + q];
+                dy[i + q] = (T)d;
+                my2 += d * d;
+            }
+        }
+    }
+    if (part != nullptr) {
+        my2 = block_sum(my2, lds);
+        if (threadIdx.x == 0) part[blockIdx.x] = my2;
+    }
+}
+
 template <typename T, int L>
 __global__ __launch_bounds__(kBlock) void k_atdy(Csr<T> Gt, int64_t n, const T *__restrict__ dy,
                                                  const T *__restrict__ x, const T *__restrict__ xs,
@@ -688,9 +722,14 @@ unsigned attempt_grid(int64_t rows, int lanes) {
 static inline int soc_blocks(const ConeLayout &K) {
     return K.nsoc_small > 0 ? (int)nblocks((int64_t)K.nsoc_small * K.soc_small_group) : 0;
 }
+static inline int exp_blocks(const ConeLayout &K) { return (int)nblocks((int64_t)K.nexp + K.ndexp); }
+// partial slots written by the cone kernels of one attempt
+static inline int cone_part_blocks(const ConeLayout &K) {
+    return soc_blocks(K) + (K.nsoc_big > 0 ? K.nbig_blocks : 0) + exp_blocks(K);
+}
 
 template <typename T>
-static void launch_soc(hipStream_t s, const ConeLayout &K, const double *d2, T *y, const T *ys,
+static void launch_cones(hipStream_t s, const ConeLayout &K, const double *d2, T *y, const T *ys,
                        T *dy, StepState *st, int slot, double *part) {
     if (K.nsoc_small > 0) {
         const unsigned nb = (unsigned)soc_blocks(K);
@@ -709,8 +748,12 @@ static void launch_soc(hipStream_t s, const ConeLayout &K, const double *d2, T *
         k_bigsoc_pass<T><<<nb, kBlock, 0, s>>>(K, d2, y, st, slot, 1);
         k_bigsoc_finish<T><<<K.nsoc_big, 1024, 0, s>>>(K, d2, y, st, slot);
         k_bigsoc_apply<T><<<nb, kBlock, 0, s>>>(K, d2, y, ys, dy, st, slot,
-                                                part ? part + (K.nsoc_small > 0 ? soc_blocks(K) : 0) : nullptr);
+                                                part ? part + soc_blocks(K) : nullptr);
     }
+    if (K.nexp + K.ndexp > 0)
+        k_exp<T><<<exp_blocks(K), kBlock, 0, s>>>(
+            K, y, ys, dy, st, slot,
+            part ? part + soc_blocks(K) + (K.nsoc_big > 0 ? K.nbig_blocks : 0) : nullptr);
 }
 
 template <typename T>
@@ -731,7 +774,7 @@ void launch_attempt_part(int part_id, hipStream_t s, int slot, const Csr<T> &G, 
 #undef KD
         break;
     case 2:
-        launch_soc<T>(s, K, d2, it.y, it.ys, it.dy, st, slot, part + stride + bd);
+        launch_cones<T>(s, K, d2, it.y, it.ys, it.dy, st, slot, part + stride + bd);
         break;
     case 3:
 #define KA(L) k_atdy<T, L><<<bp, kBlock, 0, s>>>(Gt, P.n, it.dy, it.x, it.xs, st, slot, part + 2 * stride)
@@ -739,7 +782,7 @@ void launch_attempt_part(int part_id, hipStream_t s, int slot, const Csr<T> &G, 
 #undef KA
         break;
     case 4: {
-        const int ncone = soc_blocks(K) + (K.nsoc_big > 0 ? K.nbig_blocks : 0);
+        const int ncone = cone_part_blocks(K);
         k_step<<<1, 1024, 0, s>>>(st, slot, part, stride, (int)bp, (int)(bd + ncone), (int)bp);
         break;
     }
@@ -853,7 +896,7 @@ template <typename T>
 void launch_project_dual(hipStream_t s, const ConeLayout &K, const double *d, const double *d2, T *y) {
     (void)d;
     if (K.mnonneg > 0) k_nonneg<T><<<nblocks(K.mnonneg), kBlock, 0, s>>>(y, K.mzero, K.mzero + K.mnonneg);
-    launch_soc<T>(s, K, d2, y, nullptr, nullptr, nullptr, 0, nullptr);
+    launch_cones<T>(s, K, d2, y, nullptr, nullptr, nullptr, 0, nullptr);
 }
 template void launch_project_dual<double>(hipStream_t, const ConeLayout &, const double *, const double *, double *);
 template void launch_project_dual<float>(hipStream_t, const ConeLayout &, const double *, const double *, float *);
@@ -883,12 +926,22 @@ __global__ void k_zero_nonneg(double *r, int64_t mzero, int64_t mnonneg) {
     if (i < mzero) r[i] = 0.0;
     else if (i < mzero + mnonneg) r[i] = fmax(r[i], 0.0);
 }
+// exponential section of K itself (codes 13/16): Kexp for the nexp blocks,
+// Kexp* for the ndexp blocks
+__global__ void k_exp_primal(ConeLayout K, double *r) {
+    const int64_t b = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (b >= (int64_t)K.nexp + K.ndexp) return;
+    double *v = r + K.exp_begin + 3 * b;
+    if (b < K.nexp) exp_project(v[0], v[1], v[2]);
+    else exp_project_dual(v[0], v[1], v[2]);
+}
 void launch_project_primal_cone(hipStream_t s, const ConeLayout &K, double *r) {
     if (K.mzero + K.mnonneg > 0) k_zero_nonneg<<<nblocks(K.mzero + K.mnonneg), kBlock, 0, s>>>(r, K.mzero, K.mnonneg);
     if (K.nsoc_small > 0)
         k_soc_plain<<<nblocks((int64_t)K.nsoc_small * 64), kBlock, 0, s>>>(K.soc_small_start, K.soc_small_dim, K.nsoc_small, r);
     if (K.nsoc_big > 0)
         k_soc_plain<<<nblocks((int64_t)K.nsoc_big * 64), kBlock, 0, s>>>(K.soc_big_start, K.soc_big_dim, K.nsoc_big, r);
+    if (K.nexp + K.ndexp > 0) k_exp_primal<<<exp_blocks(K), kBlock, 0, s>>>(K, r);
 }
 
 template <typename T>
@@ -1092,6 +1145,34 @@ __global__ __launch_bounds__(kBlock) void k_kkt_soc(const int32_t *start, const 
     }
 }
 
+// exponential rows: residual of r = Gx - h against its projection onto K
+__global__ __launch_bounds__(kBlock) void k_kkt_exp(ConeLayout K, const double *__restrict__ r,
+                                                    double *__restrict__ part, int64_t bstride) {
+    __shared__ double lds[kBlock / 64];
+    const int64_t b = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    double p2 = 0.0, pinf = 0.0, projinf = 0.0;
+    if (b < (int64_t)K.nexp + K.ndexp) {
+        const double *v = r + K.exp_begin + 3 * b;
+        double p[3] = {v[0], v[1], v[2]};
+        if (b < K.nexp) exp_project(p[0], p[1], p[2]);
+        else exp_project_dual(p[0], p[1], p[2]);
+        for (int q = 0; q < 3; ++q) {
+            const double pr = p[q] - v[q];
+            p2 += pr * pr;
+            pinf = fmax(pinf, fabs(pr));
+            projinf = fmax(projinf, fabs(p[q]));
+        }
+    }
+    p2 = block_sum(p2, lds);
+    pinf = block_max(pinf, lds);
+    projinf = block_max(projinf, lds);
+    if (threadIdx.x == 0) {
+        part[Q_P2 * kKktStride + bstride + blockIdx.x] = p2;
+        part[Q_PINF * kKktStride + bstride + blockIdx.x] = pinf;
+        part[Q_PROJINF * kKktStride + bstride + blockIdx.x] = projinf;
+    }
+}
+
 struct KktCounts {
     int c[Q_N];
 };
@@ -1200,7 +1281,8 @@ void launch_kkt(hipStream_t s, const KktData &D, const T *x, const T *y, double 
     counts[Q_ATYINF] = counts[Q_SINF] = counts[Q_DINF] = (int)bn;
     counts[Q_DOBJ] = (int)(bn + bm);
     counts[Q_AXN1] = counts[Q_AXINF] = (int)bm;
-    counts[Q_P2] = counts[Q_PINF] = counts[Q_PROJINF] = (int)(bm + bs1 + bs2);
+    const unsigned be = (unsigned)exp_blocks(D.K);
+    counts[Q_P2] = counts[Q_PINF] = counts[Q_PROJINF] = (int)(bm + bs1 + bs2 + be);
 #define KN(L) k_kkt_n<T, L><<<bn, kBlock, 0, s>>>(D, x, y, part, 0)
     PDCS_LANES(D.Gt.lanes, KN)
 #undef KN
@@ -1213,6 +1295,7 @@ void launch_kkt(hipStream_t s, const KktData &D, const T *x, const T *y, double 
         k_kkt_big_norm<<<bs2, kBlock, 0, s>>>(D.K, D.rtmp, D.scratch_big);
         k_kkt_big_res<<<bs2, kBlock, 0, s>>>(D.K, D.rtmp, D.scratch_big, part, bm + bs1);
     }
+    if (be) k_kkt_exp<<<be, kBlock, 0, s>>>(D.K, D.rtmp, part, bm + bs1 + bs2);
     k_kkt_final<<<1, kBlock, 0, s>>>(part, kc, out);
 }
 template void launch_kkt<double>(hipStream_t, const KktData &, const double *, const double *, double *, KktOut *);
@@ -1264,6 +1347,17 @@ __global__ void k_sqrt_fix(double *v, int64_t n) {
     }
 }
 void launch_sqrt_fix(hipStream_t s, double *v, int64_t n) { k_sqrt_fix<<<nblocks(n), kBlock, 0, s>>>(v, n); }
+// one row scale per exponential block: the block's maximum
+__global__ void k_exp_block_max(double *Dl, int64_t begin, int64_t nblk) {
+    const int64_t b = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (b >= nblk) return;
+    double *v = Dl + begin + 3 * b;
+    const double m = fmax(v[0], fmax(v[1], v[2]));
+    v[0] = v[1] = v[2] = m;
+}
+void launch_exp_block_max(hipStream_t s, double *Dl, int64_t begin, int64_t nblk) {
+    if (nblk > 0) k_exp_block_max<<<nblocks(nblk), kBlock, 0, s>>>(Dl, begin, nblk);
+}
 __global__ void k_mul(double *a, const double *b, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < n) a[i] *= b[i];

@@ -224,7 +224,7 @@ int validate_cones(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const pdcs_c
     }
     // rpdhg_alg_gpu_gen.jl:1333-1336
     if (s != m) return fail(ctx, PDCS_ERR_ARG, "m does not match the sum of mGzero, mGnonnegative, socG, rsocG, and (expG + dual_expG) * 3");
-    if (k->nexp + k->ndualexp > 0) return fail(ctx, PDCS_ERR_UNSUPPORTED, "exponential cones are not supported by this build");
+    if (k->nexp < 0 || k->ndualexp < 0 || k->mzero < 0 || k->mnonneg < 0) return fail(ctx, PDCS_ERR_ARG, "negative cone count");
     if (m >= (int64_t)1 << 31 || n >= (int64_t)1 << 31) return fail(ctx, PDCS_ERR_ARG, "sizes exceed int32 indexing");
     return PDCS_OK;
 }
@@ -239,7 +239,7 @@ int upload(pdcs_ctx *ctx, T *dst, const T *src, int64_t n) {
 // Device cone layout: SOCs of dimension <= 1024 go to the lane-group kernel,
 // longer ones are split over workgroups of kBigBlockElems tail rows.
 int build_cone_layout(pdcs_ctx *ctx, DevBuf &M, int64_t m, int64_t mzero, int64_t mnonneg,
-                      const std::vector<int64_t> &soc, ConeLayout &K) {
+                      const std::vector<int64_t> &soc, int64_t nexp, int64_t ndexp, ConeLayout &K) {
     std::vector<int32_t> ss_start, ss_dim, sb_start, sb_dim, blk_cone, blk_first, cone_blk0, cone_nblk;
     int64_t off = mzero + mnonneg;
     int maxsmall = 0;
@@ -273,6 +273,8 @@ int build_cone_layout(pdcs_ctx *ctx, DevBuf &M, int64_t m, int64_t mzero, int64_
     while (g < 64 && g < maxsmall - 1) g <<= 1;
     K.soc_small_group = g;
     K.exp_begin = off;
+    K.nexp = (int32_t)nexp;
+    K.ndexp = (int32_t)ndexp;
     auto up32 = [&](const std::vector<int32_t> &v) -> const int32_t * {
         if (v.empty()) return nullptr;
         int32_t *d = M.alloc<int32_t>((int64_t)v.size());
@@ -317,6 +319,9 @@ int scale_device(pdcs_problem *P) {
         launch_row_colmax(s, G, Gt, rowv, colv, mode);
         launch_sqrt_fix(s, rowv, P->m);
         launch_sqrt_fix(s, colv, P->n);
+        // one scale per exponential block: Kexp is invariant under a positive
+        // multiple, not under a non-uniform diagonal
+        launch_exp_block_max(s, rowv, P->K.exp_begin, P->nexp + P->ndexp);
         launch_scale_vals(s, P->d_vals, P->d_rowptr, P->d_col, P->m, rowv, colv);
         launch_gather(s, P->d_tvals, P->d_vals, P->d_tmap, P->nnz);
         launch_mul(s, P->Dl, rowv, P->m);
@@ -401,6 +406,7 @@ int build_problem(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double 
     P->bl.assign(bl, bl + nb);
     P->bu.assign(bu, bu + nb);
     P->mzero = cones->mzero; P->mnonneg = cones->mnonneg;
+    P->nexp = cones->nexp; P->ndexp = cones->ndualexp;
     P->soc.assign(cones->soc, cones->soc + cones->nsoc);
     // transpose (CSR of G') with the slot map used to keep G' values in sync
     P->trowptr.assign(n + 1, 0);
@@ -449,7 +455,7 @@ int build_problem(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double 
             return fail(ctx, PDCS_ERR_UNSUPPORTED, "problem too large for the residual reduction layout");
         }
     }
-    if ((rc = build_cone_layout(ctx, M, m, cones->mzero, cones->mnonneg, P->soc, P->K))) {
+    if ((rc = build_cone_layout(ctx, M, m, cones->mzero, cones->mnonneg, P->soc, cones->nexp, cones->ndualexp, P->K))) {
         delete P;
         return rc;
     }
@@ -1086,7 +1092,8 @@ int pdcs_solve(pdcs_problem *P, const pdcs_params *prm_in, const double *x0, con
     if (!P->apart) {
         const int64_t bp = (P->n * P->lanesGt + kBlock - 1) / kBlock;
         const int64_t bd = (P->m * P->lanesG + kBlock - 1) / kBlock +
-                           (P->K.nsoc_small * (int64_t)P->K.soc_small_group + kBlock - 1) / kBlock + P->K.nbig_blocks + 8;
+                           (P->K.nsoc_small * (int64_t)P->K.soc_small_group + kBlock - 1) / kBlock + P->K.nbig_blocks +
+                           ((int64_t)P->K.nexp + P->K.ndexp + kBlock - 1) / kBlock + 8;
         P->apart_stride = std::max(bp, bd) + 64;
         P->apart = P->mem.alloc<double>(3 * P->apart_stride);
         if (!P->apart) return fail(ctx, PDCS_ERR_HIP, "alloc failed");
@@ -1298,7 +1305,7 @@ int pdcs_project_dual(pdcs_ctx *ctx, int32_t precision, const pdcs_cones *cones,
     DevBuf M;
     ConeLayout K{};
     std::vector<int64_t> soc(cones->soc, cones->soc + cones->nsoc);
-    if ((rc = build_cone_layout(ctx, M, m, cones->mzero, cones->mnonneg, soc, K))) return rc;
+    if ((rc = build_cone_layout(ctx, M, m, cones->mzero, cones->mnonneg, soc, cones->nexp, cones->ndualexp, K))) return rc;
     double *d2 = M.alloc<double>(m);
     if (!d2) return fail(ctx, PDCS_ERR_HIP, "alloc failed");
     std::vector<double> hd(m);
@@ -1319,7 +1326,7 @@ int pdcs_project_primal_cone(pdcs_ctx *ctx, const pdcs_cones *cones, int64_t m, 
     DevBuf M;
     ConeLayout K{};
     std::vector<int64_t> soc(cones->soc, cones->soc + cones->nsoc);
-    if ((rc = build_cone_layout(ctx, M, m, cones->mzero, cones->mnonneg, soc, K))) return rc;
+    if ((rc = build_cone_layout(ctx, M, m, cones->mzero, cones->mnonneg, soc, cones->nexp, cones->ndualexp, K))) return rc;
     launch_project_primal_cone(ctx->stream, K, r);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));

@@ -163,6 +163,76 @@ def workload2(seed: int = 1002, rows: int = 200000, cols: int = 50000,
                         bl=bl, bu=bu, soc=[m], name=f"soc_regression_{rows}x{cols}")
 
 
+def workload1(seed: int = 1001, samples: int = 20000, features: int = 2000, density: float = 0.005,
+              lam: float = 0.01) -> ConicProblem:
+    """L1-regularised logistic regression with exponential cones.
+
+    A (samples x features) at `density`, N(0,1); labels y = sign(A w*) with a
+    5%-sparse planted w* and 10% of the labels flipped.  Per sample i,
+    t_i >= log(1 + exp(-y_i a_i'w)) is written with u_i + v_i <= 1,
+    (-y_i a_i'w - t_i, 1, u_i) in Kexp and (-t_i, 1, v_i) in Kexp (MOI order
+    (x, y, z): y exp(x/y) <= z).  w = w+ - w-, w+- >= 0; objective
+    sum t_i + lam * samples * sum(w+ + w-).  Variables (w+, w-, t, u, v), all
+    listed as box variables (nb = n) with infinite bounds where free.
+    Rows: samples nonnegative rows, then 2 * samples exponential cones.
+    """
+    rng = np.random.default_rng(seed)
+    N, p = samples, features
+    nnz = int(round(density * N * p))
+    r, cidx = random_pattern(rng, N, p, nnz)
+    v = rng.standard_normal(nnz)
+    w = np.zeros(p)
+    k = max(1, int(round(0.05 * p)))
+    idx = rng.choice(p, size=k, replace=False)
+    w[idx] = rng.standard_normal(k)
+    score = np.bincount(r, weights=v * w[cidx], minlength=N)
+    lab = np.where(score >= 0, 1.0, -1.0)
+    flip = rng.uniform(size=N) < 0.1
+    lab[flip] *= -1
+    iw_p, iw_m, it, iu, iv = 0, p, 2 * p, 2 * p + N, 2 * p + 2 * N
+    n = 2 * p + 3 * N
+    rows, cols, vals = [], [], []
+    # nonnegative rows: 1 - u_i - v_i >= 0  (G row: -u_i - v_i, h = -1)
+    i = np.arange(N)
+    rows += [i, i]
+    cols += [iu + i, iv + i]
+    vals += [-np.ones(N), -np.ones(N)]
+    base = N
+    # cone 1 of sample i: rows base + 6i + (0, 1, 2); cone 2: base + 6i + (3, 4, 5)
+    r0 = base + 6 * r
+    yl = lab[r]
+    rows += [r0, r0]
+    cols += [iw_p + cidx, iw_m + cidx]
+    vals += [-yl * v, yl * v]                    # -y_i a_i'(w+ - w-)
+    rows += [base + 6 * i]
+    cols += [it + i]
+    vals += [-np.ones(N)]                        # - t_i
+    rows += [base + 6 * i + 2]
+    cols += [iu + i]
+    vals += [np.ones(N)]                         # u_i
+    rows += [base + 6 * i + 3]
+    cols += [it + i]
+    vals += [-np.ones(N)]                        # - t_i
+    rows += [base + 6 * i + 5]
+    cols += [iv + i]
+    vals += [np.ones(N)]                         # v_i
+    m = N + 6 * N
+    rowptr, colidx, vv = csr_from_coo(m, np.concatenate(rows).astype(np.int64),
+                                      np.concatenate(cols).astype(np.int64), np.concatenate(vals))
+    h = np.zeros(m)
+    h[:N] = -1.0
+    h[base + 6 * i + 1] = -1.0                   # middle entries equal 1
+    h[base + 6 * i + 4] = -1.0
+    c = np.zeros(n)
+    c[:2 * p] = lam * N
+    c[it:it + N] = 1.0
+    bl = np.full(n, -np.inf)
+    bu = np.full(n, np.inf)
+    bl[:2 * p] = 0.0
+    return ConicProblem(n=n, m=m, nb=n, c=c, rowptr=rowptr, colidx=colidx, vals=vv, h=h, bl=bl, bu=bu,
+                        mnonneg=N, nexp=2 * N, name=f"logreg_{N}x{p}")
+
+
 def lp_box(n: int, m_eq: int, m_in: int, seed: int) -> ConicProblem:
     """Small feasible, bounded LP (zero + nonnegative cones, box primal) for linprog checks."""
     rng = np.random.default_rng(seed)
@@ -179,4 +249,4 @@ def lp_box(n: int, m_eq: int, m_in: int, seed: int) -> ConicProblem:
                         name=f"lp_{n}x{m}")
 
 
-WORKLOADS = {0: workload0, 2: workload2}
+WORKLOADS = {0: workload0, 1: workload1, 2: workload2}

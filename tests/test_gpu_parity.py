@@ -103,6 +103,104 @@ def test_project_primal_cone_plain_soc(ctx):
     assert np.allclose(t.cpu().numpy(), ref, rtol=1e-13, atol=1e-14)
 
 
+def _exp_points(rng, k):
+    """k points of R^3 at mixed scales, with the special cases of the exponential
+    cone planted: inside Kexp, inside the polar, r <= 0 and s <= 0, on the
+    s = 0 face, and far out on the boundary ray (large rho)."""
+    v = rng.standard_normal((k, 3)) * rng.uniform(0.01, 10, (k, 1))
+    q = k // 8
+    s = np.abs(v[:q, 1]) + 0.1
+    v[:q, 2] = s * np.exp(v[:q, 0] / s) + rng.uniform(0, 1, q)          # inside Kexp
+    r = np.abs(v[q:2 * q, 0]) + 0.1
+    v[q:2 * q, 2] = -r * np.exp(v[q:2 * q, 1] / r - 1) - rng.uniform(0, 1, q)  # inside the polar
+    v[2 * q:3 * q, :2] = -np.abs(v[2 * q:3 * q, :2])                      # r, s <= 0
+    v[3 * q:4 * q, 1] = 0.0                                                 # s = 0
+    v[4 * q:5 * q, 0] = rng.uniform(0.001, 0.01, q)                          # near-degenerate corner
+    v[4 * q:5 * q, 1] = -rng.uniform(0.05, 0.2, q)
+    return v
+
+
+def _cone_dist(p, proj):
+    """Euclidean distance of each row of p from a cone, through the cone's
+    projection (well conditioned, unlike the exponential membership test)."""
+    return np.linalg.norm(p - np.array([proj(w) for w in p]), axis=1)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_project_exp_matches_oracle(ctx, prec):
+    """Dual-side exponential blocks: codes 12 (y in Kexp*) then 15 (y in Kexp),
+    one lane per cone, against the oracle.  The projection is unique, but near
+    the s = 0 face the distance is flat to ~1e-9 over a stretch of the boundary
+    and the oracle's root search can stop on the face heuristic (s = 0) while
+    the kernel finds the boundary ray point; so the test asserts that the
+    kernel's point is feasible, no farther from v than the oracle's (the
+    defining property of the projection), and within 2e-6 |v| of it."""
+    rng = np.random.default_rng(21 + prec)
+    ne, nd = 3000, 2000
+    v = _exp_points(rng, ne + nd)
+    dt = torch.float64 if prec == 0 else torch.float32
+    if prec == 1:
+        big = np.abs(v).max(axis=1) > 1e6          # keep the planted points inside FP32 range
+        v[big] = rng.standard_normal((int(big.sum()), 3))
+        v = v.astype(np.float32).astype(np.float64)
+    y = v.reshape(-1).copy()
+    cones = make_cones(0, 0, [], nexp=ne, ndexp=nd)
+    t = dev(y, dt)
+    dd = dev(np.ones(len(y)))
+    _lib.check(ctx.L.pdcs_project_dual(ctx.h, prec, C.byref(cones), len(y), vp(t), vp(dd)), ctx.h)
+    got = t.cpu().numpy().astype(np.float64).reshape(-1, 3)
+    ref = np.array([oracle.proj_dual_exp(w) for w in v[:ne]] + [oracle.proj_exp(w) for w in v[ne:]])
+    sc = 1 + np.linalg.norm(v, axis=1)
+    err = np.linalg.norm(got - ref, axis=1) / sc
+    assert err.max() <= (2e-6 if prec == 0 else 4e-6), (err.argmax(), v[err.argmax()], got[err.argmax()])
+    if prec == 0:
+        assert (_cone_dist(got[:ne], oracle.proj_dual_exp) / sc[:ne]).max() <= 1e-12
+        assert (_cone_dist(got[ne:], oracle.proj_exp) / sc[ne:]).max() <= 1e-12
+        L = np.longdouble
+        dg = np.sqrt(np.sum((got.astype(L) - v) ** 2, axis=1))
+        dr = np.sqrt(np.sum((ref.astype(L) - v) ** 2, axis=1))
+        assert float(((dg - dr) / sc).max()) <= 1e-13
+
+
+def test_project_primal_cone_exp(ctx):
+    """codes 13/16 of the residual check: Kexp blocks then Kexp* blocks."""
+    rng = np.random.default_rng(8)
+    ne, nd = 500, 700
+    v = _exp_points(rng, ne + nd)
+    r = np.concatenate([rng.standard_normal(5), v.reshape(-1)])
+    cones = make_cones(2, 3, [], nexp=ne, ndexp=nd)
+    ref = r.copy()
+    ref[:2] = 0
+    ref[2:5] = np.maximum(ref[2:5], 0)
+    blk = ref[5:].reshape(-1, 3)
+    blk[:ne] = [oracle.proj_exp(w) for w in v[:ne]]
+    blk[ne:] = [oracle.proj_dual_exp(w) for w in v[ne:]]
+    t = dev(r)
+    _lib.check(ctx.L.pdcs_project_primal_cone(ctx.h, C.byref(cones), len(r), vp(t)), ctx.h)
+    got = t.cpu().numpy()
+    assert np.array_equal(got[:5], ref[:5])
+    g, q = got[5:].reshape(-1, 3), ref[5:].reshape(-1, 3)
+    sc = 1 + np.linalg.norm(v, axis=1)
+    assert (np.linalg.norm(g - q, axis=1) / sc).max() <= 2e-6
+    assert (_cone_dist(g[:ne], oracle.proj_exp) / sc[:ne]).max() <= 1e-12
+    assert (_cone_dist(g[ne:], oracle.proj_dual_exp) / sc[ne:]).max() <= 1e-12
+
+
+def _logreg_small():
+    return instances.workload1(samples=2000, features=200, density=0.05, lam=1e-3)
+
+
+def test_scaling_matches_oracle_exp(ctx):
+    P = _logreg_small()
+    prob = Problem.from_instance(P, ctx)
+    Dr, Dl = prob.scaling()
+    rDr, rDl, _ = oracle.scale(P)
+    assert np.allclose(Dr, rDr, rtol=1e-12)
+    assert np.allclose(Dl, rDl, rtol=1e-12)
+    e = Dl[P.m - 3 * (P.nexp + P.ndexp):].reshape(-1, 3)
+    assert np.all(e == e[:, :1])
+
+
 def test_scaling_matches_oracle(ctx):
     P = instances.workload0()
     prob = Problem.from_instance(P, ctx)
@@ -112,9 +210,11 @@ def test_scaling_matches_oracle(ctx):
     assert np.allclose(Dl, rDl, rtol=1e-12)
 
 
-@pytest.mark.parametrize("which", ["workload0", "regression"])
+@pytest.mark.parametrize("which", ["workload0", "regression", "logreg"])
 def test_evaluate_matches_oracle_kkt(ctx, which):
-    P = instances.workload0() if which == "workload0" else instances.workload2(rows=20000, cols=5000, density=1e-3)
+    P = {"workload0": instances.workload0,
+         "regression": lambda: instances.workload2(rows=20000, cols=5000, density=1e-3),
+         "logreg": _logreg_small}[which]()
     rng = np.random.default_rng(9)
     x, y = rng.standard_normal(P.n), rng.standard_normal(P.m)
     prob = Problem.from_instance(P, ctx)
@@ -183,6 +283,17 @@ def test_solve_lp(ctx):
 
 def test_solve_regression_small(ctx):
     P = instances.workload2(rows=20000, cols=5000, density=1e-3)
+    ref, _, _ = oracle.solve(P)
+    prob = Problem.from_instance(P, ctx)
+    for prec in (0, 1):
+        res = prob.solve(precision=prec, time_limit=100.0)
+        x, y, _ = prob.solution()
+        _check_solution(P, res, x, y, ref)
+
+
+def test_solve_logreg_small(ctx):
+    """Exponential-cone program (L1 logistic regression), FP64 and mixed."""
+    P = _logreg_small()
     ref, _, _ = oracle.solve(P)
     prob = Problem.from_instance(P, ctx)
     for prec in (0, 1):

@@ -171,6 +171,17 @@ def test_scaling_invariants():
     assert np.all(np.abs(cmax[cmax > 0] - 1) < 1e-2)
 
 
+def test_scaling_exp_blocks_constant():
+    """Exponential blocks keep one row scale (the cone is invariant under a
+    positive multiple only); the scaled matrix is still Dl^-1 G Dr^-1."""
+    P = instances.workload1(samples=300, features=40, density=0.1, lam=1e-3)
+    Dr, Dl, vals = oracle.scale(P)
+    e = Dl[P.mnonneg:].reshape(-1, 3)
+    assert np.all(e == e[:, :1])
+    rows = np.repeat(np.arange(P.m), np.diff(P.rowptr))
+    assert np.allclose(vals, P.vals / Dl[rows] / Dr[P.colidx], rtol=1e-12, atol=0)
+
+
 # ------------------------------------------------------------ whole solve --
 def test_kkt_matches_independent_evaluator():
     P = instances.workload0()
@@ -210,3 +221,30 @@ def test_oracle_workload0_weak_duality():
 
     assert dual_cone_violation(P, y) < 1e-9 * (1 + np.abs(y).max())
     assert abs(r["pobj"] - res["pobj"]) < 1e-9 * (1 + abs(r["pobj"]))
+
+
+def test_oracle_logreg_vs_direct_minimisation():
+    """The exponential-cone program of workload1 is the L1-regularised logistic
+    regression  min sum_i log(1 + exp(-y_i a_i'w)) + lam N ||w||_1 ; minimise
+    that directly (L-BFGS-B on w = w+ - w-) and compare optimal values."""
+    scipy = pytest.importorskip("scipy.optimize")
+    N, p, lam = 400, 30, 1e-3
+    P = instances.workload1(samples=N, features=p, density=0.15, lam=lam)
+    res, x, y = oracle.solve(P)
+    assert res["exit_code"] == 0
+    # recover A_i' y_i from the cone rows: row N + 6i holds -y_i a_i'(w+ - w-) - t_i
+    rows = np.repeat(np.arange(P.m), np.diff(P.rowptr))
+    sel = (rows >= N) & ((rows - N) % 6 == 0) & (P.colidx < p)
+    M = np.zeros((N, p))
+    M[(rows[sel] - N) // 6, P.colidx[sel]] = -P.vals[sel]        # = y_i a_ij
+
+    def f(z):
+        w = z[:p] - z[p:]
+        m = M @ w
+        val = np.sum(np.logaddexp(0.0, -m)) + lam * N * z.sum()
+        g = -M.T @ (1.0 / (1.0 + np.exp(m)))
+        return val, np.concatenate([g + lam * N, -g + lam * N])
+
+    opt = scipy.minimize(f, np.zeros(2 * p), jac=True, method="L-BFGS-B", bounds=[(0, None)] * (2 * p),
+                         options={"ftol": 1e-15, "gtol": 1e-12, "maxiter": 10000})
+    assert abs(res["pobj"] - opt.fun) <= 1e-6 * (1 + abs(opt.fun)), (res["pobj"], opt.fun)
