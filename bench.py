@@ -83,7 +83,9 @@ def main():
         return {"time_to_tol_s": t, "iters": it, "iters_per_s": it / t, "exit_code": r["exit_code"],
                 "pobj": r["pobj"], "dobj": r["dobj"], "rel_primal_res": r["rel_primal_res"],
                 "rel_dual_res": r["rel_dual_res"], "rel_gap": r["rel_gap"], "iter_fp32": r["iter_fp32"],
-                "time_kkt_s": r["time_kkt"], "time_restart_s": r["time_restart"]}
+                "time_kkt_s": r["time_kkt"], "time_restart_s": r["time_restart"],
+                "spmv_schedule": {"lanes_G": r["sched"][0], "lanes_Gt": r["sched"][1], "cap_G": r["sched"][2],
+                                  "cap_Gt": r["sched"][3]}}
 
     sm, s64 = summary(mixed), summary(fp64)
     # dominant kernel of the mixed path

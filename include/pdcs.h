@@ -73,6 +73,10 @@ typedef struct {
     int64_t polish_check_freq;        /* MIXED: FP64 polish check interval (200; 0 = check_terminate_freq) */
     double mixed_switch_tol;          /* MIXED: the FP32 phase hands over to the FP64 polish below this
                                          README KKT (1e-5) or when a check fails to halve it */
+    int32_t autotune;                 /* 1: time the SpMV row schedules (lanes per row, workgroup cap) on
+                                         the device at the first solve of each precision and keep the
+                                         fastest; 0: fixed heuristic schedule */
+    int32_t reserved;
 } pdcs_params;
 
 /* Result of a solve; exit codes as the reference's check_termination_criteria
@@ -95,6 +99,8 @@ typedef struct {
     double time_fp32;        /* MIXED: FP32 phase seconds */
     double time_kkt;         /* seconds spent in residual checks */
     double time_restart;     /* seconds spent in restart checks */
+    int32_t sched[4];        /* SpMV schedule of the final phase: lanes per row of G, of G',
+                                workgroup cap of G, of G' */
 } pdcs_result;
 
 void pdcs_params_default(pdcs_params *p);

@@ -52,6 +52,8 @@ mutable struct PdcsParams
     mixed_check_freq::Int64
     polish_check_freq::Int64
     mixed_switch_tol::Float64
+    autotune::Int32
+    reserved::Int32
     PdcsParams() = new()
 end
 
@@ -78,6 +80,7 @@ mutable struct PdcsResult
     time_fp32::Float64
     time_kkt::Float64
     time_restart::Float64
+    sched::NTuple{4, Int32}
     PdcsResult() = new()
 end
 

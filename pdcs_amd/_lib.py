@@ -33,7 +33,8 @@ class Params(C.Structure):
                 ("use_preconditioner", c_i32), ("verbose", c_i32), ("warm_start", c_i32),
                 ("precision", c_i32), ("polish", c_i32), ("bisect_intent", c_i32),
                 ("code_primal_denominator", c_i32), ("use_graph", c_i32), ("mixed_check_freq", c_i64),
-                ("polish_check_freq", c_i64), ("mixed_switch_tol", C.c_double)]
+                ("polish_check_freq", c_i64), ("mixed_switch_tol", C.c_double),
+                ("autotune", c_i32), ("reserved", c_i32)]
 
 
 class Result(C.Structure):
@@ -44,10 +45,15 @@ class Result(C.Structure):
                 ("l2_rel_primal_res", C.c_double), ("l2_rel_dual_res", C.c_double),
                 ("rel_primal_res_code", C.c_double), ("omega", C.c_double),
                 ("time_setup", C.c_double), ("time_solve", C.c_double), ("time_fp32", C.c_double),
-                ("time_kkt", C.c_double), ("time_restart", C.c_double)]
+                ("time_kkt", C.c_double), ("time_restart", C.c_double),
+                ("sched", c_i32 * 4)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        out = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            out[k] = list(v) if isinstance(v, C.Array) else v
+        return out
 
 
 # every symbol declared in include/pdcs.h

@@ -19,6 +19,7 @@ struct Csr {
     const T *val;
     int64_t rows;
     int32_t lanes;   // lanes per row for the wave-split SpMV (power of two, 1..64)
+    int32_t grid;    // workgroup cap of the grid-stride attempt kernels (0 = kAttemptMaxGrid)
 };
 
 // Per-cone scalar state of the multi-workgroup scaled-SOC root find.
@@ -114,8 +115,8 @@ void launch_attempt(hipStream_t s, int slot, const Csr<T> &G, const Csr<T> &Gt,
 // the kernels of one attempt, individually: 0 primal (G'y + update), 1 dual (G xbar +
 // update), 2 cone projections, 3 G'dy dot, 4 step control, 5 reflection
 constexpr int kAttemptKernels = 6;
-constexpr unsigned kAttemptMaxGrid = 2048;   // workgroups of the fused SpMV kernels (grid-stride beyond)
-unsigned attempt_grid(int64_t rows, int lanes);
+constexpr unsigned kAttemptMaxGrid = 2048;   // default workgroup cap of the fused SpMV kernels (grid-stride beyond)
+unsigned attempt_grid(int64_t rows, int lanes, int cap);
 template <typename T>
 void launch_attempt_part(int part_id, hipStream_t s, int slot, const Csr<T> &G, const Csr<T> &Gt,
                          const ProbVec<T> &P, const ConeLayout &K, const double *d2, Iter<T> it,
@@ -130,6 +131,10 @@ template <typename T>
 void launch_spmv(hipStream_t s, const Csr<T> &A, const T *x, T *y);
 template <typename T>
 void launch_spmv_acc(hipStream_t s, const Csr<T> &A, const T *x, double *y); // double output
+// y = A x with the attempt kernels' row schedule (A.lanes lanes per row,
+// grid-stride over at most A.grid workgroups): the autotuner's probe
+template <typename T>
+void launch_spmv_sched(hipStream_t s, const Csr<T> &A, const T *x, T *y);
 
 // out = clamp(a + alpha*b) on the box part (alpha in device memory if dalpha)
 template <typename T>

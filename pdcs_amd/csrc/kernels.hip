@@ -703,9 +703,10 @@ __global__ __launch_bounds__(kBlock) void k_reflect(Iter<T> it, int64_t n, int64
 }
 
 static inline unsigned nblocks(int64_t work) { return (unsigned)((work + kBlock - 1) / kBlock); }
-unsigned attempt_grid(int64_t rows, int lanes) {
+unsigned attempt_grid(int64_t rows, int lanes, int cap) {
     const unsigned b = nblocks(rows * lanes);
-    return b < kAttemptMaxGrid ? b : kAttemptMaxGrid;
+    const unsigned c = cap > 0 ? (unsigned)cap : kAttemptMaxGrid;
+    return b < c ? b : c;
 }
 
 #define PDCS_LANES(L, CALL) \
@@ -760,8 +761,8 @@ template <typename T>
 void launch_attempt_part(int part_id, hipStream_t s, int slot, const Csr<T> &G, const Csr<T> &Gt,
                          const ProbVec<T> &P, const ConeLayout &K, const double *d2, Iter<T> it, StepState *st,
                          double *part, int64_t stride) {
-    const unsigned bp = attempt_grid(P.n, Gt.lanes);
-    const unsigned bd = attempt_grid(P.m, G.lanes);
+    const unsigned bp = attempt_grid(P.n, Gt.lanes, Gt.grid);
+    const unsigned bd = attempt_grid(P.m, G.lanes, G.grid);
     switch (part_id) {
     case 0:
 #define KP(L) k_primal<T, L><<<bp, kBlock, 0, s>>>(Gt, P, it.ys, it.xs, it.x, it.xbar, st, slot, part)
@@ -830,6 +831,15 @@ __global__ __launch_bounds__(kBlock) void k_spmv(Csr<T> A, const T *__restrict__
     if (row < A.rows && lane == 0) y[row] = (T)acc;
 }
 template <typename T, int L>
+__global__ __launch_bounds__(kBlock) void k_spmv_gs(Csr<T> A, const T *__restrict__ x, T *__restrict__ y) {
+    const int lane = (int)(threadIdx.x % L);
+    for (int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / L; row < A.rows;
+         row += (int64_t)gridDim.x * (kBlock / L)) {
+        const double acc = row_dot<T, L>(A, row, lane, x);
+        if (lane == 0) y[row] = (T)acc;
+    }
+}
+template <typename T, int L>
 __global__ __launch_bounds__(kBlock) void k_spmv_acc(Csr<T> A, const T *__restrict__ x, double *__restrict__ y) {
     const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t row = gid / L;
@@ -846,6 +856,15 @@ void launch_spmv(hipStream_t s, const Csr<T> &A, const T *x, T *y) {
 }
 template void launch_spmv<double>(hipStream_t, const Csr<double> &, const double *, double *);
 template void launch_spmv<float>(hipStream_t, const Csr<float> &, const float *, float *);
+template <typename T>
+void launch_spmv_sched(hipStream_t s, const Csr<T> &A, const T *x, T *y) {
+    const unsigned b = attempt_grid(A.rows, A.lanes, A.grid);
+#define KS(L) k_spmv_gs<T, L><<<b, kBlock, 0, s>>>(A, x, y)
+    PDCS_LANES(A.lanes, KS)
+#undef KS
+}
+template void launch_spmv_sched<double>(hipStream_t, const Csr<double> &, const double *, double *);
+template void launch_spmv_sched<float>(hipStream_t, const Csr<float> &, const float *, float *);
 template <typename T>
 void launch_spmv_acc(hipStream_t s, const Csr<T> &A, const T *x, double *y) {
     const unsigned b = nblocks(A.rows * A.lanes);

@@ -132,7 +132,13 @@ struct pdcs_problem {
     StepState *st = nullptr;
     ConeLayout K{};
     ConeLayout Kaux{};   // same layout, own root-finder state: host-driven projections (restart search)
-    int lanesG = 1, lanesGt = 1;
+    int lanesG = 1, lanesGt = 1;   // row schedule of the residual kernels (fixed heuristic)
+    // row schedule of the attempt kernels per precision (0 FP64, 1 FP32)
+    struct Sched {
+        int lanesG, lanesGt, gridG, gridGt;
+        bool tuned;
+    } sched[2]{};
+    bool verbose_tune = false;
     IterSet<double> f64;
     IterSet<float> f32;
     // host staging
@@ -181,6 +187,7 @@ void pdcs_params_default(pdcs_params *p) {
     p->mixed_check_freq = 200;
     p->polish_check_freq = 200;
     p->mixed_switch_tol = 1e-5;
+    p->autotune = 1;
 }
 
 int pdcs_ctx_create(int device, pdcs_ctx **out) {
@@ -447,6 +454,7 @@ int build_problem(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double 
     }
     P->lanesG = lanes_for(nnz, m);
     P->lanesGt = lanes_for(nnz, n);
+    for (auto &sc : P->sched) sc = {P->lanesG, P->lanesGt, (int)kAttemptMaxGrid, (int)kAttemptMaxGrid, false};
     // residual partial arrays hold kKktStrideHost blocks per quantity
     {
         const int64_t bn = (n * P->lanesGt + kBlock - 1) / kBlock, bm = (m * P->lanesG + kBlock - 1) / kBlock;
@@ -502,6 +510,56 @@ int build_problem(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double 
 }
 
 // scaling vectors of the dual cone projections (scale_preconditioner!, preprocess.jl:173-297)
+// SpMV row-schedule autotuner.  For G and G' in precision T, every lanes-per-row
+// L in {1..64} and workgroup cap in {1024..8192} is timed with the attempt
+// kernels' access pattern (launch_spmv_sched: grid-stride, L lanes per row,
+// FP64 accumulation) and the fastest is kept for the problem's lifetime.  The
+// fused attempt kernels share that pattern, so the probe ranks their
+// schedules; the residual kernels keep the fixed heuristic.
+constexpr int kTuneMaxGrid = 8192;
+template <typename T>
+int autotune(pdcs_problem *P, bool enable) {
+    auto &sc = P->sched[std::is_same<T, float>::value ? 1 : 0];
+    if (sc.tuned || !enable) return PDCS_OK;
+    IterSet<T> &I = iters<T>(P);
+    hipStream_t s = P->ctx->stream;
+    hipEvent_t e0, e1;
+    HIPCHK(P->ctx, hipEventCreate(&e0));
+    HIPCHK(P->ctx, hipEventCreate(&e1));
+    auto tune = [&](Csr<T> A, const T *x, T *y, int &lanes, int &grid) -> int {
+        float best = INFINITY;
+        for (int L = 1; L <= 64; L <<= 1) {
+            const int64_t need = (A.rows * L + kBlock - 1) / kBlock;
+            for (int g = 1024; g <= kTuneMaxGrid; g <<= 1) {
+                if (g > 1024 && need <= g / 2) break;   // a larger cap launches the same grid
+                A.lanes = L;
+                A.grid = g;
+                launch_spmv_sched<T>(s, A, x, y);
+                launch_spmv_sched<T>(s, A, x, y);
+                HIPCHK(P->ctx, hipEventRecord(e0, s));
+                for (int r = 0; r < 10; ++r) launch_spmv_sched<T>(s, A, x, y);
+                HIPCHK(P->ctx, hipEventRecord(e1, s));
+                HIPCHK(P->ctx, hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIPCHK(P->ctx, hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) { best = ms; lanes = L; grid = g; }
+            }
+        }
+        return PDCS_OK;
+    };
+    int rc = tune(Csr<T>{P->d_rowptr, P->d_col, I.vals, P->m, 1, 0}, I.tn1, I.tm1, sc.lanesG, sc.gridG);
+    if (!rc) rc = tune(Csr<T>{P->d_trowptr, P->d_tcol, I.tvals, P->n, 1, 0}, I.tm1, I.tn1, sc.lanesGt, sc.gridGt);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    HIPCHK(P->ctx, hipGetLastError());
+    sc.tuned = true;
+    if (P->verbose_tune)
+        std::fprintf(stderr, "[pdcs] %s SpMV schedule: G lanes %d cap %d, G' lanes %d cap %d\n",
+                     std::is_same<T, float>::value ? "FP32" : "FP64", sc.lanesG, sc.gridG, sc.lanesGt, sc.gridGt);
+    return PDCS_OK;
+}
+
 int finish_scaling(pdcs_problem *P) {
     std::vector<double> Dl(P->m), d(P->m, 1.0), d2(P->m);
     HIPCHK(P->ctx, copy_sync(P->ctx, Dl.data(), P->Dl, sizeof(double) * P->m, hipMemcpyDeviceToHost));
@@ -588,8 +646,9 @@ struct Solver {
     Solver(pdcs_problem *P_, const pdcs_params *p_) : P(P_), prm(p_), I(iters<T>(P_)), s(P_->ctx->stream) {
         n = P->n;
         m = P->m;
-        G = Csr<T>{P->d_rowptr, P->d_col, I.vals, m, P->lanesG};
-        Gt = Csr<T>{P->d_trowptr, P->d_tcol, I.tvals, n, P->lanesGt};
+        const auto &sc = P->sched[std::is_same<T, float>::value ? 1 : 0];
+        G = Csr<T>{P->d_rowptr, P->d_col, I.vals, m, sc.lanesG, sc.gridG};
+        Gt = Csr<T>{P->d_trowptr, P->d_tcol, I.tvals, n, sc.lanesGt, sc.gridGt};
         PV = ProbVec<T>{I.c, I.h, I.bl, I.bu, n, m, P->nb};
     }
 
@@ -1024,6 +1083,14 @@ double max_abs_vals(pdcs_problem *P) {
 
 }  // namespace
 
+template <typename Sc>
+static void fill_sched(const Sc &sc, pdcs_result *res) {
+    res->sched[0] = sc.lanesG;
+    res->sched[1] = sc.lanesGt;
+    res->sched[2] = sc.gridG;
+    res->sched[3] = sc.gridGt;
+}
+
 extern "C" {
 
 int pdcs_problem_create_host(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double *c, const int32_t *rowptr,
@@ -1090,8 +1157,9 @@ int pdcs_solve(pdcs_problem *P, const pdcs_params *prm_in, const double *x0, con
     int rc;
     // attempt partial buffers (per-block sums of the attempt chain)
     if (!P->apart) {
-        const int64_t bp = (P->n * P->lanesGt + kBlock - 1) / kBlock;
-        const int64_t bd = (P->m * P->lanesG + kBlock - 1) / kBlock +
+        // attempt kernels launch at most min(rows * 64 / kBlock, kTuneMaxGrid) workgroups
+        const int64_t bp = std::min<int64_t>((P->n * 64 + kBlock - 1) / kBlock, kTuneMaxGrid);
+        const int64_t bd = std::min<int64_t>((P->m * 64 + kBlock - 1) / kBlock, kTuneMaxGrid) +
                            (P->K.nsoc_small * (int64_t)P->K.soc_small_group + kBlock - 1) / kBlock + P->K.nbig_blocks +
                            ((int64_t)P->K.nexp + P->K.ndexp + kBlock - 1) / kBlock + 8;
         P->apart_stride = std::max(bp, bd) + 64;
@@ -1109,6 +1177,9 @@ int pdcs_solve(pdcs_problem *P, const pdcs_params *prm_in, const double *x0, con
     }
     if ((rc = alloc_iters<double>(P))) return rc;
     if (mixed && (rc = alloc_iters<float>(P))) return rc;
+    P->verbose_tune = prm.verbose != 0;
+    if ((rc = autotune<double>(P, prm.autotune != 0))) return rc;
+    if (mixed && (rc = autotune<float>(P, prm.autotune != 0))) return rc;
     const double ginf = max_abs_vals(P);
     res->time_setup = now_s() - t_setup;
     const double t0 = now_s();
@@ -1144,6 +1215,7 @@ int pdcs_solve(pdcs_problem *P, const pdcs_params *prm_in, const double *x0, con
             res->restarts = S32.restart_used;
             res->omega = S32.omega;
             res->time_solve = now_s() - t0;
+            fill_sched(P->sched[1], res);
             store_solution<float>(P, S32.exit_seq ? P->f32.it.xm : P->f32.it.x, S32.exit_seq ? P->f32.it.ym : P->f32.it.y);
             return PDCS_OK;
         }
@@ -1178,6 +1250,7 @@ int pdcs_solve(pdcs_problem *P, const pdcs_params *prm_in, const double *x0, con
     res->restarts = S64.restart_used;
     res->omega = S64.omega;
     res->time_solve = now_s() - t0;
+    fill_sched(P->sched[0], res);
     res->time_kkt += S64.t_kkt;
     res->time_restart += S64.t_restart;
     store_solution<double>(P, S64.exit_seq ? P->f64.it.xm : P->f64.it.x, S64.exit_seq ? P->f64.it.ym : P->f64.it.y);
@@ -1193,8 +1266,9 @@ int time_kernels(pdcs_problem *P, int reps, double *ms, double *bytes) {
     hipStream_t s = ctx->stream;
     IterSet<T> &I = iters<T>(P);
     if (!I.ready || !P->apart) return fail(ctx, PDCS_ERR_STATE, "run pdcs_solve in this precision first");
-    Csr<T> G{P->d_rowptr, P->d_col, I.vals, P->m, P->lanesG};
-    Csr<T> Gt{P->d_trowptr, P->d_tcol, I.tvals, P->n, P->lanesGt};
+    const auto &sc = P->sched[std::is_same<T, float>::value ? 1 : 0];
+    Csr<T> G{P->d_rowptr, P->d_col, I.vals, P->m, sc.lanesG, sc.gridG};
+    Csr<T> Gt{P->d_trowptr, P->d_tcol, I.tvals, P->n, sc.lanesGt, sc.gridGt};
     ProbVec<T> PV{I.c, I.h, I.bl, I.bu, P->n, P->m, P->nb};
     const double sv = sizeof(T), n = (double)P->n, m = (double)P->m, nnz = (double)P->nnz, nb = (double)P->nb;
     const double melem = (double)(P->mzero + P->mnonneg), mc = m - melem;
