@@ -72,7 +72,7 @@ typedef struct {
     int64_t mixed_check_freq;         /* MIXED: FP32 phase check interval (200; 0 = check_terminate_freq) */
     int64_t polish_check_freq;        /* MIXED: FP64 polish check interval (200; 0 = check_terminate_freq) */
     double mixed_switch_tol;          /* MIXED: the FP32 phase hands over to the FP64 polish below this
-                                         README KKT (1e-5) or when a check fails to halve it */
+                                         README KKT (1e-5) or when a check improves it by less than 10% */
     int32_t autotune;                 /* 1: time the SpMV row schedules (lanes per row, workgroup cap) on
                                          the device at the first solve of each precision and keep the
                                          fastest; 0: fixed heuristic schedule */

@@ -32,6 +32,9 @@ struct BigSocState {
     int32_t done;
     int32_t iters;     // exact Newton steps taken by the fallback
     int32_t series;    // 1 if the series solve was accepted
+    // counters since the solve started (diagnostics): root finds, series accepted
+    // in the first / refine pass, exact fallbacks and their Newton steps
+    int64_t n_root, n_first, n_refine, n_fallback, n_newton;
 };
 constexpr int kBigElems = 4;                       // tail elements per thread in a pass
 constexpr int kBigBlockElems = kBigElems * 256;    // tail elements per workgroup
@@ -116,6 +119,7 @@ void launch_attempt(hipStream_t s, int slot, const Csr<T> &G, const Csr<T> &Gt,
 // update), 2 cone projections, 3 G'dy dot, 4 step control, 5 reflection
 constexpr int kAttemptKernels = 6;
 constexpr unsigned kAttemptMaxGrid = 2048;   // default workgroup cap of the fused SpMV kernels (grid-stride beyond)
+constexpr int kTuneMaxGrid = 8192;           // largest cap the schedule autotuner tries
 unsigned attempt_grid(int64_t rows, int lanes, int cap);
 template <typename T>
 void launch_attempt_part(int part_id, hipStream_t s, int slot, const Csr<T> &G, const Csr<T> &Gt,

@@ -336,12 +336,13 @@ __device__ __forceinline__ void big_newton_update(BigSocState &S, double Ssum, d
     bool done = false;
     S.t = soc_newton_step(S.t, Ssum, S1sum, S.tL, S.tR, done);
     S.iters += 1;
+    S.n_newton += 1;
     S.done = done ? 1 : 0;
 }
 
 // Series solve of phi(t0 + h) = 0 from the moments; returns false if the
 // truncation bound is not met (the caller then falls back to exact Newton).
-__device__ bool big_series_solve(BigSocState &L, double t0, const double *M, double emax) {
+__device__ __forceinline__ bool big_series_solve(BigSocState &L, double t0, const double *M, double emax) {
     double c[kBigMoments];
     for (int k = 0; k < kBigMoments; ++k) c[k] = (double)(k + 1) * ((k & 1) ? -M[k] : M[k]);
     const double rad = 0.5 / emax;
@@ -379,8 +380,99 @@ __device__ bool big_series_solve(BigSocState &L, double t0, const double *M, dou
     return true;
 }
 
-// One moment pass.  refine == 0: first pass of a projection (case sums and
-// moments at the previous head t_prev); refine == 1: moments at the current
+// Per-element terms of a moment pass: case sums (first pass only) and the
+// moments of S around t0 (skipped when t0 is NaN).
+__device__ __forceinline__ void big_accumulate(double (&v)[kBigSums], double xv, double dd, double s, double t0,
+                                               bool first) {
+    const double a = dd * xv * xv;
+    if (first) {
+        v[0] += a;
+        v[1] += xv * xv / dd;
+        v[2] += xv * xv;
+        const double qv = xv / (1.0 + dd);
+        v[3] += dd * qv * qv;
+    }
+    if (t0 == t0) {
+        const double bb = 1.0 + dd;
+        const double iw = 1.0 / (t0 * bb - s * dd);
+        const double ei = bb * iw;
+        double term = a * iw * iw;
+#pragma unroll
+        for (int m = 0; m < kBigMoments; ++m) { v[4 + m] += term; term *= ei; }
+        v[4 + kBigMoments] = fmax(v[4 + kBigMoments], ei);
+    }
+}
+__device__ __forceinline__ void big_init(double (&v)[kBigSums], bool (&ismax)[kBigSums]) {
+#pragma unroll
+    for (int q = 0; q < kBigSums; ++q) { v[q] = 0.0; ismax[q] = (q == kBigSums - 1); }
+}
+// warm start of a first pass: the previous projection's head when it lies
+// strictly right of max(s, 0), else none (NaN)
+__device__ __forceinline__ double big_t0(const BigSocState *S, double s) {
+    const double tp = S->t_prev;
+    return (isfinite(tp) && tp > fmax(s, 0.0)) ? tp : NAN;
+}
+
+// Case decision (first pass) and series solve of a cone's projection from the
+// reduced sums `tot` of a moment pass; updates L.
+__device__ __forceinline__ void big_decide(BigSocState &L, const double *tot, double s, double t0, bool refine) {
+    if (!refine) {
+        L.s = s;
+        L.iters = 0;
+        L.series = 0;
+        L.done = 1;
+        const SocSums z = {tot[0], tot[1], tot[2]};
+        L.cs = soc_case(s, z);
+        if (L.cs == 2) L.t_closed = sqrt(tot[3]);
+        if (L.cs == 3) {
+            L.tL = fmax(s, 0.0);
+            L.tR = sqrt(s * s + tot[2]);
+            L.done = 0;
+            L.t = L.tL;
+            L.n_root += 1;
+        }
+    }
+    if (L.cs == 3 && !L.done && t0 == t0 && t0 < L.tR && tot[4] > 0.0) {
+        if (big_series_solve(L, t0, tot + 4, tot[4 + kBigMoments])) {
+            L.done = 1;
+            L.series = 1;
+            if (refine) L.n_refine += 1; else L.n_first += 1;
+        }
+    }
+}
+
+// Block-reduced sums `v` of this workgroup go to parts[slot]; the last of the
+// `np` workgroups of `cone` to arrive reduces parts[0..np) in fixed order and
+// decides (big_decide).
+__device__ __forceinline__ void big_conclude(const ConeLayout &K, int cone, double (&v)[kBigSums], const bool (&ismax)[kBigSums],
+                             double *parts, int slot, int np, double s, double t0, bool refine, double *lds,
+                             bool *flag) {
+    block_reduce_multi<kBigSums>(v, ismax, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < kBigSums; ++q) parts[(int64_t)slot * kBigSums + q] = v[q];
+    if (!big_last_arrival(K.big_count + cone, (unsigned)np, flag)) return;
+    double tot[kBigSums];
+#pragma unroll
+    for (int q = 0; q < kBigSums; ++q) tot[q] = 0.0;
+    for (int j = threadIdx.x; j < np; j += kBlock) {
+#pragma unroll
+        for (int q = 0; q < kBigSums; ++q) {
+            const double pv = __builtin_nontemporal_load(&parts[(int64_t)j * kBigSums + q]);
+            tot[q] = ismax[q] ? fmax(tot[q], pv) : tot[q] + pv;
+        }
+    }
+    block_reduce_multi<kBigSums>(tot, ismax, lds);
+    if (threadIdx.x != 0) return;
+    K.big_count[cone] = 0u;
+    BigSocState *S = K.big_state + cone;
+    BigSocState L = *S;
+    big_decide(L, tot, s, t0, refine);
+    *S = L;
+}
+
+// One moment pass over the big cones.  refine == 0: case sums and
+// moments at the previous head t_prev; refine == 1: moments at the current
 // iterate L.t of a projection whose series solve was not accepted (no-op
 // once done).
 template <typename T>
@@ -396,86 +488,18 @@ __global__ __launch_bounds__(kBlock) void k_bigsoc_pass(ConeLayout K, const doub
     const int64_t st0 = K.soc_big_start[cone];
     const int k = K.soc_big_dim[cone] - 1;
     const double s = refine ? S->s : (double)y[st0];
-    double t0;
-    if (refine) {
-        t0 = S->t;
-    } else {
-        const double tp = S->t_prev;
-        t0 = (isfinite(tp) && tp > fmax(s, 0.0)) ? tp : NAN;
-    }
+    const double t0 = refine ? S->t : big_t0(S, s);
     double v[kBigSums];
     bool ismax[kBigSums];
-#pragma unroll
-    for (int q = 0; q < kBigSums; ++q) { v[q] = 0.0; ismax[q] = (q == kBigSums - 1); }
+    big_init(v, ismax);
     const int e0 = K.big_blk_first[b];
     for (int q = 0; q < kBigElems; ++q) {
         const int e = e0 + q * kBlock + threadIdx.x;
-        if (e < k) {
-            const double xv = (double)y[st0 + 1 + e], dd = d2[st0 + 1 + e];
-            const double a = dd * xv * xv;
-            if (!refine) {
-                v[0] += a;
-                v[1] += xv * xv / dd;
-                v[2] += xv * xv;
-                const double qv = xv / (1.0 + dd);
-                v[3] += dd * qv * qv;
-            }
-            if (t0 == t0) {
-                const double bb = 1.0 + dd;
-                const double iw = 1.0 / (t0 * bb - s * dd);
-                const double ei = bb * iw;
-                double term = a * iw * iw;
-#pragma unroll
-                for (int m = 0; m < kBigMoments; ++m) { v[4 + m] += term; term *= ei; }
-                v[4 + kBigMoments] = fmax(v[4 + kBigMoments], ei);
-            }
-        }
+        if (e < k) big_accumulate(v, (double)y[st0 + 1 + e], d2[st0 + 1 + e], s, t0, !refine);
     }
-    block_reduce_multi<kBigSums>(v, ismax, lds);
-    double *part = K.big_part + (int64_t)b * kBigSums;
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int q = 0; q < kBigSums; ++q) part[q] = v[q];
-    if (!big_last_arrival(K.big_count + cone, (unsigned)K.big_cone_nblk[cone], &flag)) return;
-    double tot[kBigSums];
-    {
-        const int b0 = K.big_cone_blk0[cone], nb = K.big_cone_nblk[cone];
-#pragma unroll
-        for (int q = 0; q < kBigSums; ++q) tot[q] = 0.0;
-        for (int j = threadIdx.x; j < nb; j += kBlock) {
-#pragma unroll
-            for (int q = 0; q < kBigSums; ++q) {
-                const double pv = __builtin_nontemporal_load(&K.big_part[(int64_t)(b0 + j) * kBigSums + q]);
-                tot[q] = ismax[q] ? fmax(tot[q], pv) : tot[q] + pv;
-            }
-        }
-        block_reduce_multi<kBigSums>(tot, ismax, lds);
-    }
-    if (threadIdx.x != 0) return;
-    K.big_count[cone] = 0u;
-    BigSocState L = *S;
-    if (!refine) {
-        L.s = s;
-        L.iters = 0;
-        L.series = 0;
-        L.done = 1;
-        const SocSums z = {tot[0], tot[1], tot[2]};
-        L.cs = soc_case(s, z);
-        if (L.cs == 2) L.t_closed = sqrt(tot[3]);
-        if (L.cs == 3) {
-            L.tL = fmax(s, 0.0);
-            L.tR = sqrt(s * s + tot[2]);
-            L.done = 0;
-            L.t = L.tL;
-        }
-    }
-    if (L.cs == 3 && !L.done && t0 == t0 && t0 < L.tR && tot[4] > 0.0) {
-        if (big_series_solve(L, t0, tot + 4, tot[4 + kBigMoments])) {
-            L.done = 1;
-            L.series = 1;
-        }
-    }
-    *S = L;
+    const int b0 = K.big_cone_blk0[cone];
+    big_conclude(K, cone, v, ismax, K.big_part + (int64_t)b0 * kBigSums, b - b0, K.big_cone_nblk[cone], s, t0,
+                 refine != 0, lds, &flag);
 }
 
 // Fallback: one workgroup finishes the exact Newton iteration (no-op when done).
@@ -490,7 +514,10 @@ __global__ __launch_bounds__(1024) void k_bigsoc_finish(ConeLayout K, const doub
     if (S->done) return;
     const int64_t st0 = K.soc_big_start[cone];
     const int k = K.soc_big_dim[cone] - 1;
-    if (threadIdx.x == 0) L = *S;
+    if (threadIdx.x == 0) {
+        L = *S;
+        L.n_fallback += 1;
+    }
     __syncthreads();
     const double s = L.s;
     for (int it = 0; it < 200; ++it) {
@@ -731,7 +758,7 @@ static inline int cone_part_blocks(const ConeLayout &K) {
 
 template <typename T>
 static void launch_cones(hipStream_t s, const ConeLayout &K, const double *d2, T *y, const T *ys,
-                       T *dy, StepState *st, int slot, double *part) {
+                         T *dy, StepState *st, int slot, double *part) {
     if (K.nsoc_small > 0) {
         const unsigned nb = (unsigned)soc_blocks(K);
 #define SOCG(G) k_soc_group<T, G><<<nb, kBlock, 0, s>>>(K, d2, y, ys, dy, st, slot, part)

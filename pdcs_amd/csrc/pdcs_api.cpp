@@ -516,7 +516,6 @@ int build_problem(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double 
 // FP64 accumulation) and the fastest is kept for the problem's lifetime.  The
 // fused attempt kernels share that pattern, so the probe ranks their
 // schedules; the residual kernels keep the fixed heuristic.
-constexpr int kTuneMaxGrid = 8192;
 template <typename T>
 int autotune(pdcs_problem *P, bool enable) {
     auto &sc = P->sched[std::is_same<T, float>::value ? 1 : 0];
@@ -769,8 +768,11 @@ struct Solver {
         if (prm->verbose && P->K.nsoc_big > 0 && std::getenv("PDCS_TRACE")) {
             BigSocState b{};
             copy_sync(P->ctx, &b, P->K.big_state, sizeof(b), hipMemcpyDeviceToHost);
-            fprintf(stderr, "[pdcs trace] big cone 0: case %d done %d series %d newton iters %d t %.17g s %.17g\n", b.cs,
-                    b.done, b.series, b.iters, b.t, b.s);
+            fprintf(stderr,
+                    "[pdcs trace] big cone 0: case %d done %d series %d newton iters %d t %.17g s %.17g | root finds %ld "
+                    "series first %ld refine %ld fallback %ld (newton steps %ld)\n",
+                    b.cs, b.done, b.series, b.iters, b.t, b.s, (long)b.n_root, (long)b.n_first, (long)b.n_refine,
+                    (long)b.n_fallback, (long)b.n_newton);
         }
         if (prm->verbose)
             fprintf(stderr, "[pdcs %s] iter %ld relp %.3e reld %.3e gap %.3e pobj %.10e dobj %.10e restarts %ld\n",
@@ -962,9 +964,10 @@ struct Solver {
                     h.dec_next = std::min(0.9 + 0.1 * std::log10(max_kkt * 1e5), 1.0);
                     if (stop_tol > 0) {
                         // FP32 phase: hand over to the FP64 polish once the README KKT is
-                        // below stop_tol or stops improving (the FP32 rounding floor)
+                        // below stop_tol or improves by less than 10% over a check
+                        // interval (the FP32 rounding floor)
                         const double kk = std::min(kkt_readme(ci[0]), kkt_readme(ci[1]));
-                        const bool stalled = n_checks >= 2 && kk > 0.5 * best_kkt;
+                        const bool stalled = n_checks >= 2 && kk > 0.9 * best_kkt;
                         best_kkt = std::min(best_kkt, kk);
                         n_checks++;
                         if (kk < stop_tol || stalled) {
@@ -1199,7 +1202,8 @@ int pdcs_solve(pdcs_problem *P, const pdcs_params *prm_in, const double *x0, con
         if ((rc = conv_info<float>(P, P->f32.it.xs, P->f32.it.ys, S32.ci[0]))) return rc;
         S32.omega = S32.init_primal_weight();
         const int64_t cf = prm.mixed_check_freq > 0 ? prm.mixed_check_freq : prm.check_terminate_freq;
-        if ((rc = S32.run(0, max_total, prm.mixed_switch_tol, cf))) return rc;
+        // without a polish the FP32 phase runs to the tolerance or a limit
+        if ((rc = S32.run(0, max_total, prm.polish ? prm.mixed_switch_tol : 0.0, cf))) return rc;
         iters_total = P->h_st->iter;
         res->iter_fp32 = iters_total;
         res->time_fp32 = now_s() - t0;
@@ -1209,7 +1213,7 @@ int pdcs_solve(pdcs_problem *P, const pdcs_params *prm_in, const double *x0, con
             // finished in FP32 (optimal or limit) or no polish requested
             const ConvInfo &c = S32.ci[S32.exit_seq];
             fill_result(c, res);
-            res->exit_code = S32.exit_code == 10 ? 0 : S32.exit_code;
+            res->exit_code = S32.exit_code;
             res->sequence = S32.exit_seq;
             res->iter = iters_total;
             res->restarts = S32.restart_used;
