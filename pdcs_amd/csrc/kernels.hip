@@ -20,63 +20,12 @@
 // are accumulated in double.  Reductions are two-stage and in fixed order, so
 // results are run-to-run deterministic.
 #include "internal.h"
+#include "device_common.h"
 #include "exp_cone.h"
 
 #include <math.h>
 
 namespace pdcs {
-
-// ------------------------------------------------------------ reductions --
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-template <int G>
-__device__ __forceinline__ double group_sum(double v) {
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
-    return v;
-}
-// block-wide sum over kBlock threads; result valid in thread 0
-__device__ __forceinline__ double block_sum(double v, double *lds) {
-    v = wave_sum(v);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) lds[w] = v;
-    __syncthreads();
-    double r = 0.0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += lds[i];
-    __syncthreads();
-    return r;
-}
-__device__ __forceinline__ double block_max(double v, double *lds) {
-    v = wave_max(v);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) lds[w] = v;
-    __syncthreads();
-    double r = 0.0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r = fmax(r, lds[i]);
-    __syncthreads();
-    return r;
-}
-// block-wide sum broadcast to all threads
-__device__ __forceinline__ double block_sum_all(double v, double *lds) {
-    v = wave_sum(v);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) lds[w] = v;
-    __syncthreads();
-    double r = 0.0;
-    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += lds[i];
-    __syncthreads();
-    return r;
-}
 
 // Block-wide reduction of N values in one LDS round (kBlock threads): entries
 // with ismax[q] take the maximum, the others the sum; results broadcast to all.
@@ -172,61 +121,6 @@ __global__ __launch_bounds__(kBlock) void k_dual(Csr<T> G, ProbVec<T> P, ConeLay
     }
     d2 = block_sum(d2, lds);
     if (threadIdx.x == 0) part[blockIdx.x] = d2;
-}
-
-// Scaled-SOC projection onto {(t, z) : ||d .* z|| <= t} of (s, x).
-//
-// With multiplier lambda >= 0 the KKT conditions give z = x / (1 + lambda d^2)
-// and t = s / (1 - lambda).  Writing lambda = 1 - s/t turns the boundary
-// condition ||d z|| = t into phi(t) = S(t)^(-1/2) - 1 = 0 with
-// S(t) = sum_i a_i / w_i(t)^2, a_i = d_i^2 x_i^2, w_i = t (1 + d_i^2) - s d_i^2.
-// S^(-1/2) is a power mean of exponent -2 of the affine w_i(t), hence concave
-// and increasing in t: Newton started left of the root (t = max(s, 0), where
-// phi < 0 whenever the point is neither interior nor polar) increases
-// monotonically to the root with no overshoot.  A bracket [tL, tR] with
-// tR = ||(s, x)|| (non-expansiveness) guards round-off.  This replaces the
-// reference's bisection on xi = lambda/2 (soc_proj_diagonal!,
-// src/pdcs_cpu/soc_rsoc_proj.jl:257-373) with the same fixed point.
-struct SocSums {
-    double nd2, ni2, nx2;   // ||d x||^2, ||x/d||^2, ||x||^2
-};
-
-// per-element helpers shared by the group and block variants
-__device__ __forceinline__ void soc_newton_terms(double t, double s, double x, double dd2,
-                                                 double &S, double &S1) {
-    const double a = dd2 * x * x;
-    const double w = t * (1.0 + dd2) - s * dd2;
-    const double iw = 1.0 / w;
-    const double q = a * iw * iw;
-    S += q;
-    S1 += q * (1.0 + dd2) * iw;
-}
-
-// Decide the case from the sums; returns 0 interior, 1 polar (zero), 2 s==0
-// closed form, 3 Newton.
-__device__ __forceinline__ int soc_case(double s, const SocSums &z) {
-    if (sqrt(z.ni2) <= -s) return 1;
-    if (sqrt(z.nd2) <= s) return 0;
-    if (s == 0.0) return 2;
-    return 3;
-}
-
-// One safeguarded Newton update of phi(t) = S^(-1/2) - 1 given S and S1 = sum a (1+d^2)/w^3
-// at t (phi' = S1 / S^(3/2)).  Convergence: phi == 0 keeps t; a Newton step
-// shorter than 1e-8 t leaves an error ~ (1e-8)^2 t by quadratic convergence, so
-// its result is final; a bracket narrower than 4 ulps ends a bisection tail.
-__device__ __forceinline__ double soc_newton_step(double t, double S, double S1, double &tL,
-                                                  double &tR, bool &done) {
-    const double isq = 1.0 / sqrt(S);
-    const double phi = isq - 1.0;
-    if (phi == 0.0) { done = true; return t; }
-    if (phi < 0.0) tL = fmax(tL, t); else tR = fmin(tR, t);
-    const double dphi = S1 * isq * isq * isq;
-    double tn = t - phi / dphi;
-    bool newton = true;
-    if (!(tn >= tL && tn <= tR)) { tn = 0.5 * (tL + tR); newton = false; }
-    done = (newton && fabs(tn - t) <= 1e-8 * fabs(tn)) || (tR - tL) <= 4.5e-16 * fabs(tR);
-    return tn;
 }
 
 template <typename T, int G>

@@ -97,7 +97,8 @@ struct IterSet {
     Iter<T> it{};
     T *x0 = nullptr, *y0 = nullptr, *x0l = nullptr, *y0l = nullptr;   // restart anchors
     T *tn1 = nullptr, *tn2 = nullptr, *tm1 = nullptr, *tm2 = nullptr;  // scratch
-    T *h1 = nullptr, *h2 = nullptr, *xd = nullptr, *yd = nullptr;
+    T *h1 = nullptr, *h2 = nullptr, *xd = nullptr, *yd = nullptr;   // xd/yd: epoch start, then its difference
+    T *rx = nullptr, *ry = nullptr;                                   // restart-check differences
     T *c = nullptr, *h = nullptr, *bl = nullptr, *bu = nullptr;        // scaled data
     T *vals = nullptr, *tvals = nullptr;
     bool ready = false;
@@ -356,6 +357,7 @@ int alloc_iters(pdcs_problem *P) {
     I.x0 = M.alloc<T>(n); I.y0 = M.alloc<T>(m); I.x0l = M.alloc<T>(n); I.y0l = M.alloc<T>(m);
     I.tn1 = M.alloc<T>(n); I.tn2 = M.alloc<T>(n); I.tm1 = M.alloc<T>(m); I.tm2 = M.alloc<T>(m);
     I.h1 = M.alloc<T>(n); I.h2 = M.alloc<T>(m); I.xd = M.alloc<T>(n); I.yd = M.alloc<T>(m);
+    I.rx = M.alloc<T>(n); I.ry = M.alloc<T>(m);
     I.c = M.alloc<T>(n); I.h = M.alloc<T>(m); I.bl = M.alloc<T>(P->nb); I.bu = M.alloc<T>(P->nb);
     if (std::is_same<T, double>::value) {
         I.vals = (T *)P->d_vals;
@@ -367,7 +369,7 @@ int alloc_iters(pdcs_problem *P) {
         launch_convert(P->ctx->stream, (float *)I.vals, P->d_vals, P->nnz);
         launch_convert(P->ctx->stream, (float *)I.tvals, P->d_tvals, P->nnz);
     }
-    if (!I.it.x || !I.h2 || !I.bu || !I.yd) return fail(P->ctx, PDCS_ERR_HIP, "iterate allocation failed");
+    if (!I.it.x || !I.h2 || !I.bu || !I.yd || !I.rx || !I.ry) return fail(P->ctx, PDCS_ERR_HIP, "iterate allocation failed");
     // scaled data in precision T: c/Dr, h/Dl, bl*Dr, bu*Dr
     std::vector<double> hDr(P->n), hDl(P->m);
     HIPCHK(P->ctx, copy_sync(P->ctx, hDr.data(), P->Dr, sizeof(double) * P->n, hipMemcpyDeviceToHost));
@@ -796,9 +798,9 @@ struct Solver {
     void restart_threshold(bool dg) {
         const double t0 = now_s();
         if (dg) {
-            launch_sub<T>(s, I.xd, I.x0, I.x0l, n);
-            launch_sub<T>(s, I.yd, I.y0, I.y0l, m);
-            const double r = Mnorm2(I.xd, I.yd, tau, sigma);
+            launch_sub<T>(s, I.rx, I.x0, I.x0l, n);
+            launch_sub<T>(s, I.ry, I.y0, I.y0l, m);
+            const double r = Mnorm2(I.rx, I.ry, tau, sigma);
             dg_threshold = binary_search(r, I.x0, I.y0, tau, sigma);
             if (dg_threshold < 0) restart_dg_flag = false;
         } else {
@@ -813,9 +815,11 @@ struct Solver {
         bool res = false;
         Iter<T> &it = I.it;
         if (dg) {
-            launch_sub<T>(s, I.xd, it.xs, I.x0, n);
-            launch_sub<T>(s, I.yd, it.ys, I.y0, m);
-            const double r = Mnorm2(I.xd, I.yd, tau, sigma);
+            // own buffers: xd/yd hold the epoch start for the primal weight update
+            // (Dr_temp / Dl_temp in restart_condition_check_diagonal!, scaling.jl:397-398)
+            launch_sub<T>(s, I.rx, it.xs, I.x0, n);
+            launch_sub<T>(s, I.ry, it.ys, I.y0, m);
+            const double r = Mnorm2(I.rx, I.ry, tau, sigma);
             const double rho = binary_search(r, it.xs, it.ys, tau, sigma);
             const double rho_m = binary_search(r, it.xm, it.ym, tau, sigma);
             for (int cond = 0; cond < 2 && !res; ++cond) {

@@ -233,6 +233,40 @@ def workload1(seed: int = 1001, samples: int = 20000, features: int = 2000, dens
                         mnonneg=N, nexp=2 * N, name=f"logreg_{N}x{p}")
 
 
+def portfolio(n: int, rng, gamma: float = 1.0, name: str = "") -> ConicProblem:
+    """One factor-model portfolio: k = n // 10 factors, F ~ N(0, 1/k) (n x k),
+    D = diag(U(0.01, 0.1)), mu ~ N(0.05, 0.02^2);
+        min -mu'x + gamma t  s.t.  (t, F'x, D^(1/2) x) in SOC(1 + k + n),
+                                   1'x = 1,  0 <= x <= 0.2.
+    Variables (x, t) with x boxed (nb = n) and t free.  Rows: the budget row
+    (zero cone), then the SOC block [t; F'x; D^(1/2) x]."""
+    k = max(1, n // 10)
+    F = rng.standard_normal((n, k)) / np.sqrt(k)
+    D = rng.uniform(0.01, 0.1, n)
+    mu = rng.normal(0.05, 0.02, n)
+    N = n + 1
+    m = 1 + 1 + k + n
+    rows = [np.zeros(n, np.int64), np.array([1]), np.repeat(np.arange(2, 2 + k), n), 2 + k + np.arange(n)]
+    cols = [np.arange(n), np.array([n]), np.tile(np.arange(n), k), np.arange(n)]
+    vals = [np.ones(n), np.array([1.0]), F.T.reshape(-1), np.sqrt(D)]
+    rowptr, colidx, vv = csr_from_coo(m, np.concatenate(rows), np.concatenate(cols), np.concatenate(vals))
+    h = np.zeros(m)
+    h[0] = 1.0
+    c = np.concatenate([-mu, [gamma]])
+    bl = np.zeros(n)
+    bu = np.full(n, 0.2)
+    return ConicProblem(n=N, m=m, nb=n, c=c, rowptr=rowptr, colidx=colidx, vals=vv, h=h, bl=bl, bu=bu,
+                        mzero=1, soc=[1 + k + n], name=name or f"portfolio_{n}")
+
+
+def workload3(count: int = 4096, seed: int = 1003, nmin: int = 20, nmax: int = 500) -> list:
+    """Batch of uneven portfolios: n_i ~ U{nmin, ..., nmax}, each built by
+    :func:`portfolio` from one shared seeded stream."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(nmin, nmax + 1, size=count)
+    return [portfolio(int(n), rng, name=f"portfolio_{i}_{n}") for i, n in enumerate(sizes)]
+
+
 def lp_box(n: int, m_eq: int, m_in: int, seed: int) -> ConicProblem:
     """Small feasible, bounded LP (zero + nonnegative cones, box primal) for linprog checks."""
     rng = np.random.default_rng(seed)
@@ -249,4 +283,4 @@ def lp_box(n: int, m_eq: int, m_in: int, seed: int) -> ConicProblem:
                         name=f"lp_{n}x{m}")
 
 
-WORKLOADS = {0: workload0, 1: workload1, 2: workload2}
+WORKLOADS = {0: workload0, 1: workload1, 2: workload2, 3: workload3}

@@ -300,3 +300,20 @@ def test_solve_logreg_small(ctx):
         res = prob.solve(precision=prec, time_limit=100.0)
         x, y, _ = prob.solution()
         _check_solution(P, res, x, y, ref)
+
+
+def test_two_epochs_match_oracle(ctx):
+    """A solve past the first duality-gap restart check (attempt 2000: forced
+    epoch break, primal weight update) into a second epoch and a restart:
+    iteration count, restarts, omega and objectives follow the oracle."""
+    P = instances.workload3(count=7)[6]          # portfolio, n = 437: 4000 attempts, 1 restart
+    r1, x1, y1 = oracle.solve(P)
+    prob = Problem.from_instance(P, ctx)
+    r2 = prob.solve(use_graph=0)
+    assert (r1["iter"], r1["restarts"]) == (r2["iter"], r2["restarts"]), (r1, r2)
+    assert r1["iter"] > 2000
+    # the trajectories drift apart at ~1e-5 by attempt 2000 (summation order);
+    # the omega of a mis-computed epoch difference would be off by O(1)
+    assert abs(r1["omega"] - r2["omega"]) <= 1e-4 * abs(r1["omega"]), (r1["omega"], r2["omega"])
+    for k in ("pobj", "dobj"):
+        assert abs(r1[k] - r2[k]) <= 1e-6 * (1 + abs(r1[k]))
