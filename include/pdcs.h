@@ -172,6 +172,46 @@ int pdcs_project_dual(pdcs_ctx *ctx, int32_t precision, const pdcs_cones *cones,
 /* In-place projection of r (m) onto K itself (con_proj!, codes 2/3/5/13/16). */
 int pdcs_project_primal_cone(pdcs_ctx *ctx, const pdcs_cones *cones, int64_t m, double *r);
 
+/* ---- batches of independent problems, solved in one persistent launch ----
+ * No counterpart in the reference, which solves instances one at a time
+ * (rpdhg_gpu_solve per instance, rpdhg_alg_gpu_gen.jl:1219): each instance is
+ * solved by one workgroup with the same algorithm and parameters, so a batch of
+ * small problems shares the GPU instead of leaving it idle.
+ *
+ * Flat input layout: instance i's arrays follow those of instances 0..i-1.
+ * c, bl, bu: n_i (bl/bu: nb_i) entries each; h: m_i; rowptr: m_i + 1 entries
+ * starting at 0 (instance-local CSR); colidx/vals: rowptr_i[m_i] entries with
+ * instance-local column indices; soc: nsoc_i dimensions.  Rows of each
+ * instance are in the cone order of pdcs_cones. */
+typedef struct pdcs_batch pdcs_batch;
+
+typedef struct {
+    int64_t n, m, nb;
+    int64_t mzero, mnonneg;
+    int64_t nsoc;            /* SOC blocks of this instance */
+    int64_t nexp, ndualexp;
+} pdcs_batch_shape;
+
+typedef struct {
+    int32_t exit_code;       /* as pdcs_result */
+    int32_t sequence;
+    int64_t iter, iter_fp32, restarts;
+    double pobj, dobj;
+    double rel_primal_res, rel_dual_res, rel_gap;   /* README l_inf residuals */
+    double time;             /* seconds the instance spent in the solve launch(es) */
+} pdcs_batch_result;
+
+int pdcs_batch_create_host(pdcs_ctx *ctx, int64_t count, const pdcs_batch_shape *shapes, const double *c,
+                           const int32_t *rowptr, const int32_t *colidx, const double *vals, const double *h,
+                           const double *bl, const double *bu, const int64_t *soc, pdcs_batch **out);
+int pdcs_batch_destroy(pdcs_batch *b);
+/* Solves every instance (params as pdcs_solve; PDCS_MIXED runs an FP32 launch
+ * then an FP64 polish launch over the instances it handed over).  res: count
+ * entries; *time_total: seconds from the first launch to the end of the last. */
+int pdcs_batch_solve(pdcs_batch *b, const pdcs_params *params, pdcs_batch_result *res, double *time_total);
+/* unscaled solutions of the last solve, flat: x sum n_i, y sum m_i */
+int pdcs_batch_get_solution(pdcs_batch *b, double *x, double *y);
+
 #ifdef __cplusplus
 }
 #endif

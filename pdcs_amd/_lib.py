@@ -56,11 +56,27 @@ class Result(C.Structure):
         return out
 
 
+class BatchShape(C.Structure):
+    _fields_ = [("n", c_i64), ("m", c_i64), ("nb", c_i64), ("mzero", c_i64), ("mnonneg", c_i64),
+                ("nsoc", c_i64), ("nexp", c_i64), ("ndualexp", c_i64)]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [("exit_code", c_i32), ("sequence", c_i32), ("iter", c_i64), ("iter_fp32", c_i64),
+                ("restarts", c_i64), ("pobj", C.c_double), ("dobj", C.c_double),
+                ("rel_primal_res", C.c_double), ("rel_dual_res", C.c_double), ("rel_gap", C.c_double),
+                ("time", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # every symbol declared in include/pdcs.h
 EXPORTS = ["pdcs_params_default", "pdcs_ctx_create", "pdcs_ctx_destroy", "pdcs_last_error",
            "pdcs_problem_create_host", "pdcs_problem_create_device", "pdcs_problem_destroy",
            "pdcs_solve", "pdcs_get_solution", "pdcs_get_scaling", "pdcs_evaluate", "pdcs_spmv",
-           "pdcs_project_dual", "pdcs_project_primal_cone", "pdcs_time_kernels"]
+           "pdcs_project_dual", "pdcs_project_primal_cone", "pdcs_time_kernels",
+           "pdcs_batch_create_host", "pdcs_batch_destroy", "pdcs_batch_solve", "pdcs_batch_get_solution"]
 
 _lib = None
 
@@ -97,6 +113,11 @@ def load():
     L.pdcs_project_dual.argtypes = [vp, c_i32, C.POINTER(Cones), c_i64, vp, vp]
     L.pdcs_project_primal_cone.argtypes = [vp, C.POINTER(Cones), c_i64, vp]
     L.pdcs_time_kernels.argtypes = [vp, c_i32, c_i32, c_dp, c_dp]
+    L.pdcs_batch_create_host.argtypes = [vp, c_i64, C.POINTER(BatchShape), c_dp, C.POINTER(c_i32),
+                                         C.POINTER(c_i32), c_dp, c_dp, c_dp, c_dp, C.POINTER(c_i64), C.POINTER(vp)]
+    L.pdcs_batch_destroy.argtypes = [vp]
+    L.pdcs_batch_solve.argtypes = [vp, C.POINTER(Params), C.POINTER(BatchResult), c_dp]
+    L.pdcs_batch_get_solution.argtypes = [vp, c_dp, c_dp]
     _lib = L
     return L
 

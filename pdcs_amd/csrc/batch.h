@@ -1,0 +1,108 @@
+// batch.h -- device layout of a batch of independent conic programs solved in
+// one persistent launch (batch.hip).  Every instance keeps its own CSR of G
+// and of G' with instance-local indices, its own cone list and its own
+// iterate vectors; all of them live in flat device arrays addressed through
+// the per-instance offsets of BatchInst.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pdcs {
+
+// Nonzeros per segment of the segmented CSR: a row longer than this is split
+// into several segments whose partial products are summed in fixed order, so
+// a workgroup's products are balanced over its threads whatever the row
+// lengths (the portfolio rows range from 1 to n nonzeros).
+constexpr int kSegNnz = 32;
+
+struct BatchInst {
+    int32_t n, m, nb, mzero, mnonneg, nsoc, nexp, ndexp;
+    int32_t nnz, nseg, ntseg, npart;   // npart = max(nseg, ntseg)
+    int64_t xo, yo;                    // offsets of the n- and m-vectors
+    int64_t ko;                        // offset of the nonzeros
+    int64_t ro, tro;                   // offsets of rowptr (m + 1) and trowptr (n + 1)
+    int64_t so;                        // offset of the SOC starts / dimensions
+    int64_t sgo, tsgo;                 // offsets of segk (nseg + 1) / segptr (m + 1), tsegk / tsegptr
+    int64_t po;                        // offset of the segment partials (npart)
+    double hN1, hNinf, cN1, cNinf;     // norms of the original h and c
+};
+
+// Solve controls of the batched kernel (the pdcs_params fields it uses).
+struct BatchParams {
+    double rel_tol, abs_tol;
+    double stop_tol;                   // > 0: FP32 phase, hand over below this README KKT or on stalling
+    int64_t check_freq;
+    int64_t restart_freq;              // duality-gap restart check interval (0 = off)
+    int64_t kkt_restart_freq;          // KKT restart check interval (0 = off)
+    int64_t max_outer, max_total;
+    int32_t use_kkt_restart, use_dg_restart, bisect_intent, code_primal_denominator;
+    int32_t polish_phase;              // 1: FP64 polish continuing from the FP32 phase's state
+    int32_t pad;
+    int64_t time_limit_ticks;          // wall-clock ticks per instance (0 = none)
+};
+
+// Per-instance result and the state handed from the FP32 phase to the polish.
+struct BatchOut {
+    int32_t exit_code, sequence;
+    int32_t polished, pad;             // 1: the FP64 polish produced the result
+    int64_t iter, iter_fp32, restarts;
+    double pobj, dobj, rel_p, rel_d, rel_gap, abs_p, abs_d, abs_gap, l2_rel_p, l2_rel_d, rel_p_code;
+    double omega, eta, tau, sigma, bs_t0;
+    double ci_l2max;                   // max of the l2 residual ratios of the last check (max_kkt)
+    uint64_t t0, t1, t2;               // wall clock: start, first-phase ticks, polish ticks
+};
+
+template <typename T>
+struct BatchVecs {                     // iterate vectors of one precision, flat over the batch
+    T *x, *xs, *xm, *xbar, *x0, *x0l, *xd, *rx, *tn1, *tn2, *h1;
+    T *y, *ys, *ym, *dy, *y0, *y0l, *yd, *ry, *tm1, *tm2, *h2;
+    T *c, *h, *bl, *bu;                // scaled data
+    const T *val, *tval;               // matrix values in T
+};
+
+// Read-only view of one instance in precision T (instance-local indices),
+// built on the host: the solver reads it through a wave-uniform pointer, so its
+// fields come in by scalar loads and each method loads only what it uses.
+template <typename T>
+struct BView {
+    int n, m, nb, mzero, cb, nsoc, nexp, ndexp, nseg, ntseg, eb;
+    const int32_t *rp, *cl, *sk, *sp, *trp, *tcl, *tsk, *tsp, *ss, *sd;
+    const T *val, *tval;
+    const double *val64, *tval64, *dsc2, *Dr, *Dl, *c_raw, *h_raw, *blf, *buf, *laml, *lamu;
+    const T *c, *h, *bl, *bu;
+    T *x, *xs, *xm, *xbar, *x0, *x0l, *xd, *rx, *tn1, *tn2, *h1;
+    T *y, *ys, *ym, *dy, *y0, *y0l, *yd, *ry, *tm1, *tm2, *h2;
+    double *part, *rtmp;
+    double hN1, hNinf, cN1, cNinf;
+};
+
+struct BatchData {
+    const BatchInst *inst;
+    int32_t count;
+    const int32_t *order;              // processing order of the work queue
+    unsigned int *next;                // work-queue counter
+    const int32_t *rowptr, *col, *segk, *segptr;
+    const int32_t *trowptr, *tcol, *tsegk, *tsegptr;
+    const int32_t *tmap;               // transposed slot -> original slot
+    double *val64, *tval64;            // FP64 values, scaled in place
+    const int32_t *soc_start, *soc_dim;
+    const double *c_raw, *h_raw, *bl_raw, *bu_raw;
+    double *Dr, *Dl, *dsc2, *rtmp, *part;
+    double *blf, *buf, *laml, *lamu;
+    double *rowv, *colv;               // scaling scratch (m, n)
+    BatchOut *out;
+};
+
+// launchers (batch.hip)
+void launch_batch_scale(hipStream_t s, const BatchData &B);
+template <typename T>
+void launch_batch_prepare(hipStream_t s, const BatchData &B, BatchVecs<T> V);
+template <typename T>
+void launch_batch_solve(hipStream_t s, const BatchData &B, const BView<T> *views, const BatchParams *p, int grid);
+void launch_batch_promote(hipStream_t s, const BatchData &B, BatchVecs<float> A, BatchVecs<double> D);
+template <typename T>
+void launch_batch_recover(hipStream_t s, const BatchData &B, BatchVecs<T> V, double *x_out, double *y_out,
+                          int polished_only);
+int batch_max_grid(int threads);
+
+}  // namespace pdcs

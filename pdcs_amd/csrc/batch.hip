@@ -1,0 +1,909 @@
+// batch.hip -- many independent small conic programs in one persistent launch.
+//
+// One workgroup solves one instance from start to finish: the same PDCS
+// algorithm as the single-problem path (Ruiz x10 + Pock-Chambolle scaling,
+// the adaptive-step PDHG attempt, reflection and eta-weighted averaging, the
+// duality-gap restart search, primal weight updates, README residual checks;
+// rpdhg_alg_gpu_gen_scaling.jl:19-117,270-307,354-468,656-896), with the
+// host's control flow restated as workgroup-uniform device code.  Workgroups
+// take instances from an atomic work queue (largest first), so a grid sized
+// to the resident capacity of the GPU stays busy however uneven the instances.
+//
+// Sparse products use the segmented CSR of batch.h: threads compute segment
+// partials (at most kSegNnz nonzeros each), then per-row sums in fixed order;
+// results are deterministic and balanced whatever the row lengths.  SOCs are
+// projected one wavefront per cone (wave-level Newton, no workgroup barrier),
+// exponential cones one lane per cone.  Scalars are replicated in every
+// thread from broadcast reductions, so all control flow is uniform.
+#include "batch.h"
+#include "device_common.h"
+#include "exp_cone.h"
+#include "internal.h"
+
+#include <math.h>
+
+// the per-method view prologue declares every field; unused ones are dropped
+#pragma clang diagnostic ignored "-Wunused-variable"
+
+namespace pdcs {
+
+#define BFOR(i, N) for (int i = (int)threadIdx.x; i < (N); i += kBlock)
+
+// ------------------------------------------------------------- scaling --
+// Ruiz l_inf (10 passes) then Pock-Chambolle alpha = 1 on one instance
+// (rescale_problem!, src/pdcs_gpu/preprocess.jl:347-575), exponential blocks
+// with one row scale each.  G and G' values are divided by the same product,
+// so they stay bitwise transposes of each other.
+__global__ __launch_bounds__(kBlock) void k_batch_scale(BatchData B) {
+    const int q = blockIdx.x;
+    const BatchInst I = B.inst[q];
+    const int32_t *rp = B.rowptr + I.ro, *cl = B.col + I.ko, *trp = B.trowptr + I.tro, *tcl = B.tcol + I.ko;
+    double *val = B.val64 + I.ko, *tval = B.tval64 + I.ko;
+    double *Dr = B.Dr + I.xo, *Dl = B.Dl + I.yo, *rowv = B.rowv + I.yo, *colv = B.colv + I.xo;
+    BFOR(j, I.n) Dr[j] = 1.0;
+    BFOR(i, I.m) Dl[i] = 1.0;
+    const int eb = I.m - 3 * (I.nexp + I.ndexp);
+    for (int pass = 0; pass < 11; ++pass) {
+        const bool sum = pass == 10;
+        BFOR(i, I.m) {
+            double v = 0.0;
+            for (int32_t k = rp[i]; k < rp[i + 1]; ++k) v = sum ? v + fabs(val[k]) : fmax(v, fabs(val[k]));
+            const double w = sqrt(v);
+            rowv[i] = w == 0.0 ? 1.0 : w;
+        }
+        BFOR(j, I.n) {
+            double v = 0.0;
+            for (int32_t k = trp[j]; k < trp[j + 1]; ++k) v = sum ? v + fabs(tval[k]) : fmax(v, fabs(tval[k]));
+            const double w = sqrt(v);
+            colv[j] = w == 0.0 ? 1.0 : w;
+        }
+        __syncthreads();
+        BFOR(b, I.nexp + I.ndexp) {
+            double *r = rowv + eb + 3 * b;
+            const double mx = fmax(r[0], fmax(r[1], r[2]));
+            r[0] = r[1] = r[2] = mx;
+        }
+        __syncthreads();
+        BFOR(i, I.m) {
+            const double dl = rowv[i];
+            for (int32_t k = rp[i]; k < rp[i + 1]; ++k) val[k] /= (dl * colv[cl[k]]);
+        }
+        BFOR(j, I.n) {
+            const double dr = colv[j];
+            for (int32_t k = trp[j]; k < trp[j + 1]; ++k) tval[k] /= (rowv[tcl[k]] * dr);
+        }
+        BFOR(i, I.m) Dl[i] *= rowv[i];
+        BFOR(j, I.n) Dr[j] *= colv[j];
+        __syncthreads();
+    }
+}
+
+// scaled data in precision T, SOC scales d^2 = (Dl[start] / Dl[i])^2, zero iterates
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_batch_prepare(BatchData B, BatchVecs<T> V) {
+    const int q = blockIdx.x;
+    const BatchInst I = B.inst[q];
+    const double *Dr = B.Dr + I.xo, *Dl = B.Dl + I.yo;
+    BFOR(j, I.n) {
+        const int64_t g = I.xo + j;
+        V.c[g] = (T)(B.c_raw[g] / Dr[j]);
+        if (j < I.nb) {
+            V.bl[g] = (T)(B.bl_raw[g] * Dr[j]);
+            V.bu[g] = (T)(B.bu_raw[g] * Dr[j]);
+        }
+        V.x[g] = V.xs[g] = V.xm[g] = V.x0[g] = V.x0l[g] = V.xbar[g] = (T)0;
+    }
+    BFOR(i, I.m) {
+        const int64_t g = I.yo + i;
+        V.h[g] = (T)(B.h_raw[g] / Dl[i]);
+        V.y[g] = V.ys[g] = V.ym[g] = V.y0[g] = V.y0l[g] = V.dy[g] = (T)0;
+        B.dsc2[g] = 1.0;
+    }
+    __syncthreads();
+    for (int c = 0; c < I.nsoc; ++c) {
+        const int st = B.soc_start[I.so + c], dim = B.soc_dim[I.so + c];
+        const double d0 = Dl[st];
+        for (int e = threadIdx.x; e < dim; e += kBlock) {
+            const double d = d0 / Dl[st + e];
+            B.dsc2[I.yo + st + e] = d * d;
+        }
+    }
+}
+
+__global__ void k_batch_f32(const double *a, float *b, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        b[i] = (float)a[i];
+}
+
+// ---------------------------------------------------------- the solver --
+struct BConv {   // converge_info_calculation_diagonal! ratios
+    double pobj, dobj, abs_gap, rel_gap, l2_abs_p, l2_rel_p, linf_abs_p, linf_rel_p, linf_rel_p_code;
+    double l2_abs_d, l2_rel_d, linf_abs_d, linf_rel_d;
+};
+
+struct BShared {
+    double red[(kBlock / 64) * 16];
+    BConv ci[2];
+};
+
+#define BV                                                                                                    \
+    const BView<T> &V_ = *vw;                                                                                 \
+    const int n = V_.n, m = V_.m, nb = V_.nb, mzero = V_.mzero, cb = V_.cb, nsoc = V_.nsoc, nexp = V_.nexp,   \
+              ndexp = V_.ndexp, nseg = V_.nseg, ntseg = V_.ntseg, eb = V_.eb;                                 \
+    const int32_t *const rp = V_.rp, *const cl = V_.cl, *const sk = V_.sk, *const sp = V_.sp,                 \
+                  *const trp = V_.trp, *const tcl = V_.tcl, *const tsk = V_.tsk, *const tsp = V_.tsp,         \
+                  *const ss = V_.ss, *const sd = V_.sd;                                                       \
+    const T *const val = V_.val, *const tval = V_.tval, *const c = V_.c, *const h = V_.h, *const bl = V_.bl,  \
+            *const bu = V_.bu;                                                                                \
+    const double *const val64 = V_.val64, *const tval64 = V_.tval64, *const dsc2 = V_.dsc2, *const Dr = V_.Dr, \
+                 *const Dl = V_.Dl, *const c_raw = V_.c_raw, *const h_raw = V_.h_raw, *const blf = V_.blf,    \
+                 *const buf = V_.buf, *const laml = V_.laml, *const lamu = V_.lamu;                           \
+    T *const x = V_.x, *const xs = V_.xs, *const xm = V_.xm, *const xbar = V_.xbar, *const x0 = V_.x0,        \
+      *const x0l = V_.x0l, *const xd = V_.xd, *const rx = V_.rx, *const tn1 = V_.tn1, *const tn2 = V_.tn2,    \
+      *const h1 = V_.h1;                                                                                      \
+    T *const y = V_.y, *const ys = V_.ys, *const ym = V_.ym, *const dy = V_.dy, *const y0 = V_.y0,            \
+      *const y0l = V_.y0l, *const yd = V_.yd, *const ry = V_.ry, *const tm1 = V_.tm1, *const tm2 = V_.tm2,    \
+      *const h2 = V_.h2;                                                                                      \
+    double *const part = V_.part, *const rtmp = V_.rtmp;                                                      \
+    const double hN1 = V_.hN1, hNinf = V_.hNinf, cN1 = V_.cN1, cNinf = V_.cNinf
+
+template <typename T>
+struct BSolve {
+    const BView<T> *vw;   // device memory, wave-uniform address
+    BShared *sh;
+    const BatchParams *P;
+    // solver state (identical in every thread)
+    double omega, tau, sigma, bs_t0, max_kkt, min_kkt, dg_threshold, kkt_threshold, kkt_err0, kkt_err1;
+    double best_kkt, eta, dec, dec_next, eta_cum, extra, eta_init;
+    int64_t iter, iter_stepsize, inner, count, restart_used;
+    int n_checks, exit_code, exit_seq;
+    bool restart_dg_flag;
+
+    __device__ void bar() { __syncthreads(); }
+
+    // ---- reductions (broadcast to every thread)
+    template <int N>
+    __device__ void reduce(double (&v)[N], const bool (&mx)[N]) { block_reduce_multi<N>(v, mx, sh->red); }
+    __device__ double dot(const T *a, const T *b, int len) {
+        BV;
+        double s = 0.0;
+        BFOR(i, len) s += (double)a[i] * (double)b[i];
+        double v[1] = {s};
+        const bool mx[1] = {false};
+        reduce<1>(v, mx);
+        return v[0];
+    }
+
+    // ---- segmented sparse products: part[s] = sum of segment s; row sums in fixed order
+    template <typename V, typename X>
+    __device__ void seg_products(const int32_t *segk, int ns, const int32_t *col, const V *vv, const X *xv) {
+        BV;
+        for (int s = threadIdx.x; s < ns; s += kBlock) {
+            double acc = 0.0;
+            for (int32_t k = segk[s]; k < segk[s + 1]; ++k) acc += (double)vv[k] * (double)xv[col[k]];
+            part[s] = acc;
+        }
+        __syncthreads();
+    }
+    __device__ double seg_row(const int32_t *segptr, int r) {
+        BV;
+        double acc = 0.0;
+        for (int32_t s = segptr[r]; s < segptr[r + 1]; ++s) acc += part[s];
+        return acc;
+    }
+    // out = G v (T values), out of length m
+    __device__ void spmv_G(const T *v, T *out) {
+        BV;
+        seg_products(sk, nseg, cl, val, v);
+        BFOR(i, m) out[i] = (T)seg_row(sp, i);
+        bar();
+    }
+    __device__ void spmv_Gt(const T *v, T *out) {
+        BV;
+        seg_products(tsk, ntseg, tcl, tval, v);
+        BFOR(j, n) out[j] = (T)seg_row(tsp, j);
+        bar();
+    }
+
+    // ---- projection of rows [cb, m) onto the scaled dual cone (codes 6/12/15);
+    // elementwise rows too when `elem`; dy = y - ys and sum dy^2 when dyv
+    __device__ double project(T *yv, const T *ysv, T *dyv, bool elem) {
+        BV;
+        double my2 = 0.0;
+        if (elem) BFOR(i, cb) {
+            if (i >= mzero) yv[i] = (T)fmax((double)yv[i], 0.0);
+        }
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for (int q = wave; q < nsoc; q += kBlock / 64) {
+            const int st0 = ss[q], k = sd[q] - 1;
+            T *yt = yv + st0 + 1;
+            const double *dt = dsc2 + st0 + 1;
+            const double s = (double)yv[st0];
+            SocSums z = {0, 0, 0};
+            for (int e = lane; e < k; e += 64) {
+                const double xv = (double)yt[e], dd = dt[e];
+                z.nd2 += dd * xv * xv;
+                z.ni2 += xv * xv / dd;
+                z.nx2 += xv * xv;
+            }
+            z.nd2 = wave_sum(z.nd2);
+            z.ni2 = wave_sum(z.ni2);
+            z.nx2 = wave_sum(z.nx2);
+            const int cs = soc_case(s, z);
+            double t = s, lam = 0.0;
+            if (cs == 2) {
+                double S = 0.0;
+                for (int e = lane; e < k; e += 64) {
+                    const double xv = (double)yt[e], dd = dt[e];
+                    const double qv = xv / (1.0 + dd);
+                    S += dd * qv * qv;
+                }
+                t = sqrt(wave_sum(S));
+                lam = 1.0;
+            } else if (cs == 3) {
+                double tL = fmax(s, 0.0), tR = sqrt(s * s + z.nx2);
+                t = tL;
+                bool done = false;
+                for (int it = 0; it < 100 && !done; ++it) {
+                    double S = 0.0, S1 = 0.0;
+                    for (int e = lane; e < k; e += 64) soc_newton_terms(t, s, (double)yt[e], dt[e], S, S1);
+                    S = wave_sum(S);
+                    S1 = wave_sum(S1);
+                    t = soc_newton_step(t, S, S1, tL, tR, done);
+                }
+                lam = 1.0 - s / t;
+            }
+            if (cs == 1) {
+                for (int e = lane; e <= k; e += 64) yv[st0 + e] = (T)0.0;
+            } else if (cs >= 2) {
+                if (lane == 0) yv[st0] = (T)t;
+                for (int e = lane; e < k; e += 64) yt[e] = (T)((double)yt[e] / (1.0 + lam * dt[e]));
+            }
+        }
+        BFOR(b, nexp + ndexp) {
+            T *v = yv + eb + 3 * b;
+            double a0 = (double)v[0], a1 = (double)v[1], a2 = (double)v[2];
+            if (b < nexp) exp_project_dual(a0, a1, a2);
+            else exp_project(a0, a1, a2);
+            v[0] = (T)a0;
+            v[1] = (T)a1;
+            v[2] = (T)a2;
+        }
+        bar();
+        if (dyv != nullptr) {
+            for (int i = cb + threadIdx.x; i < m; i += kBlock) {
+                const double d = (double)yv[i] - (double)ysv[i];
+                dyv[i] = (T)d;
+                my2 += d * d;
+            }
+        }
+        return my2;
+    }
+
+    // ---- residuals (converge_info_calculation_diagonal!) of (xv, yv) on the FP64 data
+    __device__ void conv(const T *xv, const T *yv, BConv &ci) {
+        BV;
+        double v[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        // 0 pobj 1 atyN1 2 dobj 3 sN1 4 d2 5 axN1 6 p2 | 7 atyInf 8 sInf 9 dInf 10 axInf 11 pInf 12 projInf
+        const bool mx[13] = {false, false, false, false, false, false, false, true, true, true, true, true, true};
+        seg_products(tsk, ntseg, tcl, tval64, yv);
+        BFOR(j, n) {
+            const double aty = seg_row(tsp, j) * Dr[j];
+            const double xr = (double)xv[j] / Dr[j];
+            const double lam = c_raw[j] - aty;
+            v[0] += c_raw[j] * xr;
+            v[1] += fabs(aty);
+            v[7] = fmax(v[7], fabs(aty));
+            double spv = 0.0;
+            if (j < nb) {
+                v[2] += blf[j] * fmax(lam, 0.0) + buf[j] * fmin(lam, 0.0);
+                spv = fmin(fmax(lam, laml[j]), lamu[j]);
+            }
+            v[3] += fabs(spv);
+            v[8] = fmax(v[8], fabs(spv));
+            const double r = spv - lam;
+            v[4] += r * r;
+            v[9] = fmax(v[9], fabs(r));
+        }
+        bar();
+        seg_products(sk, nseg, cl, val64, xv);
+        BFOR(i, m) {
+            const double axr = seg_row(sp, i) * Dl[i];
+            const double yr = (double)yv[i] / Dl[i];
+            v[2] += h_raw[i] * yr;
+            v[5] += fabs(axr);
+            v[10] = fmax(v[10], fabs(axr));
+            const double r = axr - h_raw[i];
+            if (i < cb) {
+                const double p = i < mzero ? 0.0 : fmax(r, 0.0);
+                const double pr = p - r;
+                v[6] += pr * pr;
+                v[11] = fmax(v[11], fabs(pr));
+                v[12] = fmax(v[12], fabs(p));
+            } else {
+                rtmp[i] = r;
+            }
+        }
+        bar();
+        // plain SOC (code 5) and Kexp / Kexp* (codes 13 / 16) of the residual rows
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for (int q = wave; q < nsoc; q += kBlock / 64) {
+            const double *w = rtmp + ss[q];
+            const int k = sd[q] - 1;
+            double nx = 0.0;
+            for (int e = lane; e < k; e += 64) nx += w[1 + e] * w[1 + e];
+            nx = sqrt(wave_sum(nx));
+            const double t = w[0];
+            double cf = 1.0, th = t;
+            if (nx <= -t) { cf = 0.0; th = 0.0; }
+            else if (nx > t) { cf = (1.0 + t / nx) / 2.0; th = nx * cf; }
+            for (int e = lane; e <= k; e += 64) {
+                const double p = e == 0 ? th : w[e] * cf;
+                const double pr = p - w[e];
+                v[6] += pr * pr;
+                v[11] = fmax(v[11], fabs(pr));
+                v[12] = fmax(v[12], fabs(p));
+            }
+        }
+        BFOR(b, nexp + ndexp) {
+            const double *w = rtmp + eb + 3 * b;
+            double p0 = w[0], p1 = w[1], p2 = w[2];
+            if (b < nexp) exp_project(p0, p1, p2);
+            else exp_project_dual(p0, p1, p2);
+            const double pp[3] = {p0, p1, p2};
+            for (int e = 0; e < 3; ++e) {
+                const double pr = pp[e] - w[e];
+                v[6] += pr * pr;
+                v[11] = fmax(v[11], fabs(pr));
+                v[12] = fmax(v[12], fabs(pp[e]));
+            }
+        }
+        reduce<13>(v, mx);
+        ci.pobj = v[0];
+        ci.dobj = v[2];
+        ci.abs_gap = fabs(v[0] - v[2]);
+        ci.rel_gap = ci.abs_gap / (1.0 + fmax(fabs(v[0]), fabs(v[2])));
+        ci.l2_abs_p = sqrt(v[6]);
+        ci.l2_rel_p = ci.l2_abs_p / (1.0 + fmax(v[3], fmax(hN1, v[5])));
+        ci.linf_abs_p = v[11];
+        ci.linf_rel_p = v[11] / (1.0 + fmax(hNinf, fmax(v[10], v[12])));
+        ci.linf_rel_p_code = v[11] / (1.0 + fmax(v[8], fmax(hNinf, v[10])));
+        ci.l2_abs_d = sqrt(v[4]);
+        ci.l2_rel_d = ci.l2_abs_d / (1.0 + fmax(cN1, v[1]));
+        ci.linf_abs_d = v[9];
+        ci.linf_rel_d = v[9] / (1.0 + fmax(cNinf, v[7]));
+    }
+    __device__ static double max_l2(const BConv &c) { return fmax(fmax(c.l2_rel_p, c.l2_rel_d), c.rel_gap); }
+    __device__ static double kkt_readme(const BConv &c) { return fmax(fmax(c.linf_rel_p, c.linf_rel_d), c.rel_gap); }
+    __device__ static double kkt_error(double om, const BConv &c) {
+        return sqrt(om * om * c.l2_abs_p * c.l2_abs_p + c.l2_abs_d * c.l2_abs_d / (om * om) + c.abs_gap * c.abs_gap);
+    }
+
+    // ---- restart search (Mnorm, approximate_cal_diagonal!, binary_search_duality_gap_diagonal!)
+    __device__ double Mnorm2(const T *xv, const T *yv, double ta, double sg) {
+        BV;
+        spmv_G(xv, tm2);
+        double v[3] = {0, 0, 0};
+        const bool mx[3] = {false, false, false};
+        BFOR(j, n) v[0] += (double)xv[j] * (double)xv[j];
+        BFOR(i, m) {
+            v[1] += (double)yv[i] * (double)tm2[i];
+            v[2] += (double)yv[i] * (double)yv[i];
+        }
+        reduce<3>(v, mx);
+        return v[0] / ta - 2 * v[1] + v[2] / sg;
+    }
+    __device__ double bs_eval(double t, const T *pr, const T *du, double ta, double sg) {
+        BV;
+        const double a = t * ta / 2, b = t * sg / 2;
+        BFOR(j, n) {
+            double w = (double)pr[j] + a * (double)h1[j];
+            if (j < nb) w = fmin(fmax(w, (double)bl[j]), (double)bu[j]);
+            tn1[j] = (T)w;
+        }
+        BFOR(i, m) tm1[i] = (T)((double)du[i] + b * (double)h2[i]);
+        bar();
+        project(tm1, nullptr, nullptr, true);
+        BFOR(j, n) tn2[j] = (T)((double)tn1[j] - (double)pr[j]);
+        BFOR(i, m) tm1[i] = (T)((double)tm1[i] - (double)du[i]);
+        bar();
+        return Mnorm2(tn2, tm1, ta, sg);
+    }
+    __device__ double binary_search(double r, const T *pr, const T *du, double ta, double sg) {
+        BV;
+        double t = bs_t0, tR = t, tL = 0.0;
+        spmv_Gt(du, h1);
+        BFOR(j, n) h1[j] = (T)((double)h1[j] - (double)c[j]);
+        spmv_G(pr, h2);
+        BFOR(i, m) h2[i] = (T)((double)h[i] - (double)h2[i]);
+        bar();
+        for (int k = 1; k <= 1000; ++k) {
+            double v = bs_eval(t, pr, du, ta, sg);
+            if (v > r && v < r + 1e-6) {
+                tR = t;
+                tL = fmax(t - 0.01, 0.0);
+                v = bs_eval(tL, pr, du, ta, sg);
+                if (v < r) break;
+            }
+            if (v < r && v > r - 1e-6) {
+                tL = t;
+                tR = t + 0.01;
+                v = bs_eval(tR, pr, du, ta, sg);
+                if (v > r) break;
+            }
+            if (v > r) {
+                if (k == 1) break;
+                if (P->bisect_intent) { tR = t; tL = t / 2; }
+                break;
+            }
+            t *= 2;
+        }
+        double tM = (tR + tL) / 2;
+        int64_t cnt = 0;
+        while (tR - tL > 1e-6) {
+            tM = (tR + tL) / 2;
+            const double v = bs_eval(tM, pr, du, ta, sg);
+            if (v < r) tL = tM; else tR = tM;
+            if (++cnt > 1000000) break;
+        }
+        bs_t0 = tM;
+        double v[1] = {0.0};
+        const bool mx[1] = {false};
+        BFOR(j, n) v[0] += (double)h1[j] * (double)tn2[j];
+        BFOR(i, m) v[0] += (double)h2[i] * (double)tm1[i];
+        reduce<1>(v, mx);
+        return v[0] / r;
+    }
+    __device__ double nrm2(const T *a, int len) { return sqrt(dot(a, a, len)); }
+    __device__ double init_primal_weight() {
+        BV;
+        const double hn = nrm2(h, m), cn = nrm2(c, n);
+        return (hn > 1e-10 && cn > 1e-10) ? cn / hn : 1.0;
+    }
+    __device__ double primal_weight_update(double om) {
+        BV;
+        const double xn = nrm2(xd, n), yn = nrm2(yd, m);
+        const int64_t ru = restart_used;
+        if (om > 10000 || om < 1.0 / 10000)
+            if (ru % 20 == 0 && ru != 0) return init_primal_weight();
+        if (om > 30000 || om < 1.0 / 30000)
+            if (ru % 10 == 0 && ru != 0) return init_primal_weight();
+        if (xn > 1e-10 && yn > 1e-10) return exp(0.5 * log(yn / xn) + 0.5 * log(om));
+        return om;
+    }
+    __device__ void copy_x(T *dst, const T *src) {
+        const int n = vw->n;
+        BFOR(j, n) dst[j] = src[j];
+    }
+    __device__ void copy_y(T *dst, const T *src) {
+        const int m = vw->m;
+        BFOR(i, m) dst[i] = src[i];
+    }
+
+    __device__ void restart_threshold(bool dg) {
+        BV;
+        if (dg) {
+            BFOR(j, n) rx[j] = (T)((double)x0[j] - (double)x0l[j]);
+            BFOR(i, m) ry[i] = (T)((double)y0[i] - (double)y0l[i]);
+            bar();
+            const double r = Mnorm2(rx, ry, tau, sigma);
+            dg_threshold = binary_search(r, x0, y0, tau, sigma);
+            if (dg_threshold < 0) restart_dg_flag = false;
+        } else {
+            conv(xs, ys, sh->ci[0]);
+            kkt_threshold = kkt_error(omega, sh->ci[0]);
+        }
+    }
+    __device__ bool restart_check(bool dg) {
+        BV;
+        bool res = false;
+        if (dg) {
+            BFOR(j, n) rx[j] = (T)((double)xs[j] - (double)x0[j]);
+            BFOR(i, m) ry[i] = (T)((double)ys[i] - (double)y0[i]);
+            bar();
+            const double r = Mnorm2(rx, ry, tau, sigma);
+            const double rho = binary_search(r, xs, ys, tau, sigma);
+            const double rho_m = binary_search(r, xm, ym, tau, sigma);
+            for (int cond = 0; cond < 2 && !res; ++cond) {
+                const double beta = cond == 0 ? 0.4 : 0.8;
+                if (rho < beta * dg_threshold || rho_m < beta * dg_threshold) {
+                    restart_used++;
+                    if (rho < rho_m && restart_dg_flag) {
+                        copy_x(xm, xs);
+                        copy_y(ym, ys);
+                        bar();
+                    }
+                    res = true;
+                }
+            }
+        } else {
+            conv(xs, ys, sh->ci[0]);
+            conv(xm, ym, sh->ci[1]);
+            const double lag0 = kkt_err0, lag1 = kkt_err1;
+            const double e0 = kkt_error(omega, sh->ci[0]), e1 = kkt_error(omega, sh->ci[1]);
+            kkt_err0 = e0;
+            kkt_err1 = e1;
+            const bool c1 = e0 < 0.4 * kkt_threshold || e1 < 0.4 * kkt_threshold;
+            const bool c2 = (e0 < 0.8 * kkt_threshold && e0 > lag0) || (e1 < 0.8 * kkt_threshold && e1 > lag1);
+            if (c1 || c2) {
+                restart_used++;
+                if (e0 < e1) {
+                    copy_x(xm, xs);
+                    copy_y(ym, ys);
+                    bar();
+                }
+                res = true;
+            }
+        }
+        if (!res && (double)inner >= (double)iter * 0.223 && iter > 100000) { restart_used++; res = true; }
+        return res;
+    }
+
+    // exit_condition_check_diagonal! + check_termination_criteria on (x, y) and the average
+    __device__ bool exit_check(uint64_t tstart) {
+        BV;
+        conv(x, y, sh->ci[0]);
+        conv(xm, ym, sh->ci[1]);
+        for (int q = 0; q < 2; ++q) {
+            const BConv &ci = sh->ci[q];
+            const double rp = P->code_primal_denominator ? ci.linf_rel_p_code : ci.linf_rel_p;
+            const bool opt = fmax(rp, fmax(ci.linf_rel_d, ci.rel_gap)) < P->rel_tol ||
+                             fmax(ci.linf_abs_p, fmax(ci.linf_abs_d, ci.abs_gap)) < P->abs_tol;
+            if (isnan(ci.rel_gap)) { exit_code = 8; exit_seq = q; return true; }
+            if (isinf(ci.rel_gap)) { exit_code = 9; exit_seq = q; return true; }
+            if (opt) { exit_code = 0; exit_seq = q; return true; }
+        }
+        if (iter >= P->max_total) { exit_code = 1; exit_seq = 0; return true; }
+        if (P->time_limit_ticks > 0 && wall_clock64() - tstart > (uint64_t)P->time_limit_ticks) {
+            exit_code = 6;
+            exit_seq = 0;
+            return true;
+        }
+        return false;
+    }
+
+    // one attempt (pdhg_one_iter_diagonal_rescaling_adaptive_step_size!, scaling.jl:656-767):
+    // returns dx2, dy2, <dx, G' dy> (broadcast)
+    __device__ void attempt(double &dx2, double &dy2, double &dt) {
+        BV;
+        const double ps = eta / omega * dec, ds = eta * omega * dec;
+        seg_products(tsk, ntseg, tcl, tval, ys);
+        double a = 0.0;
+        BFOR(j, n) {
+            const double acc = seg_row(tsp, j);
+            const double xl = (double)xs[j];
+            double w = xl + ps * (acc - (double)c[j]);
+            if (j < nb) w = fmin(fmax(w, (double)bl[j]), (double)bu[j]);
+            const T xv = (T)w;
+            x[j] = xv;
+            xbar[j] = (T)(2.0 * (double)xv - xl);
+            const double d = (double)xv - xl;
+            a += d * d;
+        }
+        bar();
+        seg_products(sk, nseg, cl, val, xbar);
+        double b = 0.0;
+        BFOR(i, m) {
+            const double acc = seg_row(sp, i);
+            const double yl = (double)ys[i];
+            double w = yl - ds * (acc - (double)h[i]);
+            if (i < cb) {
+                if (i >= mzero) w = fmax(w, 0.0);
+                const T yv = (T)w;
+                y[i] = yv;
+                const double d = (double)yv - yl;
+                dy[i] = (T)d;
+                b += d * d;
+            } else {
+                y[i] = (T)w;
+            }
+        }
+        bar();
+        b += project(y, ys, dy, false);
+        bar();
+        seg_products(tsk, ntseg, tcl, tval, dy);
+        double cc = 0.0;
+        BFOR(j, n) cc += ((double)x[j] - (double)xs[j]) * seg_row(tsp, j);
+        double v[3] = {a, b, cc};
+        const bool mx[3] = {false, false, false};
+        reduce<3>(v, mx);
+        dx2 = v[0];
+        dy2 = v[1];
+        dt = v[2];
+    }
+    // reflection_update (utils.cu:38-57)
+    __device__ void reflect(double k, double e, double et, double cum) {
+        BV;
+        const double tmp = (k + 1.0) / (k + 2.0), tail = 1.0 / (k + 2.0);
+        BFOR(j, n) {
+            const double xv = (double)x[j], xl = (double)xs[j];
+            const T nx = (T)(tmp * ((1.0 + e) * xv - e * xl) + tail * xv);
+            xs[j] = nx;
+            xm[j] = (T)(((double)xm[j] * cum + (double)nx * et) / (cum + et));
+        }
+        BFOR(i, m) {
+            const double yv = (double)y[i], yl = (double)ys[i];
+            const T ny = (T)(tmp * ((1.0 + e) * yv - e * yl) + tail * yv);
+            ys[i] = ny;
+            ym[i] = (T)(((double)ym[i] * cum + (double)ny * et) / (cum + et));
+        }
+        bar();
+    }
+
+    // the aggressive main loop (scaling.jl:770-896); returns when an exit is decided
+    __device__ void run(uint64_t tstart) {
+        BV;
+        exit_code = 7;
+        bool done = false;
+        for (int64_t outer = 1; outer <= P->max_outer && !done; ++outer) {
+            max_kkt = max_l2(sh->ci[0]);
+            min_kkt = fmin(fmin(fmin(sh->ci[0].l2_rel_p, sh->ci[0].l2_rel_d), sh->ci[0].rel_gap), min_kkt);
+            extra = -0.1 * log10(max_kkt) + 0.2;
+            copy_x(x0l, x0);
+            copy_y(y0l, y0);
+            bar();
+            copy_x(x0, xm);
+            copy_y(y0, ym);
+            copy_x(xs, xm);
+            copy_y(ys, ym);
+            bar();
+            if (P->use_kkt_restart) restart_threshold(false);
+            else restart_threshold(restart_dg_flag);
+            copy_x(xd, xs);
+            copy_y(yd, ys);
+            bar();
+            inner = 0;
+            eta_cum = 0.0;
+            dec_next = fmin(0.9 + 0.1 * log10(max_kkt * 1e5), 1.0);
+            dec = dec_next;
+            for (;;) {
+                double dx2, dy2, dt;
+                attempt(dx2, dy2, dt);
+                iter += 1;
+                iter_stepsize += 1;
+                const double eta_try = eta;
+                tau = eta / omega;
+                sigma = eta * omega;
+                const double eta_bar = (omega * dx2 + dy2 / omega) / (2.0 * fabs(dt) + 1e-20);
+                const double k1 = (double)(iter_stepsize + 1);
+                const double eta_prime = fmin((1.0 - pow(k1, -0.3)) * eta_bar, (1.0 + pow(k1, -0.6)) * eta);
+                const bool accept = eta <= eta_bar;
+                if (!accept) count += 1;
+                const bool force = !accept && count > 1000000;
+                if (iter % P->check_freq == 0) {
+                    if (exit_check(tstart)) { done = true; break; }
+                    max_kkt = max_l2(sh->ci[0]);
+                    dec_next = fmin(0.9 + 0.1 * log10(max_kkt * 1e5), 1.0);
+                    if (P->stop_tol > 0) {
+                        const double kk = fmin(kkt_readme(sh->ci[0]), kkt_readme(sh->ci[1]));
+                        const bool stalled = n_checks >= 2 && kk > 0.9 * best_kkt;
+                        best_kkt = fmin(best_kkt, kk);
+                        n_checks++;
+                        if (kk < P->stop_tol || stalled) { exit_code = 10; done = true; break; }
+                    }
+                    if (iter >= P->max_total) { exit_code = 1; done = true; break; }
+                }
+                if (accept || force) {
+                    const double eta_acc = fmax(force && !accept ? eta_prime : eta_try, 1e-30);
+                    reflect((double)(inner + 1), extra, eta_acc, eta_cum);
+                    eta_cum += eta_acc;
+                    inner += 1;
+                    eta = fmax(eta_prime, 1e-6);
+                    count = 0;
+                    dec = dec_next;
+                    const bool art = ((double)inner >= (double)iter * 0.223 && iter > 10000) || (force && inner > 100);
+                    if (art) { restart_used++; break; }
+                    bool brk = false;
+                    if (P->use_kkt_restart && P->kkt_restart_freq > 0 && iter % P->kkt_restart_freq == 0) {
+                        if (restart_check(false)) brk = true;
+                        else if (outer == 1) brk = true;
+                    }
+                    if (!brk && P->use_dg_restart && P->restart_freq > 0 && iter % P->restart_freq == 0) {
+                        if (restart_check(restart_dg_flag)) brk = true;
+                        else if (outer == 1) brk = true;
+                    }
+                    if (brk) break;
+                } else {
+                    eta = eta_prime;
+                }
+            }
+            if (done) break;
+            BFOR(j, n) xd[j] = (T)((double)xd[j] - (double)xs[j]);
+            BFOR(i, m) yd[i] = (T)((double)yd[i] - (double)ys[i]);
+            bar();
+            omega = primal_weight_update(omega);
+        }
+        if (!done) exit_code = 1;
+    }
+};
+
+// Persistent solve: each workgroup takes instances from the queue until it is
+// empty (every workgroup reaches the exit).  polish_phase: only instances the
+// FP32 phase handed over (exit code 10) are run, continuing its state.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_batch_solve(BatchData B, const BView<T> *__restrict__ views,
+                                                        const BatchParams *__restrict__ Pp) {
+    __shared__ BShared sh;
+    __shared__ int qslot;
+    const BatchParams &P = *Pp;
+    for (;;) {
+        if (threadIdx.x == 0) qslot = (int)atomicAdd(B.next, 1u);
+        __syncthreads();
+        const int slot = __builtin_amdgcn_readfirstlane(qslot);
+        __syncthreads();
+        if (slot >= B.count) return;
+        const int q = __builtin_amdgcn_readfirstlane(B.order[slot]);
+        BatchOut *O = B.out + q;
+        if (P.polish_phase && O->exit_code != 10) continue;
+        const BView<T> &view = views[q];
+        BSolve<T> S;
+        S.vw = &view;
+        S.sh = &sh;
+        S.P = Pp;
+        const uint64_t t0 = wall_clock64();
+        S.restart_dg_flag = true;
+        S.kkt_err0 = S.kkt_err1 = 0.0;
+        S.best_kkt = INFINITY;
+        S.n_checks = 0;
+        S.count = 0;
+        S.min_kkt = 1e30;
+        S.dg_threshold = S.kkt_threshold = 0.0;
+        S.exit_seq = 0;
+        if (P.polish_phase) {
+            // continue from the FP32 phase: fresh epoch from its iterate, same scalars
+            S.omega = O->omega;
+            S.eta = fmax(O->eta, 1e-6);
+            S.tau = O->tau;
+            S.sigma = O->sigma;
+            S.bs_t0 = O->bs_t0;
+            S.restart_used = O->restarts;
+            S.iter = S.iter_stepsize = O->iter;
+            S.max_kkt = 1e30;
+            S.conv(view.x, view.y, sh.ci[0]);   // the hand-over check's residuals (current iterate)
+        } else {
+            double g = 0.0;
+            const int nnz = B.inst[q].nnz;
+            for (int k = threadIdx.x; k < nnz; k += kBlock) g = fmax(g, fabs(view.val64[k]));
+            double v[1] = {g};
+            const bool mx[1] = {true};
+            block_reduce_multi<1>(v, mx, sh.red);
+            S.eta = fmax(1.0 / v[0], 1e-6);
+            S.tau = S.sigma = 0.9;
+            S.bs_t0 = 1.0;
+            S.restart_used = 0;
+            S.iter = S.iter_stepsize = 0;
+            S.max_kkt = 1e30;
+            S.conv(view.xs, view.ys, sh.ci[0]);
+            S.omega = S.init_primal_weight();
+        }
+        S.run(t0);
+        const uint64_t t1 = wall_clock64();
+        if (threadIdx.x == 0) {
+            const BConv &c = sh.ci[S.exit_seq];
+            O->exit_code = S.exit_code;
+            O->sequence = S.exit_seq;
+            if (!P.polish_phase) O->iter_fp32 = sizeof(T) == 4 ? S.iter : 0;
+            O->iter = S.iter;
+            O->restarts = S.restart_used;
+            O->pobj = c.pobj;
+            O->dobj = c.dobj;
+            O->rel_p = c.linf_rel_p;
+            O->rel_d = c.linf_rel_d;
+            O->rel_gap = c.rel_gap;
+            O->abs_p = c.linf_abs_p;
+            O->abs_d = c.linf_abs_d;
+            O->abs_gap = c.abs_gap;
+            O->l2_rel_p = c.l2_rel_p;
+            O->l2_rel_d = c.l2_rel_d;
+            O->rel_p_code = c.linf_rel_p_code;
+            O->omega = S.omega;
+            O->eta = S.eta;
+            O->tau = S.tau;
+            O->sigma = S.sigma;
+            O->bs_t0 = S.bs_t0;
+            O->ci_l2max = BSolve<T>::max_l2(sh.ci[0]);
+            if (P.polish_phase) {
+                O->t2 = t1 - t0;
+                O->polished = 1;
+            } else {
+                O->t0 = t0;
+                O->t1 = t1 - t0;
+                O->t2 = 0;
+                O->polished = 0;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// FP32 -> FP64 iterates of the instances handed over to the polish; the polish
+// epoch starts from the sequence that was best at the hand-over
+__global__ __launch_bounds__(kBlock) void k_batch_promote(BatchData B, BatchVecs<float> A, BatchVecs<double> D) {
+    const int q = blockIdx.x;
+    if (B.out[q].exit_code != 10) return;
+    const BatchInst I = B.inst[q];
+    const bool cur = B.out[q].sequence == 0;
+    BFOR(j, I.n) {
+        const int64_t g = I.xo + j;
+        D.x[g] = A.x[g];
+        D.xs[g] = A.xs[g];
+        D.xm[g] = cur ? (double)A.x[g] : (double)A.xm[g];
+        D.x0[g] = A.x0[g];
+        D.x0l[g] = A.x0l[g];
+    }
+    BFOR(i, I.m) {
+        const int64_t g = I.yo + i;
+        D.y[g] = A.y[g];
+        D.ys[g] = A.ys[g];
+        D.ym[g] = cur ? (double)A.y[g] : (double)A.ym[g];
+        D.y0[g] = A.y0[g];
+        D.y0l[g] = A.y0l[g];
+    }
+}
+
+// unscaled solution of the returned sequence (recover_solution!, postprocess.jl:10)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_batch_recover(BatchData B, BatchVecs<T> V, double *xo, double *yo,
+                                                          int polished_only) {
+    const int q = blockIdx.x;
+    const BatchOut &O = B.out[q];
+    if (polished_only >= 0 && O.polished != polished_only) return;
+    const BatchInst I = B.inst[q];
+    const bool avg = O.sequence == 1;
+    BFOR(j, I.n) {
+        const int64_t g = I.xo + j;
+        xo[g] = (double)(avg ? V.xm[g] : V.x[g]) / B.Dr[g];
+    }
+    BFOR(i, I.m) {
+        const int64_t g = I.yo + i;
+        yo[g] = (double)(avg ? V.ym[g] : V.y[g]) / B.Dl[g];
+    }
+}
+
+// ------------------------------------------------------------ launchers --
+void launch_batch_scale(hipStream_t s, const BatchData &B) { k_batch_scale<<<B.count, kBlock, 0, s>>>(B); }
+
+template <typename T>
+void launch_batch_prepare(hipStream_t s, const BatchData &B, BatchVecs<T> V) {
+    k_batch_prepare<T><<<B.count, kBlock, 0, s>>>(B, V);
+}
+template void launch_batch_prepare<double>(hipStream_t, const BatchData &, BatchVecs<double>);
+template void launch_batch_prepare<float>(hipStream_t, const BatchData &, BatchVecs<float>);
+
+void launch_batch_f32(hipStream_t s, const double *a, float *b, int64_t n) {
+    int64_t g = (n + kBlock - 1) / kBlock;
+    if (g > 8192) g = 8192;
+    if (g > 0) k_batch_f32<<<(unsigned)g, kBlock, 0, s>>>(a, b, n);
+}
+
+template <typename T>
+void launch_batch_solve(hipStream_t s, const BatchData &B, const BView<T> *views, const BatchParams *p, int grid) {
+    k_batch_solve<T><<<grid, kBlock, 0, s>>>(B, views, p);
+}
+template void launch_batch_solve<double>(hipStream_t, const BatchData &, const BView<double> *, const BatchParams *, int);
+template void launch_batch_solve<float>(hipStream_t, const BatchData &, const BView<float> *, const BatchParams *, int);
+
+void launch_batch_promote(hipStream_t s, const BatchData &B, BatchVecs<float> A, BatchVecs<double> D) {
+    k_batch_promote<<<B.count, kBlock, 0, s>>>(B, A, D);
+}
+
+template <typename T>
+void launch_batch_recover(hipStream_t s, const BatchData &B, BatchVecs<T> V, double *x_out, double *y_out,
+                          int polished_only) {
+    k_batch_recover<T><<<B.count, kBlock, 0, s>>>(B, V, x_out, y_out, polished_only);
+}
+template void launch_batch_recover<double>(hipStream_t, const BatchData &, BatchVecs<double>, double *, double *, int);
+template void launch_batch_recover<float>(hipStream_t, const BatchData &, BatchVecs<float>, double *, double *, int);
+
+// workgroups of k_batch_solve that fit on the device at once
+int batch_max_grid(int threads) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_solve<float>, threads, 0) != hipSuccess) return 0;
+    int per64 = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per64, k_batch_solve<double>, threads, 0) != hipSuccess) return 0;
+    return cus * (per < per64 ? per : per64);
+}
+
+}  // namespace pdcs

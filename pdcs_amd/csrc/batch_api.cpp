@@ -1,0 +1,407 @@
+// batch_api.cpp -- C ABI of the batched solve (include/pdcs.h, pdcs_batch_*).
+//
+// The host builds, per instance, the transposed CSR, the segment lists of both
+// CSRs (rows cut into pieces of at most kSegNnz nonzeros), the cone offsets
+// and the norms of the original data, uploads everything into flat device
+// arrays and orders the work queue by decreasing nonzero count (longest
+// first).  Scaling, the solve and the recovery of the unscaled solution run
+// on the device (batch.hip).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "../../include/pdcs.h"
+#include "batch.h"
+#include "host_common.h"
+#include "internal.h"
+
+using namespace pdcs;
+using namespace pdcs_host;
+
+namespace pdcs {
+void launch_batch_f32(hipStream_t s, const double *a, float *b, int64_t n);
+}
+
+template <typename T>
+struct BatchVecStore {
+    BatchVecs<T> v{};
+    BView<T> *views = nullptr;   // per-instance views (device)
+    bool ready = false;
+};
+
+struct pdcs_batch {
+    pdcs_ctx *ctx = nullptr;
+    int64_t count = 0, sum_n = 0, sum_m = 0, sum_nnz = 0;
+    DevBuf mem;
+    std::vector<BatchInst> inst;
+    BatchData B{};
+    BatchVecStore<double> d64;
+    BatchVecStore<float> d32;
+    float *val32 = nullptr, *tval32 = nullptr;
+    double *xo = nullptr, *yo = nullptr;   // unscaled solution (device)
+    BatchParams *params = nullptr;         // device copy of the solve controls
+    bool solved = false;
+};
+
+namespace {
+
+// segments of a CSR: row r's nonzeros [rp[r], rp[r+1]) cut into pieces of at
+// most kSegNnz; segk holds the piece boundaries, segptr[r] the first piece of r
+void segments(const int32_t *rp, int64_t rows, std::vector<int32_t> &segk, std::vector<int32_t> &segptr) {
+    segk.clear();
+    segptr.assign(rows + 1, 0);
+    for (int64_t r = 0; r < rows; ++r) {
+        segptr[r] = (int32_t)segk.size();
+        for (int32_t k = rp[r]; k < rp[r + 1]; k += kSegNnz) segk.push_back(k);
+    }
+    segptr[rows] = (int32_t)segk.size();
+    segk.push_back(rows > 0 ? rp[rows] : 0);
+}
+
+// view of instance I in precision T (instance-local pointers into the flat arrays)
+template <typename T>
+BView<T> make_view(const BatchData &B, const BatchVecs<T> &V, const BatchInst &I) {
+    BView<T> S{};
+    S.n = I.n; S.m = I.m; S.nb = I.nb; S.mzero = I.mzero; S.cb = I.mzero + I.mnonneg;
+    S.nsoc = I.nsoc; S.nexp = I.nexp; S.ndexp = I.ndexp; S.nseg = I.nseg; S.ntseg = I.ntseg;
+    S.eb = I.m - 3 * (I.nexp + I.ndexp);
+    S.rp = B.rowptr + I.ro; S.cl = B.col + I.ko; S.sk = B.segk + I.sgo; S.sp = B.segptr + I.ro;
+    S.trp = B.trowptr + I.tro; S.tcl = B.tcol + I.ko; S.tsk = B.tsegk + I.tsgo; S.tsp = B.tsegptr + I.tro;
+    S.ss = B.soc_start + I.so; S.sd = B.soc_dim + I.so;
+    S.val = V.val + I.ko; S.tval = V.tval + I.ko; S.val64 = B.val64 + I.ko; S.tval64 = B.tval64 + I.ko;
+    S.dsc2 = B.dsc2 + I.yo; S.Dr = B.Dr + I.xo; S.Dl = B.Dl + I.yo; S.c_raw = B.c_raw + I.xo; S.h_raw = B.h_raw + I.yo;
+    S.blf = B.blf + I.xo; S.buf = B.buf + I.xo; S.laml = B.laml + I.xo; S.lamu = B.lamu + I.xo;
+    S.c = V.c + I.xo; S.h = V.h + I.yo; S.bl = V.bl + I.xo; S.bu = V.bu + I.xo;
+    S.x = V.x + I.xo; S.xs = V.xs + I.xo; S.xm = V.xm + I.xo; S.xbar = V.xbar + I.xo; S.x0 = V.x0 + I.xo;
+    S.x0l = V.x0l + I.xo; S.xd = V.xd + I.xo; S.rx = V.rx + I.xo; S.tn1 = V.tn1 + I.xo; S.tn2 = V.tn2 + I.xo;
+    S.h1 = V.h1 + I.xo;
+    S.y = V.y + I.yo; S.ys = V.ys + I.yo; S.ym = V.ym + I.yo; S.dy = V.dy + I.yo; S.y0 = V.y0 + I.yo;
+    S.y0l = V.y0l + I.yo; S.yd = V.yd + I.yo; S.ry = V.ry + I.yo; S.tm1 = V.tm1 + I.yo; S.tm2 = V.tm2 + I.yo;
+    S.h2 = V.h2 + I.yo;
+    S.part = B.part + I.po; S.rtmp = B.rtmp + I.yo;
+    S.hN1 = I.hN1; S.hNinf = I.hNinf; S.cN1 = I.cN1; S.cNinf = I.cNinf;
+    return S;
+}
+
+template <typename T>
+int build_views(pdcs_batch *b, BatchVecStore<T> &S) {
+    std::vector<BView<T>> v(b->count);
+    for (int64_t q = 0; q < b->count; ++q) v[q] = make_view<T>(b->B, S.v, b->inst[q]);
+    if (!S.views && !(S.views = b->mem.alloc<BView<T>>(b->count))) return fail(b->ctx, PDCS_ERR_HIP, "batch allocation failed");
+    HIPCHK(b->ctx, copy_sync(b->ctx, S.views, v.data(), sizeof(BView<T>) * b->count, hipMemcpyHostToDevice));
+    return PDCS_OK;
+}
+
+template <typename T>
+int alloc_vecs(pdcs_batch *b, BatchVecStore<T> &S) {
+    if (S.ready) return PDCS_OK;
+    auto &M = b->mem;
+    const int64_t n = b->sum_n, m = b->sum_m;
+    BatchVecs<T> &V = S.v;
+    T **xs[] = {&V.x, &V.xs, &V.xm, &V.xbar, &V.x0, &V.x0l, &V.xd, &V.rx, &V.tn1, &V.tn2, &V.h1, &V.c, &V.bl, &V.bu};
+    for (T **p : xs)
+        if (!(*p = M.alloc<T>(n))) return fail(b->ctx, PDCS_ERR_HIP, "batch allocation failed");
+    T **ys[] = {&V.y, &V.ys, &V.ym, &V.dy, &V.y0, &V.y0l, &V.yd, &V.ry, &V.tm1, &V.tm2, &V.h2, &V.h};
+    for (T **p : ys)
+        if (!(*p = M.alloc<T>(m))) return fail(b->ctx, PDCS_ERR_HIP, "batch allocation failed");
+    S.ready = true;
+    return PDCS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pdcs_batch_create_host(pdcs_ctx *ctx, int64_t count, const pdcs_batch_shape *shapes, const double *c,
+                           const int32_t *rowptr, const int32_t *colidx, const double *vals, const double *h,
+                           const double *bl, const double *bu, const int64_t *soc, pdcs_batch **out) {
+    if (!ctx || !shapes || !c || !rowptr || !colidx || !vals || !h || !out) return fail(ctx, PDCS_ERR_ARG, "NULL argument");
+    if (count <= 0 || count >= (int64_t)1 << 31) return fail(ctx, PDCS_ERR_ARG, "bad batch size %ld", (long)count);
+    *out = nullptr;
+    std::vector<BatchInst> inst(count);
+    int64_t xo = 0, yo = 0, ko = 0, ro = 0, tro = 0, so = 0, sgo = 0, tsgo = 0, po = 0, bo = 0;
+    std::vector<int32_t> h_trowptr, h_tcol, h_tmap, h_segk, h_segptr, h_tsegk, h_tsegptr, h_sst, h_sdim;
+    std::vector<double> h_tvals, blf, buf, laml, lamu, blr, bur;
+    std::vector<int32_t> sk, sp;
+    for (int64_t q = 0; q < count; ++q) {
+        const pdcs_batch_shape &S = shapes[q];
+        const int32_t *rp = rowptr + ro;
+        if (S.n <= 0 || S.m <= 0 || S.nb < 0 || S.nb > S.n || S.mzero < 0 || S.mnonneg < 0 || S.nsoc < 0 ||
+            S.nexp < 0 || S.ndualexp < 0 || S.n >= (int64_t)1 << 30 || S.m >= (int64_t)1 << 30)
+            return fail(ctx, PDCS_ERR_ARG, "instance %ld: bad sizes", (long)q);
+        if (S.nb > 0 && (!bl || !bu)) return fail(ctx, PDCS_ERR_ARG, "NULL bounds");
+        if (S.nsoc > 0 && !soc) return fail(ctx, PDCS_ERR_ARG, "NULL soc");
+        int64_t rows = S.mzero + S.mnonneg + 3 * (S.nexp + S.ndualexp);
+        std::vector<int32_t> starts;
+        int64_t off = S.mzero + S.mnonneg;
+        for (int64_t b = 0; b < S.nsoc; ++b) {
+            const int64_t d = soc[so + b];
+            if (d < 2) return fail(ctx, PDCS_ERR_ARG, "instance %ld: The size of the SOC must be greater than 1", (long)q);
+            starts.push_back((int32_t)off);
+            off += d;
+            rows += d;
+        }
+        if (rows != S.m) return fail(ctx, PDCS_ERR_ARG, "instance %ld: cone sizes do not add up to m", (long)q);
+        if (rp[0] != 0) return fail(ctx, PDCS_ERR_ARG, "instance %ld: rowptr must start at 0", (long)q);
+        for (int64_t i = 0; i < S.m; ++i)
+            if (rp[i + 1] < rp[i]) return fail(ctx, PDCS_ERR_ARG, "instance %ld: rowptr not monotone", (long)q);
+        const int64_t nnz = rp[S.m];
+        for (int64_t k = 0; k < nnz; ++k)
+            if (colidx[ko + k] < 0 || colidx[ko + k] >= S.n)
+                return fail(ctx, PDCS_ERR_ARG, "instance %ld: column index out of range", (long)q);
+        for (int64_t j = 0; j < S.nb; ++j)
+            if (!(bl[bo + j] <= bu[bo + j]))
+                return fail(ctx, PDCS_ERR_ARG, "instance %ld: Not all elements of bl are less than or equal to the corresponding elements in bu", (long)q);
+        // transpose
+        std::vector<int32_t> trp(S.n + 1, 0), tcol(nnz), tmap(nnz);
+        for (int64_t k = 0; k < nnz; ++k) trp[colidx[ko + k] + 1]++;
+        for (int64_t j = 0; j < S.n; ++j) trp[j + 1] += trp[j];
+        {
+            std::vector<int32_t> fill(trp.begin(), trp.end() - 1);
+            for (int64_t i = 0; i < S.m; ++i)
+                for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+                    const int32_t sl = fill[colidx[ko + k]]++;
+                    tcol[sl] = (int32_t)i;
+                    tmap[sl] = k;
+                }
+        }
+        h_trowptr.insert(h_trowptr.end(), trp.begin(), trp.end());
+        h_tcol.insert(h_tcol.end(), tcol.begin(), tcol.end());
+        h_tmap.insert(h_tmap.end(), tmap.begin(), tmap.end());
+        for (int64_t k = 0; k < nnz; ++k) h_tvals.push_back(vals[ko + tmap[k]]);
+        // segments
+        segments(rp, S.m, sk, sp);
+        const int32_t nseg = (int32_t)sk.size() - 1;
+        h_segk.insert(h_segk.end(), sk.begin(), sk.end());
+        h_segptr.insert(h_segptr.end(), sp.begin(), sp.end());
+        segments(trp.data(), S.n, sk, sp);
+        const int32_t ntseg = (int32_t)sk.size() - 1;
+        h_tsegk.insert(h_tsegk.end(), sk.begin(), sk.end());
+        h_tsegptr.insert(h_tsegptr.end(), sp.begin(), sp.end());
+        for (int64_t b = 0; b < S.nsoc; ++b) {
+            h_sst.push_back(starts[b]);
+            h_sdim.push_back((int32_t)soc[so + b]);
+        }
+        // bounds in the dual-residual form (replace_inf_with_zero / gen_lambd); past nb: free
+        for (int64_t j = 0; j < S.n; ++j) {
+            const bool box = j < S.nb;
+            const double l = box ? bl[bo + j] : -INFINITY, u = box ? bu[bo + j] : INFINITY;
+            blr.push_back(l);
+            bur.push_back(u);
+            blf.push_back(std::isinf(l) ? 0.0 : l);
+            buf.push_back(std::isinf(u) ? 0.0 : u);
+            laml.push_back(u == INFINITY ? 0.0 : -INFINITY);
+            lamu.push_back(l == -INFINITY ? 0.0 : INFINITY);
+        }
+        BatchInst &I = inst[q];
+        std::memset(&I, 0, sizeof(I));
+        I.n = (int32_t)S.n; I.m = (int32_t)S.m; I.nb = (int32_t)S.nb;
+        I.mzero = (int32_t)S.mzero; I.mnonneg = (int32_t)S.mnonneg; I.nsoc = (int32_t)S.nsoc;
+        I.nexp = (int32_t)S.nexp; I.ndexp = (int32_t)S.ndualexp;
+        I.nnz = (int32_t)nnz; I.nseg = nseg; I.ntseg = ntseg; I.npart = std::max(nseg, ntseg);
+        I.xo = xo; I.yo = yo; I.ko = ko; I.ro = ro; I.tro = tro; I.so = so; I.sgo = sgo; I.tsgo = tsgo; I.po = po;
+        for (int64_t i = 0; i < S.m; ++i) { I.hN1 += std::fabs(h[yo + i]); I.hNinf = std::max(I.hNinf, std::fabs(h[yo + i])); }
+        for (int64_t j = 0; j < S.n; ++j) { I.cN1 += std::fabs(c[xo + j]); I.cNinf = std::max(I.cNinf, std::fabs(c[xo + j])); }
+        xo += S.n; yo += S.m; ko += nnz; ro += S.m + 1; tro += S.n + 1; so += S.nsoc;
+        sgo += nseg + 1; tsgo += ntseg + 1; po += I.npart; bo += S.nb;
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    auto *b = new pdcs_batch();
+    b->ctx = ctx;
+    b->count = count;
+    b->sum_n = xo;
+    b->sum_m = yo;
+    b->sum_nnz = ko;
+    b->inst = inst;
+    auto &M = b->mem;
+    BatchData &B = b->B;
+    B.count = (int32_t)count;
+    auto up = [&](auto *dst, const auto *src, int64_t len) -> bool {
+        if (!dst) return false;
+        if (len > 0 && copy_sync(ctx, (void *)dst, (const void *)src, sizeof(*src) * (size_t)len, hipMemcpyHostToDevice) != hipSuccess)
+            return false;
+        return true;
+    };
+    // processing order: most nonzeros first
+    std::vector<int32_t> order(count);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t c2) { return inst[a].nnz > inst[c2].nnz; });
+    BatchInst *d_inst = M.alloc<BatchInst>(count);
+    int32_t *d_order = M.alloc<int32_t>(count);
+    B.next = M.alloc<unsigned int>(1);
+    int32_t *d_rowptr = M.alloc<int32_t>(ro), *d_col = M.alloc<int32_t>(ko), *d_segk = M.alloc<int32_t>(sgo);
+    int32_t *d_segptr = M.alloc<int32_t>(ro), *d_trowptr = M.alloc<int32_t>(tro), *d_tcol = M.alloc<int32_t>(ko);
+    int32_t *d_tsegk = M.alloc<int32_t>(tsgo), *d_tsegptr = M.alloc<int32_t>(tro), *d_tmap = M.alloc<int32_t>(ko);
+    int32_t *d_sst = M.alloc<int32_t>(so), *d_sdim = M.alloc<int32_t>(so);
+    B.val64 = M.alloc<double>(ko);
+    B.tval64 = M.alloc<double>(ko);
+    double *d_c = M.alloc<double>(xo), *d_h = M.alloc<double>(yo), *d_bl = M.alloc<double>(xo), *d_bu = M.alloc<double>(xo);
+    B.Dr = M.alloc<double>(xo); B.Dl = M.alloc<double>(yo); B.dsc2 = M.alloc<double>(yo); B.rtmp = M.alloc<double>(yo);
+    B.part = M.alloc<double>(po);
+    B.blf = M.alloc<double>(xo); B.buf = M.alloc<double>(xo); B.laml = M.alloc<double>(xo); B.lamu = M.alloc<double>(xo);
+    B.rowv = M.alloc<double>(yo); B.colv = M.alloc<double>(xo);
+    B.out = M.alloc<BatchOut>(count);
+    b->xo = M.alloc<double>(xo);
+    b->yo = M.alloc<double>(yo);
+    bool ok = up(d_inst, inst.data(), count) && up(d_order, order.data(), count) && up(d_rowptr, rowptr, ro) &&
+              up(d_col, colidx, ko) && up(d_segk, h_segk.data(), sgo) && up(d_segptr, h_segptr.data(), ro) &&
+              up(d_trowptr, h_trowptr.data(), tro) && up(d_tcol, h_tcol.data(), ko) && up(d_tsegk, h_tsegk.data(), tsgo) &&
+              up(d_tsegptr, h_tsegptr.data(), tro) && up(d_tmap, h_tmap.data(), ko) && up(B.val64, vals, ko) &&
+              up(B.tval64, h_tvals.data(), ko) && up(d_c, c, xo) && up(d_h, h, yo) && up(d_bl, blr.data(), xo) &&
+              up(d_bu, bur.data(), xo) && up(B.blf, blf.data(), xo) && up(B.buf, buf.data(), xo) &&
+              up(B.laml, laml.data(), xo) && up(B.lamu, lamu.data(), xo) && B.Dr && B.Dl && B.dsc2 && B.rtmp &&
+              B.part && B.rowv && B.colv && B.out && B.next && b->xo && b->yo && d_sst && d_sdim;
+    if (ok && so > 0) ok = up(d_sst, h_sst.data(), so) && up(d_sdim, h_sdim.data(), so);
+    if (!ok) {
+        delete b;
+        return fail(ctx, PDCS_ERR_HIP, "batch upload failed");
+    }
+    B.inst = d_inst; B.order = d_order;
+    B.rowptr = d_rowptr; B.col = d_col; B.segk = d_segk; B.segptr = d_segptr;
+    B.trowptr = d_trowptr; B.tcol = d_tcol; B.tsegk = d_tsegk; B.tsegptr = d_tsegptr; B.tmap = d_tmap;
+    B.soc_start = d_sst; B.soc_dim = d_sdim;
+    B.c_raw = d_c; B.h_raw = d_h; B.bl_raw = d_bl; B.bu_raw = d_bu;
+    launch_batch_scale(ctx->stream, B);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        delete b;
+        return fail(ctx, PDCS_ERR_HIP, "batch scaling failed");
+    }
+    *out = b;
+    return PDCS_OK;
+}
+
+int pdcs_batch_destroy(pdcs_batch *b) {
+    if (!b) return PDCS_OK;
+    hipSetDevice(b->ctx->device);
+    delete b;
+    return PDCS_OK;
+}
+
+int pdcs_batch_solve(pdcs_batch *b, const pdcs_params *prm_in, pdcs_batch_result *res, double *time_total) {
+    if (!b || !res) return fail(b ? b->ctx : nullptr, PDCS_ERR_ARG, "NULL argument");
+    pdcs_params prm;
+    if (prm_in) prm = *prm_in; else pdcs_params_default(&prm);
+    pdcs_ctx *ctx = b->ctx;
+    if (prm.check_terminate_freq <= 0) return fail(ctx, PDCS_ERR_ARG, "check_terminate_freq must be positive");
+    if (prm.use_preconditioner != 1) return fail(ctx, PDCS_ERR_UNSUPPORTED, "only the preconditioned path is built");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const bool mixed = prm.precision == PDCS_MIXED;
+    int rc;
+    if ((rc = alloc_vecs<double>(b, b->d64))) return rc;
+    b->d64.v.val = b->B.val64;
+    b->d64.v.tval = b->B.tval64;
+    if (!b->d64.views && (rc = build_views<double>(b, b->d64))) return rc;
+    if (!b->params && !(b->params = b->mem.alloc<BatchParams>(1))) return fail(ctx, PDCS_ERR_HIP, "batch allocation failed");
+    if (mixed) {
+        if ((rc = alloc_vecs<float>(b, b->d32))) return rc;
+        if (!b->val32) {
+            b->val32 = b->mem.alloc<float>(b->sum_nnz);
+            b->tval32 = b->mem.alloc<float>(b->sum_nnz);
+            if (!b->val32 || !b->tval32) return fail(ctx, PDCS_ERR_HIP, "batch allocation failed");
+            launch_batch_f32(s, b->B.val64, b->val32, b->sum_nnz);
+            launch_batch_f32(s, b->B.tval64, b->tval32, b->sum_nnz);
+        }
+        b->d32.v.val = b->val32;
+        b->d32.v.tval = b->tval32;
+        if (!b->d32.views && (rc = build_views<float>(b, b->d32))) return rc;
+    }
+    BatchParams P{};
+    P.rel_tol = prm.rel_tol;
+    P.abs_tol = prm.abs_tol;
+    P.restart_freq = prm.use_duality_gap_restart ? prm.duality_gap_restart_freq : 0;
+    P.kkt_restart_freq = prm.use_kkt_restart ? prm.kkt_restart_freq : 0;
+    P.max_outer = prm.max_outer_iter;
+    P.max_total = prm.max_outer_iter * prm.max_inner_iter;
+    P.use_kkt_restart = prm.use_kkt_restart;
+    P.use_dg_restart = prm.use_duality_gap_restart;
+    P.bisect_intent = prm.bisect_intent;
+    P.code_primal_denominator = prm.code_primal_denominator;
+    int dev = 0, khz = 0;
+    HIPCHK(ctx, hipGetDevice(&dev));
+    HIPCHK(ctx, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    P.time_limit_ticks = std::isfinite(prm.time_limit) ? (int64_t)(prm.time_limit * khz * 1e3) : 0;
+    const int cap = batch_max_grid(kBlock);
+    if (cap <= 0) return fail(ctx, PDCS_ERR_HIP, "occupancy query failed");
+    const int grid = (int)std::min<int64_t>(b->count, cap);
+    hipEvent_t e0, e1;
+    HIPCHK(ctx, hipEventCreate(&e0));
+    HIPCHK(ctx, hipEventCreate(&e1));
+    HIPCHK(ctx, hipMemsetAsync(b->B.out, 0, sizeof(BatchOut) * b->count, s));
+    launch_batch_prepare<double>(s, b->B, b->d64.v);
+    if (mixed) launch_batch_prepare<float>(s, b->B, b->d32.v);
+    HIPCHK(ctx, hipEventRecord(e0, s));
+    if (mixed) {
+        P.check_freq = prm.mixed_check_freq > 0 ? prm.mixed_check_freq : prm.check_terminate_freq;
+        P.stop_tol = prm.polish ? prm.mixed_switch_tol : 0.0;
+        P.polish_phase = 0;
+        HIPCHK(ctx, copy_sync(ctx, b->params, &P, sizeof(P), hipMemcpyHostToDevice));
+        HIPCHK(ctx, hipMemsetAsync(b->B.next, 0, sizeof(unsigned int), s));
+        launch_batch_solve<float>(s, b->B, b->d32.views, b->params, grid);
+        if (prm.polish) {
+            launch_batch_promote(s, b->B, b->d32.v, b->d64.v);
+            P.check_freq = prm.polish_check_freq > 0 ? prm.polish_check_freq : prm.check_terminate_freq;
+            P.stop_tol = 0.0;
+            P.polish_phase = 1;
+            HIPCHK(ctx, copy_sync(ctx, b->params, &P, sizeof(P), hipMemcpyHostToDevice));
+            HIPCHK(ctx, hipMemsetAsync(b->B.next, 0, sizeof(unsigned int), s));
+            launch_batch_solve<double>(s, b->B, b->d64.views, b->params, grid);
+        }
+    } else {
+        P.check_freq = prm.check_terminate_freq;
+        P.stop_tol = 0.0;
+        P.polish_phase = 0;
+        HIPCHK(ctx, copy_sync(ctx, b->params, &P, sizeof(P), hipMemcpyHostToDevice));
+        HIPCHK(ctx, hipMemsetAsync(b->B.next, 0, sizeof(unsigned int), s));
+        launch_batch_solve<double>(s, b->B, b->d64.views, b->params, grid);
+    }
+    HIPCHK(ctx, hipEventRecord(e1, s));
+    HIPCHK(ctx, hipGetLastError());
+    hipError_t se = hipEventSynchronize(e1);
+    if (se != hipSuccess) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return fail(ctx, PDCS_ERR_HIP, "batch solve failed: %s", hipGetErrorString(se));
+    }
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (time_total) *time_total = ms * 1e-3;
+    // unscaled solutions: FP32 results first, then the polished ones over them
+    if (mixed) launch_batch_recover<float>(s, b->B, b->d32.v, b->xo, b->yo, -1);
+    if (!mixed || prm.polish) launch_batch_recover<double>(s, b->B, b->d64.v, b->xo, b->yo, mixed ? 1 : -1);
+    std::vector<BatchOut> O(b->count);
+    HIPCHK(ctx, copy_sync(ctx, O.data(), b->B.out, sizeof(BatchOut) * b->count, hipMemcpyDeviceToHost));
+    for (int64_t q = 0; q < b->count; ++q) {
+        const BatchOut &o = O[q];
+        pdcs_batch_result &r = res[q];
+        r.exit_code = o.exit_code;
+        r.sequence = o.sequence;
+        r.iter = o.iter;
+        r.iter_fp32 = o.iter_fp32;
+        r.restarts = o.restarts;
+        r.pobj = o.pobj;
+        r.dobj = o.dobj;
+        r.rel_primal_res = o.rel_p;
+        r.rel_dual_res = o.rel_d;
+        r.rel_gap = o.rel_gap;
+        r.time = (double)(o.t1 + o.t2) / (khz * 1e3);
+    }
+    b->solved = true;
+    return PDCS_OK;
+}
+
+int pdcs_batch_get_solution(pdcs_batch *b, double *x, double *y) {
+    if (!b) return fail(nullptr, PDCS_ERR_ARG, "NULL batch");
+    if (!b->solved) return fail(b->ctx, PDCS_ERR_STATE, "no solve has run");
+    if (x) HIPCHK(b->ctx, copy_sync(b->ctx, x, b->xo, sizeof(double) * b->sum_n, hipMemcpyDeviceToHost));
+    if (y) HIPCHK(b->ctx, copy_sync(b->ctx, y, b->yo, sizeof(double) * b->sum_m, hipMemcpyDeviceToHost));
+    return PDCS_OK;
+}
+
+}  // extern "C"

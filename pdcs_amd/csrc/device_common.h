@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include "internal.h"
+
 namespace pdcs {
 
 // ------------------------------------------------------------ reductions --
@@ -57,6 +59,26 @@ __device__ __forceinline__ double block_sum_all(double v, double *lds) {
     for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += lds[i];
     __syncthreads();
     return r;
+}
+
+// Block-wide reduction of N values in one LDS round (kBlock threads): entries
+// with ismax[q] take the maximum, the others the sum; results broadcast to all.
+template <int N>
+__device__ __forceinline__ void block_reduce_multi(double (&v)[N], const bool (&ismax)[N], double *lds /* 4*N */) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) v[q] = ismax[q] ? wave_max(v[q]) : wave_sum(v[q]);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < N; ++q) lds[w * N + q] = v[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        double r = lds[q];
+        for (int i = 1; i < kBlock / 64; ++i) r = ismax[q] ? fmax(r, lds[i * N + q]) : r + lds[i * N + q];
+        v[q] = r;
+    }
+    __syncthreads();
 }
 
 // Scaled-SOC projection onto {(t, z) : ||d .* z|| <= t} of (s, x).

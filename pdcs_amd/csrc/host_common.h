@@ -1,0 +1,76 @@
+// host_common.h -- host-side helpers shared by the C ABI translation units
+// (pdcs_api.cpp, batch_api.cpp): the context, error reporting and device
+// allocation.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/pdcs.h"
+
+struct pdcs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+};
+
+namespace pdcs_host {
+
+inline thread_local std::string g_tls_err;
+
+inline int fail(pdcs_ctx *ctx, int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    g_tls_err = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail((ctx), PDCS_ERR_HIP, "%s failed: %s (%s:%d)", #call,                \
+                        hipGetErrorString(e_), __FILE__, __LINE__);                         \
+    } while (0)
+
+// Stream-ordered synchronous copy on the context's stream (the stream is
+// non-blocking, so plain hipMemcpy would not wait for its kernels).
+inline hipError_t copy_sync(pdcs_ctx *ctx, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, ctx->stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(ctx->stream);
+}
+
+inline double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct DevBuf {
+    std::vector<void *> ptrs;
+    template <typename T>
+    T *alloc(int64_t n) {
+        void *p = nullptr;
+        if (hipMalloc(&p, sizeof(T) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess) return nullptr;
+        if (hipMemset(p, 0, sizeof(T) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess ||
+            hipDeviceSynchronize() != hipSuccess) {
+            hipFree(p);
+            return nullptr;
+        }
+        ptrs.push_back(p);
+        return (T *)p;
+    }
+    ~DevBuf() {
+        for (void *p : ptrs) hipFree(p);
+    }
+};
+
+}  // namespace pdcs_host

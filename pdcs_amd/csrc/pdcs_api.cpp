@@ -20,50 +20,13 @@
 #include <vector>
 
 #include "../../include/pdcs.h"
+#include "host_common.h"
 #include "internal.h"
 
 using namespace pdcs;
-
-struct pdcs_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-};
+using namespace pdcs_host;
 
 namespace {
-
-thread_local std::string g_tls_err;
-
-int fail(pdcs_ctx *ctx, int code, const char *fmt, ...) {
-    char buf[1024];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    if (ctx) ctx->err = buf;
-    g_tls_err = buf;
-    return code;
-}
-
-#define HIPCHK(ctx, call)                                                                   \
-    do {                                                                                    \
-        hipError_t e_ = (call);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            return fail((ctx), PDCS_ERR_HIP, "%s failed: %s (%s:%d)", #call,                \
-                        hipGetErrorString(e_), __FILE__, __LINE__);                         \
-    } while (0)
-
-// Stream-ordered synchronous copy on the context's stream (the stream is
-// non-blocking, so plain hipMemcpy would not wait for its kernels).
-hipError_t copy_sync(pdcs_ctx *ctx, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
-    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, ctx->stream);
-    if (e != hipSuccess) return e;
-    return hipStreamSynchronize(ctx->stream);
-}
-
-double now_s() {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
 
 int lanes_for(int64_t nnz, int64_t rows) {
     const double avg = rows > 0 ? (double)nnz / (double)rows : 1.0;
@@ -71,25 +34,6 @@ int lanes_for(int64_t nnz, int64_t rows) {
     while (L < 64 && L < avg) L <<= 1;
     return L;
 }
-
-struct DevBuf {
-    std::vector<void *> ptrs;
-    template <typename T>
-    T *alloc(int64_t n) {
-        void *p = nullptr;
-        if (hipMalloc(&p, sizeof(T) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess) return nullptr;
-        if (hipMemset(p, 0, sizeof(T) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess ||
-            hipDeviceSynchronize() != hipSuccess) {
-            hipFree(p);
-            return nullptr;
-        }
-        ptrs.push_back(p);
-        return (T *)p;
-    }
-    ~DevBuf() {
-        for (void *p : ptrs) hipFree(p);
-    }
-};
 
 // iterate vectors of one precision
 template <typename T>

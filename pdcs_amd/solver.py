@@ -165,6 +165,65 @@ class Problem:
             pass
 
 
+class Batch:
+    """Many independent conic programs solved together in one persistent launch
+    (pdcs_batch_*): one workgroup per instance, same algorithm and parameters
+    as :class:`Problem`."""
+
+    def __init__(self, problems, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.problems = list(problems)
+        k = len(self.problems)
+        shapes = (_lib.BatchShape * k)()
+        for i, P in enumerate(self.problems):
+            shapes[i] = _lib.BatchShape(P.n, P.m, P.nb, P.mzero, P.mnonneg, len(P.soc), P.nexp, P.ndexp)
+        cat = lambda key, dt: np.ascontiguousarray(np.concatenate([np.asarray(getattr(P, key)) for P in self.problems]), dtype=dt)
+        self._arrs = [cat("c", np.float64), cat("rowptr", np.int32), cat("colidx", np.int32), cat("vals", np.float64),
+                      cat("h", np.float64),
+                      np.ascontiguousarray(np.concatenate([P.bl[:P.nb] for P in self.problems]), dtype=np.float64),
+                      np.ascontiguousarray(np.concatenate([P.bu[:P.nb] for P in self.problems]), dtype=np.float64),
+                      np.ascontiguousarray(np.concatenate([np.asarray(P.soc, dtype=np.int64).reshape(-1)
+                                                           for P in self.problems] + [np.zeros(1, np.int64)]),
+                                           dtype=np.int64)]
+        a = self._arrs
+        self._shapes = shapes
+        self.h = C.c_void_p()
+        _lib.check(self.ctx.L.pdcs_batch_create_host(self.ctx.h, k, shapes, _p(a[0]), _p(a[1], C.c_int32),
+                                                     _p(a[2], C.c_int32), _p(a[3]), _p(a[4]), _p(a[5]), _p(a[6]),
+                                                     _p(a[7], C.c_int64), C.byref(self.h)), self.ctx.h)
+        self._xo = np.cumsum([0] + [P.n for P in self.problems])
+        self._yo = np.cumsum([0] + [P.m for P in self.problems])
+        _live_problems.add(self)
+
+    def solve(self, params: _lib.Params | None = None, **kw):
+        """Returns (list of per-instance result dicts, total seconds of the launches)."""
+        prm = params or _lib.default_params()
+        for k, v in kw.items():
+            setattr(prm, k, v)
+        res = (_lib.BatchResult * len(self.problems))()
+        tt = C.c_double(0.0)
+        _lib.check(self.ctx.L.pdcs_batch_solve(self.h, C.byref(prm), res, C.byref(tt)), self.ctx.h)
+        return [r.as_dict() for r in res], tt.value
+
+    def solutions(self):
+        """Unscaled (x, y) of every instance after a solve."""
+        x = np.zeros(self._xo[-1])
+        y = np.zeros(self._yo[-1])
+        _lib.check(self.ctx.L.pdcs_batch_get_solution(self.h, _p(x), _p(y)), self.ctx.h)
+        return [(x[self._xo[i]:self._xo[i + 1]], y[self._yo[i]:self._yo[i + 1]]) for i in range(len(self.problems))]
+
+    def close(self):
+        if self.h and self.ctx.h:
+            self.ctx.L.pdcs_batch_destroy(self.h)
+        self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _solution_struct(x, y, s, res):
     info = SimpleNamespace(exit_code=res["exit_code"], exit_status=EXIT_STATUS.get(res["exit_code"], "unknown"),
                            pObj=res["pobj"], dObj=res["dobj"], time=res["time_solve"], iter=res["iter"],

@@ -1,0 +1,71 @@
+"""GPU parity of the batched solve (pdcs_batch_*): every instance of a batch of
+uneven problems against the CPU FP64 oracle solving it alone."""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from pdcs_amd import instances
+from pdcs_amd.solver import Batch
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed_batch():
+    """Portfolios of the workload-3 recipe (zero + SOC rows, box) plus SOC
+    programs and a small logistic regression (nonnegative + exponential rows)."""
+    probs = instances.workload3(count=16, seed=77, nmin=20, nmax=300)
+    probs.append(instances.soc_program(100, 200, 0.05, 10, seed=5))
+    probs.append(instances.soc_program(60, 150, 0.1, 5, seed=6))
+    probs.append(instances.workload1(samples=150, features=20, density=0.2, lam=1e-3))
+    w3 = instances.workload3(count=9)
+    probs += [w3[6], w3[8]]          # n = 437 and 404: a restart and 4000 attempts each
+    return probs
+
+
+@pytest.fixture(scope="module")
+def batch_and_refs(ctx):
+    probs = _mixed_batch()
+    refs = [oracle.solve(P) for P in probs]
+    B = Batch(probs, ctx)
+    yield probs, refs, B
+    B.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_matches_oracle(batch_and_refs, precision):
+    probs, refs, B = batch_and_refs
+    res, total = B.solve(precision=precision, time_limit=120.0)
+    sols = B.solutions()
+    assert total > 0
+    for P, (r1, _, _), r2, (x, y) in zip(probs, refs, res, sols):
+        assert r2["exit_code"] == 0, (P.name, r2)
+        assert max(r2["rel_primal_res"], r2["rel_dual_res"], r2["rel_gap"]) < 1e-6, (P.name, r2)
+        for k in ("pobj", "dobj"):
+            assert abs(r2[k] - r1[k]) <= 1e-6 * (1 + abs(r1[k])), (P.name, k, r2[k], r1[k])
+        chk = oracle.kkt(P, x, y)   # independent re-evaluation of the returned solution
+        assert max(chk["rel_primal_res"], chk["rel_dual_res"], chk["rel_gap"]) < 1e-6, (P.name, chk)
+        assert r2["time"] > 0
+        if precision == 1:
+            assert r2["iter_fp32"] > 0
+
+
+def test_batch_fp64_follows_oracle_iterations(batch_and_refs):
+    """Same algorithm per instance: the FP64 batch stops at the oracle's check
+    (iteration counts equal on most instances; reduction order may shift a few
+    by one check interval)."""
+    probs, refs, B = batch_and_refs
+    res, _ = B.solve(precision=0, time_limit=120.0)
+    same = sum(int(r2["iter"] == r1["iter"]) for (r1, _, _), r2 in zip(refs, res))
+    assert same >= len(probs) - 2, [(r1["iter"], r2["iter"]) for (r1, _, _), r2 in zip(refs, res)]
+
+
+def test_batch_solutions_repeatable(batch_and_refs):
+    probs, refs, B = batch_and_refs
+    r1, _ = B.solve(precision=0, time_limit=120.0)
+    s1 = B.solutions()
+    r2, _ = B.solve(precision=0, time_limit=120.0)
+    s2 = B.solutions()
+    for a, b in zip(r1, r2):
+        assert a["iter"] == b["iter"] and a["pobj"] == b["pobj"]
+    for (x1, y1), (x2, y2) in zip(s1, s2):
+        assert np.array_equal(x1, x2) and np.array_equal(y1, y2)
