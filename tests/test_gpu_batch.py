@@ -31,17 +31,27 @@ def batch_and_refs(ctx):
     B.close()
 
 
+@pytest.fixture(scope="module")
+def optima(batch_and_refs):
+    """Reference objectives: the oracle solved to 1e-9 (a 1e-6 solution on each
+    side could differ from another by up to ~2e-6)."""
+    probs, _, _ = batch_and_refs
+    return [oracle.solve(P, rel_tol=1e-9, abs_tol=1e-12)[0] for P in probs]
+
+
 @pytest.mark.parametrize("precision", [0, 1])
-def test_batch_matches_oracle(batch_and_refs, precision):
-    probs, refs, B = batch_and_refs
+def test_batch_matches_oracle(batch_and_refs, optima, precision):
+    probs, _, B = batch_and_refs
     res, total = B.solve(precision=precision, time_limit=120.0)
     sols = B.solutions()
     assert total > 0
-    for P, (r1, _, _), r2, (x, y) in zip(probs, refs, res, sols):
+    for P, r1, r2, (x, y) in zip(probs, optima, res, sols):
         assert r2["exit_code"] == 0, (P.name, r2)
         assert max(r2["rel_primal_res"], r2["rel_dual_res"], r2["rel_gap"]) < 1e-6, (P.name, r2)
+        # a solution meeting the README tolerance 1e-6 on gap and residuals can sit
+        # up to ~2e-6 (1 + |obj|) from the optimum (gap + residual contributions)
         for k in ("pobj", "dobj"):
-            assert abs(r2[k] - r1[k]) <= 1e-6 * (1 + abs(r1[k])), (P.name, k, r2[k], r1[k])
+            assert abs(r2[k] - r1[k]) <= 2e-6 * (1 + abs(r1[k])), (P.name, k, r2[k], r1[k])
         chk = oracle.kkt(P, x, y)   # independent re-evaluation of the returned solution
         assert max(chk["rel_primal_res"], chk["rel_dual_res"], chk["rel_gap"]) < 1e-6, (P.name, chk)
         assert r2["time"] > 0
