@@ -212,6 +212,22 @@ int pdcs_batch_solve(pdcs_batch *b, const pdcs_params *params, pdcs_batch_result
 /* unscaled solutions of the last solve, flat: x sum n_i, y sum m_i */
 int pdcs_batch_get_solution(pdcs_batch *b, double *x, double *y);
 
+/* ---- reverse-mode derivatives through the solution map ----
+ * No counterpart in the reference (it has no differentiation).  Given the
+ * gradient of a loss L with respect to the last solve's unscaled solution,
+ * dL_dx (n) and optionally dL_dy (m, NULL = 0), returns dL_dc (n), dL_dh (m)
+ * and dL_dG (nnz, on G's CSR pattern) by implicit differentiation of the KKT
+ * fixed point x = P_B(x - c + G'y), y = P_K*(y - G x + h) (dense FP64 solve of
+ * the transposed Jacobian).  Zero, nonnegative and SOC rows and the box;
+ * exponential cones return PDCS_ERR_UNSUPPORTED; a singular Jacobian at the
+ * solution returns PDCS_ERR_STATE.  Host pointers. */
+int pdcs_adjoint(pdcs_problem *p, const double *dL_dx, const double *dL_dy, double *dL_dc, double *dL_dh,
+                 double *dL_dG);
+/* The same for every instance of a batch, flat arrays as pdcs_batch_create_host
+ * (dL_dG: sum of nnz_i); status (count, may be NULL): 0 ok, 1 singular. */
+int pdcs_batch_adjoint(pdcs_batch *b, const double *dL_dx, const double *dL_dy, double *dL_dc, double *dL_dh,
+                       double *dL_dG, int32_t *status);
+
 #ifdef __cplusplus
 }
 #endif

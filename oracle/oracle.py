@@ -135,6 +135,23 @@ def kkt(P, x, y):
     return res.as_dict()
 
 
+def tangent(P, x, y, dc=None, dh=None, dG=None, iters=200000, tol=1e-15):
+    """Dual-number forward mode (oracle_tangent): returns (x, y, dx, dy, iterations)
+    with (x, y) the refined solution and (dx, dy) its derivative along (dc, dh, dG)."""
+    H = _Held(P)
+    x = np.array(x, dtype=np.float64)
+    y = np.array(y, dtype=np.float64)
+    dx = np.zeros(P.n)
+    dy = np.zeros(P.m)
+    keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (dc, dh, dG)]
+    done = c_i64(0)
+    rc = lib().oracle_tangent(C.byref(H.s), _p(x), _p(y), *[None if a is None else _p(a) for a in keep],
+                              c_i64(iters), C.c_double(tol), _p(dx), _p(dy), C.byref(done))
+    if rc != 0:
+        raise RuntimeError(f"oracle_tangent failed: {rc}")
+    return x, y, dx, dy, done.value
+
+
 def spmv(rowptr, colidx, vals, x):
     m = len(rowptr) - 1
     y = np.zeros(m)

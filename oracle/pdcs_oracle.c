@@ -1086,3 +1086,120 @@ double oracle_time_attempts(const oracle_problem *P, int64_t attempts, int64_t *
     if (done) *done = res.iter;
     return res.time_solve;
 }
+
+
+/* ---------------------------------------------------- forward mode (dual) --
+ * Directional derivative of the solution map by dual numbers (value, tangent):
+ * the PDHG map with constant steps on the original data,
+ *     x+ = P_B(x - tau (c - G'y)),   y+ = P_K*(y - sigma (G (2 x+ - x) - h)),
+ * evaluated in dual arithmetic with data c + e dc, h + e dh, G + e dG.  Its
+ * fixed points are the KKT points, so started at a solution with zero
+ * tangents the tangent iterates converge to the derivative of the solution
+ * (x*, y*) along the data direction e.
+ * The same iteration with zero data tangents refines a solution.  Cones:
+ * zero (dual free), nonnegative, SOC (self-dual; projection soc_proj!,
+ * soc_rsoc_proj.jl:4-18, on dual numbers).  Steps tau = sigma = 0.95 / ||G||_2
+ * (power iteration). */
+typedef struct { double v, d; } dnum;
+static dnum dn(double v, double d) { dnum r = {v, d}; return r; }
+static dnum dadd(dnum a, dnum b) { return dn(a.v + b.v, a.d + b.d); }
+static dnum dsub(dnum a, dnum b) { return dn(a.v - b.v, a.d - b.d); }
+static dnum dmul(dnum a, dnum b) { return dn(a.v * b.v, a.d * b.v + a.v * b.d); }
+static dnum dscal(double s, dnum a) { return dn(s * a.v, s * a.d); }
+static dnum ddiv(dnum a, dnum b) { return dn(a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)); }
+static dnum dsqrt(dnum a) { double r = sqrt(a.v); return dn(r, r > 0 ? a.d / (2.0 * r) : 0.0); }
+
+static void dproj_soc(dnum *v, int64_t len) {
+    dnum s = dn(0, 0);
+    for (int64_t i = 1; i < len; ++i) s = dadd(s, dmul(v[i], v[i]));
+    const dnum nx = dsqrt(s), t = v[0];
+    if (nx.v <= -t.v) {
+        for (int64_t i = 0; i < len; ++i) v[i] = dn(0, 0);
+    } else if (nx.v <= t.v) {
+        return;
+    } else {
+        const dnum c = dscal(0.5, dadd(dn(1, 0), ddiv(t, nx)));
+        v[0] = dmul(nx, c);
+        for (int64_t i = 1; i < len; ++i) v[i] = dmul(v[i], c);
+    }
+}
+
+static double op_norm2(const oracle_problem *P) {
+    const int64_t n = P->n, m = P->m;
+    double *u = dalloc(n), *w = dalloc(m), lam = 0.0;
+    for (int64_t j = 0; j < n; ++j) u[j] = 1.0 / sqrt((double)n) * (1.0 + 0.01 * (double)(j % 7));
+    for (int it = 0; it < 200; ++it) {
+        oracle_spmv(m, P->rowptr, P->colidx, P->vals, u, w);
+        for (int64_t j = 0; j < n; ++j) u[j] = 0.0;
+        for (int64_t i = 0; i < m; ++i)
+            for (int32_t k = P->rowptr[i]; k < P->rowptr[i + 1]; ++k) u[P->colidx[k]] += P->vals[k] * w[i];
+        double nu = 0.0;
+        for (int64_t j = 0; j < n; ++j) nu += u[j] * u[j];
+        nu = sqrt(nu);
+        lam = sqrt(nu);
+        for (int64_t j = 0; j < n; ++j) u[j] /= nu;
+    }
+    free(u);
+    free(w);
+    return lam;
+}
+
+int oracle_tangent(const oracle_problem *P, double *x, double *y, const double *dc, const double *dh,
+                   const double *dG, int64_t iters, double tol, double *dx, double *dy, int64_t *done) {
+    if (P->nexp + P->ndexp > 0) return -2;
+    const int64_t n = P->n, m = P->m;
+    const double step = 0.95 / op_norm2(P);
+    dnum *X = (dnum *)calloc((size_t)n, sizeof(dnum)), *Y = (dnum *)calloc((size_t)m, sizeof(dnum));
+    dnum *Xn = (dnum *)calloc((size_t)n, sizeof(dnum)), *Yn = (dnum *)calloc((size_t)m, sizeof(dnum));
+    dnum *g = (dnum *)calloc((size_t)n, sizeof(dnum)), *bx = (dnum *)calloc((size_t)n, sizeof(dnum));
+    for (int64_t j = 0; j < n; ++j) X[j] = dn(x[j], 0);
+    for (int64_t i = 0; i < m; ++i) Y[i] = dn(y[i], 0);
+    int64_t it = 0;
+    for (; it < iters; ++it) {
+        /* G'y */
+        for (int64_t j = 0; j < n; ++j) g[j] = dn(0, 0);
+        for (int64_t i = 0; i < m; ++i)
+            for (int32_t k = P->rowptr[i]; k < P->rowptr[i + 1]; ++k) {
+                const dnum a = dn(P->vals[k], dG ? dG[k] : 0.0);
+                g[P->colidx[k]] = dadd(g[P->colidx[k]], dmul(a, Y[i]));
+            }
+        double chg = 0.0, scl = 0.0;
+        for (int64_t j = 0; j < n; ++j) {
+            const dnum cj = dn(P->c[j], dc ? dc[j] : 0.0);
+            dnum v = dsub(X[j], dscal(step, dsub(cj, g[j])));
+            if (j < P->nb) {
+                if (v.v <= P->bl[j]) v = dn(P->bl[j], 0.0);
+                else if (v.v >= P->bu[j]) v = dn(P->bu[j], 0.0);
+            }
+            Xn[j] = v;
+            bx[j] = dsub(dscal(2.0, v), X[j]);
+        }
+        for (int64_t i = 0; i < m; ++i) {
+            dnum gx = dn(0, 0);
+            for (int32_t k = P->rowptr[i]; k < P->rowptr[i + 1]; ++k)
+                gx = dadd(gx, dmul(dn(P->vals[k], dG ? dG[k] : 0.0), bx[P->colidx[k]]));
+            const dnum hi = dn(P->h[i], dh ? dh[i] : 0.0);
+            dnum v = dsub(Y[i], dscal(step, dsub(gx, hi)));
+            if (i >= P->mzero && i < P->mzero + P->mnonneg && v.v < 0.0) v = dn(0.0, 0.0);
+            Yn[i] = v;
+        }
+        int64_t off = P->mzero + P->mnonneg;
+        for (int64_t b = 0; b < P->nsoc; ++b) { dproj_soc(Yn + off, P->soc[b]); off += P->soc[b]; }
+        for (int64_t j = 0; j < n; ++j) {
+            chg = dmax(chg, fabs(Xn[j].d - X[j].d) + fabs(Xn[j].v - X[j].v));
+            scl = dmax(scl, fabs(Xn[j].d) + fabs(Xn[j].v));
+            X[j] = Xn[j];
+        }
+        for (int64_t i = 0; i < m; ++i) {
+            chg = dmax(chg, fabs(Yn[i].d - Y[i].d) + fabs(Yn[i].v - Y[i].v));
+            scl = dmax(scl, fabs(Yn[i].d) + fabs(Yn[i].v));
+            Y[i] = Yn[i];
+        }
+        if (chg <= tol * (1.0 + scl)) { ++it; break; }
+    }
+    for (int64_t j = 0; j < n; ++j) { x[j] = X[j].v; if (dx) dx[j] = X[j].d; }
+    for (int64_t i = 0; i < m; ++i) { y[i] = Y[i].v; if (dy) dy[i] = Y[i].d; }
+    if (done) *done = it;
+    free(X); free(Y); free(Xn); free(Yn); free(g); free(bx);
+    return 0;
+}

@@ -109,6 +109,14 @@ void oracle_proj_dual_exp(double *v);
 void oracle_scale(const oracle_problem *prob, int ruiz_iters, int pock, double *Dr,
                   double *Dl, double *vals_scaled);
 
+/* Forward mode by dual numbers: constant-step PDHG on the original data started
+ * at (x, y) with data tangents (dc, dh, dG; NULL = 0) for at most `iters`
+ * iterations (stops when the change of values and tangents is below tol);
+ * x, y are updated in place (a refinement of the solution) and dx, dy receive
+ * the tangents.  Zero, nonnegative and SOC rows only (returns -2 otherwise). */
+int oracle_tangent(const oracle_problem *P, double *x, double *y, const double *dc, const double *dh,
+                   const double *dG, int64_t iters, double tol, double *dx, double *dy, int64_t *done);
+
 /* README residuals of (x, y) in the original space */
 void oracle_kkt(const oracle_problem *prob, const double *x, const double *y,
                 oracle_result *res);

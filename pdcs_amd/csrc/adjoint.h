@@ -1,0 +1,29 @@
+// adjoint.h -- per-instance description of a reverse-mode derivative
+// (adjoint.hip): problem data, the solution, the loss gradient and the
+// device workspace of the dense Jacobian solve.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pdcs {
+
+struct AdjInst {
+    int32_t n, m, nb, mzero, mnonneg, nsoc;
+    const int32_t *rowptr, *col, *trowptr, *tcol, *soc_start, *soc_dim;   // instance-local
+    const double *val, *tval;      // scaled values Gs (G = Dl Gs Dr) of G and G'
+    const double *Dl, *Dr;
+    const double *c, *h, *bl, *bu; // original data (bl/bu: nb entries)
+    const double *x, *y;           // unscaled solution
+    const double *gx, *gy;         // dL/dx (n), dL/dy (m, may be null)
+    double *J;                     // (n + m)^2 workspace: J'
+    double *w;                     // n + m: right-hand side, then J'^-1 g
+    double *vx, *vy, *dB, *dK;     // n, m, n, m workspace
+    double *soc_t, *soc_r;         // nsoc workspace
+    int32_t *soc_cs;
+    double *dc, *dh, *dG;          // outputs: n, m, nnz (CSR order of G)
+    int32_t *status;               // 0 ok, 1 singular Jacobian
+};
+
+void launch_adjoint(hipStream_t s, const AdjInst *insts, int count);
+
+}  // namespace pdcs

@@ -1,0 +1,267 @@
+// adjoint.hip -- reverse-mode derivatives of the solution map of a conic program.
+//
+// At a solution z = (x, y) of  min c'x  s.t.  G x - h in K,  bl <= x[:nb] <= bu,
+// the KKT conditions are the fixed point F(z) = 0 of the projection map
+//     F_x = x - P_B(x - c + G'y),     F_y = y - P_K*(y - G x + h)
+// (stationarity with the box; feasibility, dual feasibility and
+// complementarity through Moreau's decomposition).  With D_B and D_K the
+// Jacobians of P_B and P_K* at those points (0/1 on box and nonnegative rows,
+// 1 on zero-cone rows, the closed-form SOC projection Jacobian on each SOC)
+//     J = dF/dz = [[I - D_B, -D_B G'], [D_K G, I - D_K]],
+// and for a loss with gradient g = (dL/dx, dL/dy) the implicit function
+// theorem gives dL/dtheta = -w' dF/dtheta with J' w = g:
+//     dL/dc = -D_B w_x,   dL/dh = D_K w_y,
+//     dL/dG_ij = (D_B w_x)_j y_i - (D_K w_y)_i x_j   (on the sparsity pattern).
+// J' is assembled densely per instance and solved by Gaussian elimination with
+// partial pivoting, one workgroup per instance; everything is FP64.
+#include "adjoint.h"
+#include "device_common.h"
+#include "internal.h"
+
+#include <math.h>
+
+namespace pdcs {
+
+constexpr int kAdjThreads = 1024;
+
+// Jacobian of the SOC projection at v = (t, z), as a function D(a, b) of the
+// block-local row a and column b (D is symmetric).
+struct SocJac {
+    double t, r;   // head, ||z||
+    int cs;        // 0 interior (I), 1 polar (0), 2 boundary
+    __device__ double at(const double *z, int a, int b) const {
+        if (cs == 0) return a == b ? 1.0 : 0.0;
+        if (cs == 1) return 0.0;
+        if (a == 0 && b == 0) return 0.5;
+        if (a == 0) return 0.5 * z[b - 1] / r;
+        if (b == 0) return 0.5 * z[a - 1] / r;
+        const double za = z[a - 1] / r, zb = z[b - 1] / r;
+        return 0.5 * ((a == b ? 1.0 + t / r : 0.0) - (t / r) * za * zb);
+    }
+};
+
+// v_x = x - c + G'y and v_y = y - G x + h (stored in vx, vy), then J' into A
+// (row-major N x N, N = n + m) and the right-hand side g into w.
+__global__ __launch_bounds__(kAdjThreads) void k_adj_build(const AdjInst *insts) {
+    const AdjInst &I = insts[blockIdx.x];
+    const int n = I.n, m = I.m, N = n + m;
+    double *A = I.J;
+    for (int64_t e = threadIdx.x; e < (int64_t)N * N; e += kAdjThreads) A[e] = 0.0;
+    // G x and G'y with the original values G_ij = Dl_i * Gs_ij * Dr_j
+    for (int j = threadIdx.x; j < n; j += kAdjThreads) I.vx[j] = I.x[j] - I.c[j];
+    for (int i = threadIdx.x; i < m; i += kAdjThreads) {
+        double gx = 0.0;
+        for (int32_t k = I.rowptr[i]; k < I.rowptr[i + 1]; ++k) gx += I.Dl[i] * I.val[k] * I.Dr[I.col[k]] * I.x[I.col[k]];
+        I.vy[i] = I.y[i] - gx + I.h[i];
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < n; j += kAdjThreads) {
+        double gy = 0.0;
+        for (int32_t k = I.trowptr[j]; k < I.trowptr[j + 1]; ++k) {
+            const int i = I.tcol[k];
+            gy += I.Dl[i] * I.tval[k] * I.Dr[j] * I.y[i];
+        }
+        I.vx[j] += gy;
+    }
+    __syncthreads();
+    // D_B (box rows j < nb: 1 strictly inside the box, else 0; free variables: 1)
+    for (int j = threadIdx.x; j < n; j += kAdjThreads) {
+        double d = 1.0;
+        if (j < I.nb) d = (I.vx[j] > I.bl[j] && I.vx[j] < I.bu[j]) ? 1.0 : 0.0;
+        I.dB[j] = d;
+    }
+    // D_K on the elementwise rows (zero cone: dual free -> 1; nonnegative: 1 if v > 0)
+    const int cb = I.mzero + I.mnonneg;
+    for (int i = threadIdx.x; i < cb; i += kAdjThreads) I.dK[i] = i < I.mzero ? 1.0 : (I.vy[i] > 0.0 ? 1.0 : 0.0);
+    __syncthreads();
+    // x rows of J (F_x): J[j][j] = 1 - dB_j, J[j][n + i] = -dB_j G_ij  ->  A = J': A[j][j], A[n + i][j]
+    for (int j = threadIdx.x; j < n; j += kAdjThreads) {
+        A[(int64_t)j * N + j] += 1.0 - I.dB[j];
+        for (int32_t k = I.trowptr[j]; k < I.trowptr[j + 1]; ++k) {
+            const int i = I.tcol[k];
+            A[(int64_t)(n + i) * N + j] += -I.dB[j] * (I.Dl[i] * I.tval[k] * I.Dr[j]);
+        }
+    }
+    // elementwise y rows: J[n + i][j] = dK_i G_ij, J[n + i][n + i] = 1 - dK_i
+    for (int i = threadIdx.x; i < cb; i += kAdjThreads) {
+        const double d = I.dK[i];
+        A[(int64_t)(n + i) * N + (n + i)] += 1.0 - d;
+        if (d != 0.0)
+            for (int32_t k = I.rowptr[i]; k < I.rowptr[i + 1]; ++k) {
+                const int j = I.col[k];
+                A[(int64_t)j * N + (n + i)] += d * (I.Dl[i] * I.val[k] * I.Dr[j]);
+            }
+    }
+    __syncthreads();
+    // SOC blocks: J[n + a][j] = sum_r D[a][r] G_rj, J[n + a][n + b] = delta - D[a][b]
+    for (int q = 0; q < I.nsoc; ++q) {
+        const int s0 = I.soc_start[q], dim = I.soc_dim[q];
+        const double *v = I.vy + s0;
+        __shared__ double red[kAdjThreads / 64];
+        double zz = 0.0;
+        for (int e = 1 + threadIdx.x; e < dim; e += kAdjThreads) zz += v[e] * v[e];
+        zz = block_sum_all(zz, red);
+        SocJac D;
+        D.t = v[0];
+        D.r = sqrt(zz);
+        D.cs = D.r <= D.t ? 0 : (D.r <= -D.t ? 1 : 2);
+        const double *z = v + 1;
+        // identity minus D on the block
+        for (int e = threadIdx.x; e < dim * dim; e += kAdjThreads) {
+            const int a = e / dim, b = e % dim;
+            A[(int64_t)(n + s0 + b) * N + (n + s0 + a)] += (a == b ? 1.0 : 0.0) - D.at(z, a, b);
+        }
+        __syncthreads();
+        // D G on the block: row a of the block is owned by one thread, which adds
+        // D[a][r] G_rj over the block's rows r in fixed order (column n + s0 + a of J')
+        for (int a = threadIdx.x; a < dim; a += kAdjThreads) {
+            for (int rr = 0; rr < dim; ++rr) {
+                const double dar = D.at(z, a, rr);
+                if (dar == 0.0) continue;
+                const int r = s0 + rr;
+                for (int32_t k = I.rowptr[r]; k < I.rowptr[r + 1]; ++k) {
+                    const int j = I.col[k];
+                    A[(int64_t)j * N + (n + s0 + a)] += dar * (I.Dl[r] * I.val[k] * I.Dr[j]);
+                }
+            }
+        }
+        if (threadIdx.x == 0) {
+            I.soc_t[q] = D.t;
+            I.soc_r[q] = D.r;
+            I.soc_cs[q] = D.cs;
+        }
+        __syncthreads();
+    }
+    // right-hand side
+    for (int j = threadIdx.x; j < n; j += kAdjThreads) I.w[j] = I.gx[j];
+    for (int i = threadIdx.x; i < m; i += kAdjThreads) I.w[n + i] = I.gy ? I.gy[i] : 0.0;
+}
+
+// Gaussian elimination with partial pivoting on [A | w], then back substitution;
+// w <- A^{-1} w.  status: 0 ok, 1 zero pivot.
+__global__ __launch_bounds__(kAdjThreads) void k_adj_solve(const AdjInst *insts) {
+    const AdjInst &I = insts[blockIdx.x];
+    const int N = I.n + I.m;
+    double *A = I.J, *w = I.w;
+    __shared__ double pv[kAdjThreads / 64];
+    __shared__ int pi[kAdjThreads / 64];
+    __shared__ int piv;
+    __shared__ double scale;
+    double amax = 0.0;
+    for (int64_t e = threadIdx.x; e < (int64_t)N * N; e += kAdjThreads) amax = fmax(amax, fabs(A[e]));
+    amax = block_max(amax, pv);
+    if (threadIdx.x == 0) scale = amax;
+    __syncthreads();
+    const double tiny = 1e-14 * scale;
+    for (int k = 0; k < N; ++k) {
+        // pivot: largest |A[i][k]|, i >= k (ties: smallest i)
+        double best = -1.0;
+        int bi = k;
+        for (int i = k + threadIdx.x; i < N; i += kAdjThreads) {
+            const double a = fabs(A[(int64_t)i * N + k]);
+            if (a > best) { best = a; bi = i; }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const double ob = __shfl_xor(best, o, 64);
+            const int oi = __shfl_xor(bi, o, 64);
+            if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+        }
+        if ((threadIdx.x & 63) == 0) { pv[threadIdx.x >> 6] = best; pi[threadIdx.x >> 6] = bi; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double b = pv[0];
+            int p = pi[0];
+            for (int q = 1; q < kAdjThreads / 64; ++q)
+                if (pv[q] > b || (pv[q] == b && pi[q] < p)) { b = pv[q]; p = pi[q]; }
+            piv = (b <= tiny) ? -1 : p;
+        }
+        __syncthreads();
+        const int p = piv;
+        if (p < 0) {
+            if (threadIdx.x == 0) *I.status = 1;
+            return;
+        }
+        if (p != k) {
+            for (int j = k + threadIdx.x; j < N; j += kAdjThreads) {
+                const double t = A[(int64_t)k * N + j];
+                A[(int64_t)k * N + j] = A[(int64_t)p * N + j];
+                A[(int64_t)p * N + j] = t;
+            }
+            if (threadIdx.x == 0) {
+                const double t = w[k];
+                w[k] = w[p];
+                w[p] = t;
+            }
+        }
+        __syncthreads();
+        const double akk = A[(int64_t)k * N + k], wk = w[k];
+        // multipliers in column k, right-hand side
+        for (int i = k + 1 + threadIdx.x; i < N; i += kAdjThreads) {
+            const double l = A[(int64_t)i * N + k] / akk;
+            A[(int64_t)i * N + k] = l;
+            w[i] -= l * wk;
+        }
+        __syncthreads();
+        // trailing update: rows over wavefronts, columns over lanes (coalesced)
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for (int i = k + 1 + wave; i < N; i += kAdjThreads / 64) {
+            const double l = A[(int64_t)i * N + k];
+            if (l == 0.0) continue;
+            double *Ai = A + (int64_t)i * N;
+            const double *Ak = A + (int64_t)k * N;
+            for (int j = k + 1 + lane; j < N; j += 64) Ai[j] -= l * Ak[j];
+        }
+        __syncthreads();
+    }
+    // back substitution, column oriented
+    for (int k = N - 1; k >= 0; --k) {
+        __shared__ double wk;
+        if (threadIdx.x == 0) {
+            wk = w[k] / A[(int64_t)k * N + k];
+            w[k] = wk;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < k; i += kAdjThreads) w[i] -= A[(int64_t)i * N + k] * wk;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *I.status = 0;
+}
+
+// dL/dc = -D_B w_x, dL/dh = D_K w_y, dL/dG_ij = (D_B w_x)_j y_i - (D_K w_y)_i x_j
+__global__ __launch_bounds__(kAdjThreads) void k_adj_grad(const AdjInst *insts) {
+    const AdjInst &I = insts[blockIdx.x];
+    if (*I.status != 0) return;
+    const int n = I.n, m = I.m;
+    const double *wx = I.w, *wy = I.w + n;
+    for (int j = threadIdx.x; j < n; j += kAdjThreads) I.dc[j] = -I.dB[j] * wx[j];
+    const int cb = I.mzero + I.mnonneg;
+    for (int i = threadIdx.x; i < cb; i += kAdjThreads) I.dh[i] = I.dK[i] * wy[i];
+    for (int q = 0; q < I.nsoc; ++q) {
+        const int s0 = I.soc_start[q], dim = I.soc_dim[q];
+        SocJac D;
+        D.t = I.soc_t[q];
+        D.r = I.soc_r[q];
+        D.cs = I.soc_cs[q];
+        const double *z = I.vy + s0 + 1;
+        for (int a = threadIdx.x; a < dim; a += kAdjThreads) {
+            double s = 0.0;
+            for (int b = 0; b < dim; ++b) s += D.at(z, a, b) * wy[s0 + b];
+            I.dh[s0 + a] = s;
+        }
+    }
+    __syncthreads();
+    // dh now holds D_K w_y on every row
+    for (int i = threadIdx.x; i < m; i += kAdjThreads)
+        for (int32_t k = I.rowptr[i]; k < I.rowptr[i + 1]; ++k) {
+            const int j = I.col[k];
+            I.dG[k] = I.dB[j] * wx[j] * I.y[i] - I.dh[i] * I.x[j];
+        }
+}
+
+void launch_adjoint(hipStream_t s, const AdjInst *insts, int count) {
+    k_adj_build<<<count, kAdjThreads, 0, s>>>(insts);
+    k_adj_solve<<<count, kAdjThreads, 0, s>>>(insts);
+    k_adj_grad<<<count, kAdjThreads, 0, s>>>(insts);
+}
+
+}  // namespace pdcs

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/pdcs.h"
+#include "adjoint.h"
 #include "batch.h"
 #include "host_common.h"
 #include "internal.h"
@@ -393,6 +394,66 @@ int pdcs_batch_solve(pdcs_batch *b, const pdcs_params *prm_in, pdcs_batch_result
         r.time = (double)(o.t1 + o.t2) / (khz * 1e3);
     }
     b->solved = true;
+    return PDCS_OK;
+}
+
+int pdcs_batch_adjoint(pdcs_batch *b, const double *dL_dx, const double *dL_dy, double *dL_dc, double *dL_dh,
+                       double *dL_dG, int32_t *status) {
+    if (!b || !dL_dx || !dL_dc || !dL_dh || !dL_dG) return fail(b ? b->ctx : nullptr, PDCS_ERR_ARG, "NULL argument");
+    pdcs_ctx *ctx = b->ctx;
+    if (!b->solved) return fail(ctx, PDCS_ERR_STATE, "no solve has run");
+    for (const BatchInst &I : b->inst)
+        if (I.nexp + I.ndexp > 0) return fail(ctx, PDCS_ERR_UNSUPPORTED, "the adjoint supports zero, nonnegative and SOC rows");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int64_t k = b->count, sn = b->sum_n, sm = b->sum_m;
+    int64_t jw = 0, sw = 0;
+    for (const BatchInst &I : b->inst) {
+        jw += (int64_t)(I.n + I.m) * (I.n + I.m);
+        sw += I.nsoc;
+    }
+    // one workspace for the whole batch (freed at return)
+    DevBuf W;
+    double *J = W.alloc<double>(jw), *w = W.alloc<double>(sn + sm), *vx = W.alloc<double>(sn), *vy = W.alloc<double>(sm);
+    double *dB = W.alloc<double>(sn), *dK = W.alloc<double>(sm), *st = W.alloc<double>(sw), *sr = W.alloc<double>(sw);
+    int32_t *scs = W.alloc<int32_t>(sw), *stat = W.alloc<int32_t>(k);
+    double *gx = W.alloc<double>(sn), *gy = W.alloc<double>(sm), *dc = W.alloc<double>(sn), *dh = W.alloc<double>(sm);
+    double *dG = W.alloc<double>(b->sum_nnz);
+    AdjInst *dI = W.alloc<AdjInst>(k);
+    if (!J || !w || !vx || !vy || !dB || !dK || !st || !sr || !scs || !stat || !gx || !gy || !dc || !dh || !dG || !dI)
+        return fail(ctx, PDCS_ERR_HIP, "adjoint workspace allocation failed (%ld doubles)", (long)jw);
+    HIPCHK(ctx, copy_sync(ctx, gx, dL_dx, sizeof(double) * sn, hipMemcpyHostToDevice));
+    if (dL_dy) HIPCHK(ctx, copy_sync(ctx, gy, dL_dy, sizeof(double) * sm, hipMemcpyHostToDevice));
+    std::vector<AdjInst> A(k);
+    int64_t jo = 0, so = 0;
+    for (int64_t q = 0; q < k; ++q) {
+        const BatchInst &I = b->inst[q];
+        const BatchData &B = b->B;
+        AdjInst &a = A[q];
+        std::memset(&a, 0, sizeof(a));
+        a.n = I.n; a.m = I.m; a.nb = I.nb; a.mzero = I.mzero; a.mnonneg = I.mnonneg; a.nsoc = I.nsoc;
+        a.rowptr = B.rowptr + I.ro; a.col = B.col + I.ko; a.trowptr = B.trowptr + I.tro; a.tcol = B.tcol + I.ko;
+        a.soc_start = B.soc_start + I.so; a.soc_dim = B.soc_dim + I.so;
+        a.val = B.val64 + I.ko; a.tval = B.tval64 + I.ko; a.Dl = B.Dl + I.yo; a.Dr = B.Dr + I.xo;
+        a.c = B.c_raw + I.xo; a.h = B.h_raw + I.yo; a.bl = B.bl_raw + I.xo; a.bu = B.bu_raw + I.xo;
+        a.x = b->xo + I.xo; a.y = b->yo + I.yo;
+        a.gx = gx + I.xo; a.gy = gy + I.yo;
+        a.J = J + jo; a.w = w + I.xo + I.yo;
+        a.vx = vx + I.xo; a.vy = vy + I.yo; a.dB = dB + I.xo; a.dK = dK + I.yo;
+        a.soc_t = st + so; a.soc_r = sr + so; a.soc_cs = scs + so;
+        a.dc = dc + I.xo; a.dh = dh + I.yo; a.dG = dG + I.ko; a.status = stat + q;
+        jo += (int64_t)(I.n + I.m) * (I.n + I.m);
+        so += I.nsoc;
+    }
+    HIPCHK(ctx, copy_sync(ctx, dI, A.data(), sizeof(AdjInst) * k, hipMemcpyHostToDevice));
+    launch_adjoint(ctx->stream, dI, (int)k);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, copy_sync(ctx, dL_dc, dc, sizeof(double) * sn, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, copy_sync(ctx, dL_dh, dh, sizeof(double) * sm, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, copy_sync(ctx, dL_dG, dG, sizeof(double) * b->sum_nnz, hipMemcpyDeviceToHost));
+    std::vector<int32_t> hs(k);
+    HIPCHK(ctx, copy_sync(ctx, hs.data(), stat, sizeof(int32_t) * k, hipMemcpyDeviceToHost));
+    if (status) std::memcpy(status, hs.data(), sizeof(int32_t) * k);
     return PDCS_OK;
 }
 

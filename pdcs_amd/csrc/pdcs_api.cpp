@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/pdcs.h"
+#include "adjoint.h"
 #include "host_common.h"
 #include "internal.h"
 
@@ -1280,6 +1281,68 @@ int pdcs_get_solution(pdcs_problem *P, double *x, double *y, double *s) {
         for (int64_t i = 0; i < P->m; ++i)
             for (int32_t k = P->rowptr[i]; k < P->rowptr[i + 1]; ++k) s[P->col[k]] -= P->vals[k] * P->sol_y[i];
     }
+    return PDCS_OK;
+}
+
+int pdcs_adjoint(pdcs_problem *P, const double *dL_dx, const double *dL_dy, double *dL_dc, double *dL_dh,
+                 double *dL_dG) {
+    if (!P || !dL_dx || !dL_dc || !dL_dh || !dL_dG) return fail(P ? P->ctx : nullptr, PDCS_ERR_ARG, "NULL argument");
+    pdcs_ctx *ctx = P->ctx;
+    if (!P->solved) return fail(ctx, PDCS_ERR_STATE, "no solution: call pdcs_solve first");
+    if (P->nexp + P->ndexp > 0) return fail(ctx, PDCS_ERR_UNSUPPORTED, "the adjoint supports zero, nonnegative and SOC rows");
+    const int64_t n = P->n, m = P->m, N = n + m;
+    if (N > 16384) return fail(ctx, PDCS_ERR_UNSUPPORTED, "the dense adjoint solve is limited to n + m <= 16384");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    std::vector<int32_t> sst, sdim;
+    int64_t off = P->mzero + P->mnonneg;
+    for (int64_t q : P->soc) {
+        sst.push_back((int32_t)off);
+        sdim.push_back((int32_t)q);
+        off += q;
+    }
+    const int64_t ns = (int64_t)sst.size();
+    std::vector<double> bl(n, -INFINITY), bu(n, INFINITY);
+    for (int64_t j = 0; j < P->nb; ++j) { bl[j] = P->bl[j]; bu[j] = P->bu[j]; }
+    DevBuf W;
+    double *J = W.alloc<double>(N * N), *w = W.alloc<double>(N), *vx = W.alloc<double>(n), *vy = W.alloc<double>(m);
+    double *dB = W.alloc<double>(n), *dK = W.alloc<double>(m), *st = W.alloc<double>(ns), *sr = W.alloc<double>(ns);
+    int32_t *scs = W.alloc<int32_t>(ns), *stat = W.alloc<int32_t>(1), *dss = W.alloc<int32_t>(ns), *dsd = W.alloc<int32_t>(ns);
+    double *x = W.alloc<double>(n), *y = W.alloc<double>(m), *gx = W.alloc<double>(n), *gy = W.alloc<double>(m);
+    double *dbl = W.alloc<double>(n), *dbu = W.alloc<double>(n), *dc = W.alloc<double>(n), *dh = W.alloc<double>(m);
+    double *dG = W.alloc<double>(P->nnz);
+    AdjInst *dI = W.alloc<AdjInst>(1);
+    if (!J || !w || !vx || !vy || !dB || !dK || !st || !sr || !scs || !stat || !dss || !dsd || !x || !y || !gx || !gy ||
+        !dbl || !dbu || !dc || !dh || !dG || !dI)
+        return fail(ctx, PDCS_ERR_HIP, "adjoint workspace allocation failed");
+    HIPCHK(ctx, copy_sync(ctx, x, P->sol_x.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(ctx, copy_sync(ctx, y, P->sol_y.data(), sizeof(double) * m, hipMemcpyHostToDevice));
+    HIPCHK(ctx, copy_sync(ctx, gx, dL_dx, sizeof(double) * n, hipMemcpyHostToDevice));
+    if (dL_dy) HIPCHK(ctx, copy_sync(ctx, gy, dL_dy, sizeof(double) * m, hipMemcpyHostToDevice));
+    HIPCHK(ctx, copy_sync(ctx, dbl, bl.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(ctx, copy_sync(ctx, dbu, bu.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    if (ns > 0) {
+        HIPCHK(ctx, copy_sync(ctx, dss, sst.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice));
+        HIPCHK(ctx, copy_sync(ctx, dsd, sdim.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice));
+    }
+    AdjInst a;
+    std::memset(&a, 0, sizeof(a));
+    a.n = (int32_t)n; a.m = (int32_t)m; a.nb = (int32_t)P->nb; a.mzero = (int32_t)P->mzero;
+    a.mnonneg = (int32_t)P->mnonneg; a.nsoc = (int32_t)ns;
+    a.rowptr = P->d_rowptr; a.col = P->d_col; a.trowptr = P->d_trowptr; a.tcol = P->d_tcol;
+    a.soc_start = dss; a.soc_dim = dsd;
+    a.val = P->d_vals; a.tval = P->d_tvals; a.Dl = P->Dl; a.Dr = P->Dr;
+    a.c = P->c_raw; a.h = P->h_raw; a.bl = dbl; a.bu = dbu; a.x = x; a.y = y; a.gx = gx; a.gy = gy;
+    a.J = J; a.w = w; a.vx = vx; a.vy = vy; a.dB = dB; a.dK = dK; a.soc_t = st; a.soc_r = sr; a.soc_cs = scs;
+    a.dc = dc; a.dh = dh; a.dG = dG; a.status = stat;
+    HIPCHK(ctx, copy_sync(ctx, dI, &a, sizeof(a), hipMemcpyHostToDevice));
+    launch_adjoint(ctx->stream, dI, 1);
+    HIPCHK(ctx, hipGetLastError());
+    int32_t hs = 0;
+    HIPCHK(ctx, copy_sync(ctx, &hs, stat, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (hs != 0) return fail(ctx, PDCS_ERR_STATE, "the KKT Jacobian is singular at the solution");
+    HIPCHK(ctx, copy_sync(ctx, dL_dc, dc, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, copy_sync(ctx, dL_dh, dh, sizeof(double) * m, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, copy_sync(ctx, dL_dG, dG, sizeof(double) * P->nnz, hipMemcpyDeviceToHost));
     return PDCS_OK;
 }
 

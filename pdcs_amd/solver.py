@@ -131,6 +131,16 @@ class Problem:
         _lib.check(self.ctx.L.pdcs_get_solution(self.h, _p(x), _p(y), _p(s)), self.ctx.h)
         return x, y, s
 
+    def adjoint(self, dL_dx, dL_dy=None):
+        """Gradients (dL/dc, dL/dh, dL/dG on the CSR pattern) of a loss of the
+        last solution, by implicit differentiation of the KKT fixed point."""
+        gx = np.ascontiguousarray(dL_dx, dtype=np.float64)
+        gy = np.ascontiguousarray(dL_dy, dtype=np.float64) if dL_dy is not None else None
+        dc, dh, dG = np.zeros(self.n), np.zeros(self.m), np.zeros(len(self._arrs[3]))
+        _lib.check(self.ctx.L.pdcs_adjoint(self.h, _p(gx), _p(gy) if gy is not None else None, _p(dc), _p(dh), _p(dG)),
+                   self.ctx.h)
+        return dc, dh, dG
+
     def scaling(self):
         Dr = np.zeros(self.n)
         Dl = np.zeros(self.m)
@@ -211,6 +221,26 @@ class Batch:
         y = np.zeros(self._yo[-1])
         _lib.check(self.ctx.L.pdcs_batch_get_solution(self.h, _p(x), _p(y)), self.ctx.h)
         return [(x[self._xo[i]:self._xo[i + 1]], y[self._yo[i]:self._yo[i + 1]]) for i in range(len(self.problems))]
+
+    def adjoint(self, dL_dx, dL_dy=None):
+        """Per-instance gradients [(dL/dc, dL/dh, dL/dG)] of losses of the last
+        solutions; dL_dx (and dL_dy) are lists with one array per instance.
+        Instances whose KKT Jacobian is singular get None."""
+        gx = np.ascontiguousarray(np.concatenate(list(dL_dx)), dtype=np.float64)
+        gy = np.ascontiguousarray(np.concatenate(list(dL_dy)), dtype=np.float64) if dL_dy is not None else None
+        dc, dh = np.zeros(self._xo[-1]), np.zeros(self._yo[-1])
+        ko = np.cumsum([0] + [P.nnz for P in self.problems])
+        dG = np.zeros(ko[-1])
+        st = np.zeros(len(self.problems), dtype=np.int32)
+        _lib.check(self.ctx.L.pdcs_batch_adjoint(self.h, _p(gx), _p(gy) if gy is not None else None, _p(dc), _p(dh),
+                                                 _p(dG), _p(st, C.c_int32)), self.ctx.h)
+        out = []
+        for i in range(len(self.problems)):
+            if st[i] != 0:
+                out.append(None)
+                continue
+            out.append((dc[self._xo[i]:self._xo[i + 1]], dh[self._yo[i]:self._yo[i + 1]], dG[ko[i]:ko[i + 1]]))
+        return out
 
     def close(self):
         if self.h and self.ctx.h:

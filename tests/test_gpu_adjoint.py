@@ -1,0 +1,88 @@
+"""GPU adjoint (pdcs_adjoint / pdcs_batch_adjoint, torch.autograd) against the
+oracle's dual-number forward mode, which is itself pinned against central
+finite differences in tests/test_oracle.py."""
+import copy
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from pdcs_amd import instances
+from pdcs_amd.solver import Batch, Problem
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5   # relative agreement of directional derivatives
+
+
+def _accurate(P):
+    """oracle solution refined by constant-step PDHG (KKT ~1e-14)"""
+    r, x, y = oracle.solve(P, rel_tol=1e-8, abs_tol=1e-14)
+    assert r["exit_code"] == 0
+    x, y, _, _, _ = oracle.tangent(P, x, y, iters=300000, tol=1e-15)
+    return x, y
+
+
+def _directions(P, rng):
+    z = lambda k: np.zeros(k)
+    r = lambda k: rng.standard_normal(k)
+    return [(r(P.n), z(P.m), z(P.nnz)), (z(P.n), r(P.m), z(P.nnz)), (z(P.n), z(P.m), r(P.nnz)),
+            (r(P.n), r(P.m), r(P.nnz))]
+
+
+def _check(P, grads, xt, rng):
+    x, y = _accurate(P)
+    dc_g, dh_g, dG_g = grads
+    for dc, dh, dG in _directions(P, rng):
+        _, _, dx, _, _ = oracle.tangent(P, x, y, dc, dh, dG, iters=400000, tol=1e-15)
+        ref = 2 * (x - xt) @ dx
+        got = dc_g @ dc + dh_g @ dh + dG_g @ dG
+        assert abs(got - ref) <= TOL * max(abs(ref), 1e-8 * np.linalg.norm(x - xt)), (got, ref)
+
+
+def test_adjoint_single(ctx):
+    P = instances.workload4_layer(3)
+    rng = np.random.default_rng(11)
+    xt = rng.standard_normal(P.n)
+    prob = Problem.from_instance(P, ctx)
+    res = prob.solve(rel_tol=1e-8, abs_tol=1e-14, time_limit=60.0)
+    assert res["exit_code"] == 0
+    x, _, _ = prob.solution()
+    grads = prob.adjoint(2 * (x - xt))
+    _check(P, grads, xt, rng)
+
+
+def test_adjoint_batch_matches_single_and_forward_mode(ctx):
+    probs = [instances.workload4_layer(k) for k in range(24)]
+    rng = np.random.default_rng(12)
+    xts = [rng.standard_normal(P.n) for P in probs]
+    B = Batch(probs, ctx)
+    res, _ = B.solve(rel_tol=1e-8, abs_tol=1e-14, time_limit=60.0)
+    assert all(r["exit_code"] == 0 for r in res)
+    sols = B.solutions()
+    grads = B.adjoint([2 * (x - xt) for (x, _), xt in zip(sols, xts)])
+    assert all(g is not None for g in grads)
+    for k in (0, 7, 23):
+        _check(probs[k], grads[k], xts[k], rng)
+    # the single-problem adjoint on the same solution data agrees
+    prob = Problem.from_instance(probs[5], ctx)
+    prob.solve(rel_tol=1e-8, abs_tol=1e-14, time_limit=60.0)
+    x1, _, _ = prob.solution()
+    g1 = prob.adjoint(2 * (x1 - xts[5]))
+    for a, b in zip(g1, grads[5]):
+        assert np.abs(a - b).max() <= 1e-5 * (1 + np.abs(a).max())
+
+
+def test_torch_autograd(ctx):
+    torch = pytest.importorskip("torch")
+    from pdcs_amd.torch_layer import ConicSolve
+    P = instances.workload4_layer(9)
+    rng = np.random.default_rng(13)
+    xt = torch.as_tensor(rng.standard_normal(P.n))
+    c = torch.tensor(P.c, requires_grad=True)
+    h = torch.tensor(P.h, requires_grad=True)
+    v = torch.tensor(P.vals, requires_grad=True)
+    x = ConicSolve.apply(c, h, v, P, dict(rel_tol=1e-8, abs_tol=1e-14))
+    loss = ((x - xt) ** 2).sum()
+    loss.backward()
+    _check(P, (c.grad.numpy(), h.grad.numpy(), v.grad.numpy()), xt.numpy(), rng)

@@ -248,3 +248,34 @@ def test_oracle_logreg_vs_direct_minimisation():
     opt = scipy.minimize(f, np.zeros(2 * p), jac=True, method="L-BFGS-B", bounds=[(0, None)] * (2 * p),
                          options={"ftol": 1e-15, "gtol": 1e-12, "maxiter": 10000})
     assert abs(res["pobj"] - opt.fun) <= 1e-6 * (1 + abs(opt.fun)), (res["pobj"], opt.fun)
+
+
+def test_dual_number_forward_mode_vs_finite_differences():
+    """oracle_tangent (dual-number PDHG on the original data) against central
+    finite differences (eps 1e-6) of refined solutions, for the loss
+    ||x*(c, h, G) - x_target||^2 on a differentiation-layer instance."""
+    import copy
+    P = instances.soc_program(60, 120, 0.08, 10, seed=4242)
+
+    def accurate(Q):
+        r, x, y = oracle.solve(Q, rel_tol=1e-8, abs_tol=1e-14)
+        assert r["exit_code"] == 0
+        x, y, _, _, _ = oracle.tangent(Q, x, y, iters=300000, tol=1e-15)
+        return x, y
+
+    x, y = accurate(P)
+    k = oracle.kkt(P, x, y)
+    assert max(k["rel_primal_res"], k["rel_dual_res"], k["rel_gap"]) < 1e-11
+    rng = np.random.default_rng(5)
+    xt = rng.standard_normal(P.n)
+    L = lambda xx: float(np.sum((xx - xt) ** 2))
+    for _ in range(3):
+        dc, dh, dG = rng.standard_normal(P.n), rng.standard_normal(P.m), rng.standard_normal(P.nnz)
+        _, _, dx, _, _ = oracle.tangent(P, x, y, dc, dh, dG, iters=400000, tol=1e-15)
+        d = 2 * (x - xt) @ dx
+        eps = 1e-6
+        Q1, Q2 = copy.copy(P), copy.copy(P)
+        Q1.c, Q1.h, Q1.vals = P.c + eps * dc, P.h + eps * dh, P.vals + eps * dG
+        Q2.c, Q2.h, Q2.vals = P.c - eps * dc, P.h - eps * dh, P.vals - eps * dG
+        fd = (L(accurate(Q1)[0]) - L(accurate(Q2)[0])) / (2 * eps)
+        assert abs(d - fd) <= 1e-6 * abs(fd), (d, fd)
