@@ -38,7 +38,94 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--reps", type=int, default=50)
     ap.add_argument("--no-graph", action="store_true", help="plain launches (for profilers that cannot follow graphs)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-portfolio and adjoint measurements")
+    ap.add_argument("--schedule64", type=str, default="", metavar="LG,LGT,CG,CGT",
+                    help="pin the FP64 SpMV row schedule (lanes of G, of G', workgroup caps) instead of autotuning")
+    ap.add_argument("--schedule32", type=str, default="", metavar="LG,LGT,CG,CGT",
+                    help="pin the FP32 SpMV row schedule")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--layers", type=int, default=256)
     return ap.parse_args()
+
+
+def batch_section(a, np, cpu):
+    """4096 uneven SOC portfolios in one persistent launch (FP64, and FP32 with
+    FP64 polish): total time, per-instance p50/p99; the single-instance path on
+    a sample for comparison; the oracle on a sample as the CPU baseline."""
+    from pdcs_amd import instances
+    from pdcs_amd.solver import Batch, Problem
+
+    t0 = time.time()
+    probs = instances.workload3(count=a.batch_count)
+    B = Batch(probs)
+    setup = time.time() - t0
+    out = {"workload": f"{a.batch_count} uneven SOC portfolios, n ~ U{{20..500}}, k = n/10, budget + box",
+           "sum_n": int(sum(P.n for P in probs)), "sum_m": int(sum(P.m for P in probs)),
+           "sum_nnz": int(sum(P.nnz for P in probs)), "setup_s": round(setup, 3)}
+    for prec, tag in ((0, "fp64"), (1, "mixed")):
+        B.solve(precision=prec, time_limit=600.0)          # warm-up
+        res, total = B.solve(precision=prec, time_limit=600.0)
+        tm = np.array([r["time"] for r in res])
+        it = np.array([r["iter"] for r in res])
+        ok = int(sum(r["exit_code"] == 0 for r in res))
+        out[tag] = {"total_s": round(total, 4), "instances_per_s": round(len(res) / total, 1),
+                    "p50_instance_ms": round(float(np.percentile(tm, 50)) * 1e3, 2),
+                    "p99_instance_ms": round(float(np.percentile(tm, 99)) * 1e3, 2),
+                    "iters_mean": round(float(it.mean()), 1), "optimal": ok}
+        print(f"[bench] batch {tag}: {total:.3f}s for {len(res)} instances ({ok} optimal), "
+              f"p50 {out[tag]['p50_instance_ms']} ms p99 {out[tag]['p99_instance_ms']} ms", file=sys.stderr, flush=True)
+    B.close()
+    sample = [probs[i] for i in range(0, len(probs), max(1, len(probs) // 16))][:16]
+    ts = []
+    for P in sample:
+        prob = Problem.from_instance(P)
+        prob.solve(time_limit=60.0)
+        ts.append(prob.solve(time_limit=60.0)["time_solve"])
+        prob.close()
+    out["single_instance_path_s_mean"] = round(float(np.mean(ts)), 5)
+    out["single_instance_path_est_total_s"] = round(float(np.mean(ts)) * len(probs), 2)
+    if cpu:
+        os.environ.setdefault("OMP_NUM_THREADS", "1")
+        from oracle import oracle
+
+        t = time.time()
+        for P in sample:
+            oracle.solve(P)
+        per = (time.time() - t) / len(sample)
+        out["cpu_baseline"] = {"value": round(per * len(probs), 2), "unit": "s", "cores": 1, "kind": "port",
+                               "sample": f"FP64 oracle solves of {len(sample)} instances of the batch on one host "
+                                         f"thread ({per * 1e3:.1f} ms each), scaled to {len(probs)} instances"}
+    return out
+
+
+def adjoint_section(a, np):
+    """256 SOC layers (n = 200, m = 400, 40 SOC(10)): batched solve and batched
+    adjoint (dL/dc, dL/dh, dL/dG of ||x* - x_target||^2)."""
+    from pdcs_amd import instances
+    from pdcs_amd.solver import Batch
+
+    probs = [instances.workload4_layer(k) for k in range(a.layers)]
+    rng = np.random.default_rng(1004)
+    xts = [rng.standard_normal(P.n) for P in probs]
+    B = Batch(probs)
+    B.solve(time_limit=600.0)
+    t = time.time()
+    res, total = B.solve(time_limit=600.0)
+    sols = B.solutions()
+    g = [2 * (x - xt) for (x, _), xt in zip(sols, xts)]
+    B.adjoint(g)                                            # warm-up
+    t1 = time.time()
+    grads = B.adjoint(g)
+    t_adj = time.time() - t1
+    B.close()
+    out = {"workload": f"{a.layers} SOC layers n=200 m=400 (40 SOC(10)), loss ||x* - x_target||^2",
+           "solve_s": round(total, 4), "adjoint_s": round(t_adj, 4),
+           "optimal": int(sum(r["exit_code"] == 0 for r in res)),
+           "nonsingular": int(sum(x is not None for x in grads))}
+    print(f"[bench] layers: solve {total:.3f}s adjoint {t_adj:.3f}s", file=sys.stderr, flush=True)
+    return out
 
 
 def main():
@@ -52,11 +139,25 @@ def main():
     from pdcs_amd import instances
     from pdcs_amd.solver import Problem
 
+    if a.copy_probe > 0:
+        import torch
+
+        src = torch.ones(a.copy_probe * (1 << 20) // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copy probe: 5 copies of {a.copy_probe} MiB", file=sys.stderr, flush=True)
+        del src, dst
     t0 = time.time()
     P = instances.workload2(rows=a.rows, cols=a.cols, density=a.density)
     print(f"[bench] workload {P.name}: n={P.n} m={P.m} nnz={P.nnz} generated in {time.time() - t0:.1f}s",
           file=sys.stderr, flush=True)
     prob = Problem.from_instance(P)
+    if a.schedule64:
+        prob.set_schedule(0, [int(v) for v in a.schedule64.split(",")])
+    if a.schedule32:
+        prob.set_schedule(1, [int(v) for v in a.schedule32.split(",")])
 
     def run(precision, steps, warmup, tag):
         out = []
@@ -89,11 +190,24 @@ def main():
 
     sm, s64 = summary(mixed), summary(fp64)
     # dominant kernel of the mixed path
-    dom = max(kt_mixed, key=lambda k: kt_mixed[k]["ms"])
+    # dominant single kernel of the mixed attempt (the cone projection entry is a
+    # chain of four launches and is reported with the others in kernels_mixed)
+    sch32, sch64 = prob.schedule(1), prob.schedule(0)
+    single = {"primal_step": f"pdcs::k_primal<float, {sch32[1]}>", "dual_step": f"pdcs::k_dual<float, {sch32[0]}>",
+              "atdy_dot": f"pdcs::k_atdy<float, {sch32[1]}>"}
+    dom = max(single, key=lambda k: kt_mixed[k]["ms"])
     ach = kt_mixed[dom]["bytes"] / (kt_mixed[dom]["ms"] * 1e-3) / 1e9
-    roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernels_mixed": kt_mixed, "kernels_fp64": kt_fp64}
+    traffic, tsrc = None, None
+    tfile = os.path.join(ROOT, "profiles", "traffic_soc_regression_200000x50000.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            tj = json.load(f)
+        if single[dom] in tj.get("kernels", {}):
+            traffic, tsrc = tj["kernels"][single[dom]]["bytes_per_launch"], os.path.relpath(tfile, ROOT)
+    roof = {"bound": "hbm", "kernel": single[dom], "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsrc,
+            "kernel_ms": round(kt_mixed[dom]["ms"], 5), "algorithmic_bytes": kt_mixed[dom]["bytes"],
+            "spmv_schedule": {"fp32": sch32, "fp64": sch64}, "kernels_mixed": kt_mixed, "kernels_fp64": kt_fp64}
     cpu = None
     if not a.no_cpu:
         os.environ.setdefault("OMP_NUM_THREADS", "1")
@@ -105,6 +219,10 @@ def main():
                "iters_per_s": round(1.0 / per_attempt, 2),
                "sample": f"FP64 oracle, first {done} PDHG attempts of this workload timed on one host thread "
                          f"({t_cpu:.1f}s), scaled to the {s64['iters']:.0f} attempts the FP64 GPU solve needed"}
+    extra = {}
+    if not a.no_batch:
+        extra["batched_portfolios"] = batch_section(a, np, not a.no_cpu)
+        extra["adjoint_layers"] = adjoint_section(a, np)
     line = {"metric": METRIC, "value": round(sm["time_to_tol_s"], 5), "unit": "s", "n_gpus": 1, "steps": a.steps,
             "warmup": a.warmup, "ms_per_step": round(sm["time_to_tol_s"] * 1e3, 3), "higher_is_better": False,
             "scaling": "weak", "vs_baseline": None, "dtype": "fp32 iterates+SpMV / fp64 reductions+polish",
@@ -112,7 +230,7 @@ def main():
             "config": {"workload": f"large SOC robust regression {a.rows}x{a.cols} density {a.density:g}",
                        "n": P.n, "m": P.m, "nnz": P.nnz, "cones": f"1 SOC({P.m})", "tol": 1e-6},
             "mixed": sm, "fp64": s64, "speedup_mixed_vs_fp64": round(s64["time_to_tol_s"] / sm["time_to_tol_s"], 3),
-            "roofline": roof, "cpu_baseline": cpu}
+            "roofline": roof, "cpu_baseline": cpu, **extra}
     print(json.dumps(line), flush=True)
 
 

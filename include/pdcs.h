@@ -172,6 +172,14 @@ int pdcs_project_dual(pdcs_ctx *ctx, int32_t precision, const pdcs_cones *cones,
 /* In-place projection of r (m) onto K itself (con_proj!, codes 2/3/5/13/16). */
 int pdcs_project_primal_cone(pdcs_ctx *ctx, const pdcs_cones *cones, int64_t m, double *r);
 
+/* SpMV row schedule of the attempt kernels for one precision: sched = {lanes
+ * per row of G, of G', workgroup cap of G, of G'} (lanes a power of two in
+ * 1..64).  pdcs_set_schedule pins it (no autotuning for that precision;
+ * reproducible kernel variants), pdcs_get_schedule reads the one in use.  No
+ * counterpart in the reference (cuSPARSE chooses internally). */
+int pdcs_set_schedule(pdcs_problem *p, int32_t precision, const int32_t *sched);
+int pdcs_get_schedule(pdcs_problem *p, int32_t precision, int32_t *sched);
+
 /* ---- batches of independent problems, solved in one persistent launch ----
  * No counterpart in the reference, which solves instances one at a time
  * (rpdhg_gpu_solve per instance, rpdhg_alg_gpu_gen.jl:1219): each instance is

@@ -124,7 +124,8 @@ unsigned attempt_grid(int64_t rows, int lanes, int cap);
 template <typename T>
 void launch_attempt_part(int part_id, hipStream_t s, int slot, const Csr<T> &G, const Csr<T> &Gt,
                          const ProbVec<T> &P, const ConeLayout &K, const double *d2, Iter<T> it,
-                         StepState *st, double *part, int64_t part_stride);
+                         StepState *st, double *part, int64_t part_stride, hipEvent_t ev0 = nullptr,
+                         hipEvent_t ev1 = nullptr);
 
 // continuation of an attempt whose decision was deferred to the host (exit check)
 void launch_continue(hipStream_t s, int slot, StepState *st);

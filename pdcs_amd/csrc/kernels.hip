@@ -19,6 +19,8 @@
 // value/vector type T is double (FP64 path) or float (mixed path), products
 // are accumulated in double.  Reductions are two-stage and in fixed order, so
 // results are run-to-run deterministic.
+#include <hip/hip_ext.h>
+
 #include "internal.h"
 #include "device_common.h"
 #include "exp_cone.h"
@@ -658,20 +660,29 @@ static void launch_cones(hipStream_t s, const ConeLayout &K, const double *d2, T
             part ? part + soc_blocks(K) + (K.nsoc_big > 0 ? K.nbig_blocks : 0) : nullptr);
 }
 
+// ev0/ev1 (optional): for the single-kernel parts 0, 1, 3 and 4 the events are
+// recorded at the kernel's own start and end (hipExtLaunchKernelGGL), so their
+// elapsed time is the kernel's execution time, as a profiler reports it
 template <typename T>
 void launch_attempt_part(int part_id, hipStream_t s, int slot, const Csr<T> &G, const Csr<T> &Gt,
                          const ProbVec<T> &P, const ConeLayout &K, const double *d2, Iter<T> it, StepState *st,
-                         double *part, int64_t stride) {
+                         double *part, int64_t stride, hipEvent_t ev0, hipEvent_t ev1) {
     const unsigned bp = attempt_grid(P.n, Gt.lanes, Gt.grid);
     const unsigned bd = attempt_grid(P.m, G.lanes, G.grid);
     switch (part_id) {
     case 0:
-#define KP(L) k_primal<T, L><<<bp, kBlock, 0, s>>>(Gt, P, it.ys, it.xs, it.x, it.xbar, st, slot, part)
+#define KP(L)                                                                                                \
+    if (ev0) hipExtLaunchKernelGGL(k_primal<T, L>, dim3(bp), dim3(kBlock), 0, s, ev0, ev1, 0, Gt, P, it.ys,    \
+                                   (const T *)it.xs, it.x, it.xbar, st, slot, part);                        \
+    else k_primal<T, L><<<bp, kBlock, 0, s>>>(Gt, P, it.ys, it.xs, it.x, it.xbar, st, slot, part)
         PDCS_LANES(Gt.lanes, KP)
 #undef KP
         break;
     case 1:
-#define KD(L) k_dual<T, L><<<bd, kBlock, 0, s>>>(G, P, K, it.xbar, it.ys, it.y, it.dy, st, slot, part + stride)
+#define KD(L)                                                                                                \
+    if (ev0) hipExtLaunchKernelGGL(k_dual<T, L>, dim3(bd), dim3(kBlock), 0, s, ev0, ev1, 0, G, P, K,           \
+                                   (const T *)it.xbar, (const T *)it.ys, it.y, it.dy, st, slot, part + stride); \
+    else k_dual<T, L><<<bd, kBlock, 0, s>>>(G, P, K, it.xbar, it.ys, it.y, it.dy, st, slot, part + stride)
         PDCS_LANES(G.lanes, KD)
 #undef KD
         break;
@@ -679,13 +690,20 @@ void launch_attempt_part(int part_id, hipStream_t s, int slot, const Csr<T> &G, 
         launch_cones<T>(s, K, d2, it.y, it.ys, it.dy, st, slot, part + stride + bd);
         break;
     case 3:
-#define KA(L) k_atdy<T, L><<<bp, kBlock, 0, s>>>(Gt, P.n, it.dy, it.x, it.xs, st, slot, part + 2 * stride)
+#define KA(L)                                                                                                \
+    if (ev0) hipExtLaunchKernelGGL(k_atdy<T, L>, dim3(bp), dim3(kBlock), 0, s, ev0, ev1, 0, Gt, P.n,          \
+                                   (const T *)it.dy, (const T *)it.x, (const T *)it.xs, st, slot, part + 2 * stride); \
+    else k_atdy<T, L><<<bp, kBlock, 0, s>>>(Gt, P.n, it.dy, it.x, it.xs, st, slot, part + 2 * stride)
         PDCS_LANES(Gt.lanes, KA)
 #undef KA
         break;
     case 4: {
         const int ncone = cone_part_blocks(K);
-        k_step<<<1, 1024, 0, s>>>(st, slot, part, stride, (int)bp, (int)(bd + ncone), (int)bp);
+        if (ev0)
+            hipExtLaunchKernelGGL(k_step, dim3(1), dim3(1024), 0, s, ev0, ev1, 0, st, slot, (const double *)part, stride,
+                                  (int)bp, (int)(bd + ncone), (int)bp);
+        else
+            k_step<<<1, 1024, 0, s>>>(st, slot, part, stride, (int)bp, (int)(bd + ncone), (int)bp);
         break;
     }
     default:
@@ -695,10 +713,10 @@ void launch_attempt_part(int part_id, hipStream_t s, int slot, const Csr<T> &G, 
 }
 template void launch_attempt_part<double>(int, hipStream_t, int, const Csr<double> &, const Csr<double> &,
                                           const ProbVec<double> &, const ConeLayout &, const double *,
-                                          Iter<double>, StepState *, double *, int64_t);
+                                          Iter<double>, StepState *, double *, int64_t, hipEvent_t, hipEvent_t);
 template void launch_attempt_part<float>(int, hipStream_t, int, const Csr<float> &, const Csr<float> &,
                                          const ProbVec<float> &, const ConeLayout &, const double *,
-                                         Iter<float>, StepState *, double *, int64_t);
+                                         Iter<float>, StepState *, double *, int64_t, hipEvent_t, hipEvent_t);
 
 template <typename T>
 void launch_attempt(hipStream_t s, int slot, const Csr<T> &G, const Csr<T> &Gt, const ProbVec<T> &P,

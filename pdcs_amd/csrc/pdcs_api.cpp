@@ -1245,15 +1245,27 @@ int time_kernels(pdcs_problem *P, int reps, double *ms, double *bytes) {
         h.restart_freq = 0;
         h.kkt_restart_freq = 0;
         HIPCHK(ctx, hipMemcpyAsync(P->st, &h, sizeof(StepState), hipMemcpyHostToDevice, s));
-        launch_attempt_part<T>(k, s, 0, G, Gt, PV, P->K, P->dsc2, I.it, P->st, P->apart, P->apart_stride);  // warm
-        HIPCHK(ctx, hipEventRecord(e0, s));
-        for (int r = 0; r < reps; ++r)
-            launch_attempt_part<T>(k, s, 0, G, Gt, PV, P->K, P->dsc2, I.it, P->st, P->apart, P->apart_stride);
-        HIPCHK(ctx, hipEventRecord(e1, s));
-        HIPCHK(ctx, hipEventSynchronize(e1));
-        float el = 0.f;
-        HIPCHK(ctx, hipEventElapsedTime(&el, e0, e1));
-        if (ms) ms[k] = (double)el / reps;
+        // Each repetition runs a whole attempt (all six parts, as in a solve, so
+        // the caches see the solve's traffic) and times part k: single kernels
+        // with events recorded at their own start and end (their execution
+        // time, as a profiler reports it), the cone chain between events
+        // around its launches.
+        double tot = 0.0;
+        for (int r = 0; r <= reps; ++r) {
+            for (int q = 0; q < kAttemptKernels; ++q) {
+                const bool timed = q == k;
+                const bool single = k != 2 && k != 5;
+                if (timed && !single) HIPCHK(ctx, hipEventRecord(e0, s));
+                launch_attempt_part<T>(q, s, 0, G, Gt, PV, P->K, P->dsc2, I.it, P->st, P->apart, P->apart_stride,
+                                       timed && single ? e0 : nullptr, timed && single ? e1 : nullptr);
+                if (timed && !single) HIPCHK(ctx, hipEventRecord(e1, s));
+            }
+            HIPCHK(ctx, hipEventSynchronize(e1));
+            float el = 0.f;
+            HIPCHK(ctx, hipEventElapsedTime(&el, e0, e1));
+            if (r > 0) tot += el;   // repetition 0 warms up
+        }
+        if (ms) ms[k] = tot / reps;
         if (bytes) bytes[k] = b[k];
     }
     hipEventDestroy(e0);
@@ -1268,6 +1280,35 @@ extern "C" {
 int pdcs_time_kernels(pdcs_problem *P, int32_t precision, int32_t reps, double *ms, double *bytes) {
     if (!P || reps <= 0) return fail(P ? P->ctx : nullptr, PDCS_ERR_ARG, "bad argument");
     return precision == PDCS_FP64 ? time_kernels<double>(P, reps, ms, bytes) : time_kernels<float>(P, reps, ms, bytes);
+}
+
+int pdcs_set_schedule(pdcs_problem *P, int32_t precision, const int32_t *sched) {
+    if (!P || !sched) return fail(P ? P->ctx : nullptr, PDCS_ERR_ARG, "NULL argument");
+    for (int q = 0; q < 2; ++q)
+        if (sched[q] < 1 || sched[q] > 64 || (sched[q] & (sched[q] - 1)))
+            return fail(P->ctx, PDCS_ERR_ARG, "lanes per row must be a power of two in 1..64");
+    for (int q = 2; q < 4; ++q)
+        if (sched[q] < 1 || sched[q] > kTuneMaxGrid) return fail(P->ctx, PDCS_ERR_ARG, "workgroup cap out of range");
+    auto &sc = P->sched[precision == PDCS_FP64 ? 0 : 1];
+    sc = {sched[0], sched[1], sched[2], sched[3], true};
+    if (precision == PDCS_FP64) {
+        for (auto g : P->graphs64) hipGraphExecDestroy(g);
+        P->graphs64.clear();
+    } else {
+        for (auto g : P->graphs32) hipGraphExecDestroy(g);
+        P->graphs32.clear();
+    }
+    return PDCS_OK;
+}
+
+int pdcs_get_schedule(pdcs_problem *P, int32_t precision, int32_t *sched) {
+    if (!P || !sched) return fail(P ? P->ctx : nullptr, PDCS_ERR_ARG, "NULL argument");
+    const auto &sc = P->sched[precision == PDCS_FP64 ? 0 : 1];
+    sched[0] = sc.lanesG;
+    sched[1] = sc.lanesGt;
+    sched[2] = sc.gridG;
+    sched[3] = sc.gridGt;
+    return sc.tuned ? PDCS_OK : fail(P->ctx, PDCS_ERR_STATE, "no schedule chosen yet for this precision");
 }
 
 int pdcs_get_solution(pdcs_problem *P, double *x, double *y, double *s) {

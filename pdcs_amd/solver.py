@@ -154,6 +154,16 @@ class Problem:
                    self.ctx.h)
         return res.as_dict()
 
+    def set_schedule(self, precision: int, sched):
+        """Pin the SpMV row schedule (lanes G, lanes G', cap G, cap G') of a precision."""
+        a = (C.c_int32 * 4)(*[int(v) for v in sched])
+        _lib.check(self.ctx.L.pdcs_set_schedule(self.h, precision, a), self.ctx.h)
+
+    def schedule(self, precision: int):
+        a = (C.c_int32 * 4)()
+        _lib.check(self.ctx.L.pdcs_get_schedule(self.h, precision, a), self.ctx.h)
+        return list(a)
+
     KERNELS = ["primal_step", "dual_step", "cone_projection", "atdy_dot", "step_control", "reflection"]
 
     def time_kernels(self, precision: int = 0, reps: int = 50):
