@@ -1,0 +1,345 @@
+"""
+    PDCS_GPU_AMD
+
+Julia host shim over libpdcs_hip.so (include/pdcs.h).  It keeps the solve entry
+points and keyword arguments of the reference's `PDCS_GPU` module
+(`rpdhg_gpu_solve`, src/pdcs_gpu/rpdhg_alg_gpu_gen.jl:1219; `rpdhg_gpu_solve_input_gpu_data`,
+:387) and returns a `Solution`-shaped value exposing the fields the reference's
+MOI wrapper reads (src/pdcs_gpu/MOI_wrapper/MOI_wrapper.jl:648-659, 700-710):
+`sol.x.recovered_primal.primal_sol.x`, `sol.y.recovered_dual.dual_sol.y`,
+`sol.y.slack.primal_sol.x`, `sol.info.{exit_code, exit_status, pObj, dObj, time, iter}`.
+
+The MOI wrapper's `SparseMatrixCSC` (1-based CSC) is converted to the 0-based
+CSR the C ABI takes.  Beyond the reference's API it adds `batch_solve` (many
+independent programs in one persistent launch) and `conic_layer` (a solve whose
+reverse-mode derivative, `rrule` in ext/PDCS_GPU_AMDChainRulesExt.jl, comes
+from `pdcs_adjoint`).  This package is not exercised by the repository's tests
+(there is no Julia toolchain in the build image).
+"""
+module PDCS_GPU_AMD
+
+using SparseArrays
+
+const libpdcs = Ref{String}(normpath(joinpath(@__DIR__, "..", "..", "pdcs_amd", "libpdcs_hip.so")))
+
+struct PdcsCones
+    mzero::Int64
+    mnonneg::Int64
+    nsoc::Int64
+    soc::Ptr{Int64}
+    nrsoc::Int64
+    rsoc::Ptr{Int64}
+    nexp::Int64
+    ndualexp::Int64
+end
+
+mutable struct PdcsParams
+    max_outer_iter::Int64
+    max_inner_iter::Int64
+    abs_tol::Float64
+    rel_tol::Float64
+    time_limit::Float64
+    check_terminate_freq::Int64
+    duality_gap_restart_freq::Int64
+    kkt_restart_freq::Int64
+    use_kkt_restart::Int32
+    use_duality_gap_restart::Int32
+    use_preconditioner::Int32
+    verbose::Int32
+    warm_start::Int32
+    precision::Int32
+    polish::Int32
+    bisect_intent::Int32
+    code_primal_denominator::Int32
+    use_graph::Int32
+    mixed_check_freq::Int64
+    polish_check_freq::Int64
+    mixed_switch_tol::Float64
+    autotune::Int32
+    reserved::Int32
+    PdcsParams() = new()
+end
+
+mutable struct PdcsResult
+    exit_code::Int32
+    sequence::Int32
+    iter::Int64
+    iter_fp32::Int64
+    restarts::Int64
+    pobj::Float64
+    dobj::Float64
+    rel_primal_res::Float64
+    rel_dual_res::Float64
+    rel_gap::Float64
+    abs_primal_res::Float64
+    abs_dual_res::Float64
+    abs_gap::Float64
+    l2_rel_primal_res::Float64
+    l2_rel_dual_res::Float64
+    rel_primal_res_code::Float64
+    omega::Float64
+    time_setup::Float64
+    time_solve::Float64
+    time_fp32::Float64
+    time_kkt::Float64
+    time_restart::Float64
+    sched::NTuple{4, Int32}
+    PdcsResult() = new()
+end
+
+const EXIT_STATUS = Dict(0 => :optimal, 1 => :max_iter, 2 => :primal_infeasible_low_acc,
+    3 => :primal_infeasible_high_acc, 4 => :dual_infeasible_low_acc, 5 => :dual_infeasible_high_acc,
+    6 => :time_limit, 7 => :continue, 8 => :numerical_error_nan, 9 => :numerical_error_inf)
+
+function check(rc::Cint, ctx::Ptr{Cvoid})
+    if rc != 0
+        msg = unsafe_string(ccall((:pdcs_last_error, libpdcs[]), Cstring, (Ptr{Cvoid},), ctx))
+        error("PDCS_GPU_AMD: $msg (code $rc)")
+    end
+end
+
+"""CSC (1-based, Julia) -> CSR (0-based, Int32) of the same matrix."""
+function csr0(G::SparseMatrixCSC{Float64})
+    T = SparseMatrixCSC(transpose(G))          # CSC of G' is the CSR of G
+    return Int32.(T.colptr .- 1), Int32.(T.rowval .- 1), Vector{Float64}(T.nzval)
+end
+
+function rpdhg_gpu_solve(;
+    n::Integer, m::Integer, nb::Integer, c::Vector{Float64}, G::SparseMatrixCSC{Float64}, h::Vector{Float64},
+    mGzero::Integer, mGnonnegative::Integer, socG::Vector{<:Integer}, rsocG::Vector{<:Integer},
+    expG::Integer, dual_expG::Integer, bl::Vector{Float64}, bu::Vector{Float64},
+    soc_x::Vector{<:Integer}, rsoc_x::Vector{<:Integer}, exp_x::Integer = 0, dual_exp_x::Integer = 0,
+    rescaling_method::Symbol = :ruiz_pock_chambolle, use_preconditioner::Bool = true,
+    use_adaptive_restart::Bool = true, use_adaptive_step_size_weight::Bool = true,
+    use_aggressive::Bool = true, use_accelerated::Bool = false, use_resolving::Bool = true,
+    primal_sol::Vector{Float64} = zeros(n), dual_sol::Vector{Float64} = zeros(m), warm_start::Bool = false,
+    max_outer_iter::Integer = 10000, max_inner_iter::Integer = 500000, abs_tol::Float64 = 1e-6,
+    rel_tol::Float64 = 1e-6, eps_primal_infeasible_low_acc::Float64 = 1e-12,
+    eps_dual_infeasible_low_acc::Float64 = 1e-12, eps_primal_infeasible_high_acc::Float64 = 1e-16,
+    eps_dual_infeasible_high_acc::Float64 = 1e-16, print_freq::Integer = 2000,
+    check_terminate_freq::Integer = 2000, verbose::Integer = 1, time_limit::Float64 = Inf,
+    method::Symbol = :average, logfile_name = nothing, use_kkt_restart::Bool = false,
+    kkt_restart_freq::Integer = 2000, use_duality_gap_restart::Bool = true,
+    duality_gap_restart_freq::Integer = 2000, precision::Symbol = :fp64, device::Integer = 0)
+
+    (isempty(soc_x) && isempty(rsoc_x) && exp_x == 0 && dual_exp_x == 0) ||
+        throw(ArgumentError("primal-side cones are not supported"))
+    (rescaling_method == :ruiz_pock_chambolle && use_preconditioner) ||
+        throw(ArgumentError("only :ruiz_pock_chambolle with the preconditioner is supported"))
+    (use_adaptive_restart && use_adaptive_step_size_weight && use_resolving && use_aggressive &&
+        !use_accelerated && method == :average) ||
+        throw(ArgumentError("only the aggressive restarted adaptive main loop is supported"))
+
+    rowptr, colidx, vals = csr0(G)
+    soc = Vector{Int64}(socG)
+    rsoc = Vector{Int64}(rsocG)
+    ctx = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:pdcs_ctx_create, libpdcs[]), Cint, (Cint, Ptr{Ptr{Cvoid}}), device, ctx), C_NULL)
+    prob = Ref{Ptr{Cvoid}}(C_NULL)
+    res = PdcsResult()
+    x = zeros(n); y = zeros(m); s = zeros(n)
+    try
+        GC.@preserve soc rsoc begin
+            cones = Ref(PdcsCones(mGzero, mGnonnegative, length(soc), pointer(soc), length(rsoc), pointer(rsoc),
+                                  expG, dual_expG))
+            check(ccall((:pdcs_problem_create_host, libpdcs[]), Cint,
+                (Ptr{Cvoid}, Int64, Int64, Int64, Ptr{Float64}, Ptr{Int32}, Ptr{Int32}, Ptr{Float64},
+                 Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{PdcsCones}, Ptr{Ptr{Cvoid}}),
+                ctx[], n, m, nb, c, rowptr, colidx, vals, h, bl, bu, cones, prob), ctx[])
+        end
+        prm = PdcsParams()
+        ccall((:pdcs_params_default, libpdcs[]), Cvoid, (Ref{PdcsParams},), prm)
+        prm.max_outer_iter = max_outer_iter; prm.max_inner_iter = max_inner_iter
+        prm.abs_tol = abs_tol; prm.rel_tol = rel_tol; prm.time_limit = time_limit
+        prm.check_terminate_freq = check_terminate_freq
+        prm.use_kkt_restart = use_kkt_restart; prm.kkt_restart_freq = kkt_restart_freq
+        prm.use_duality_gap_restart = use_duality_gap_restart
+        prm.duality_gap_restart_freq = duality_gap_restart_freq
+        prm.verbose = verbose > 1 ? 1 : 0
+        prm.precision = precision == :mixed ? 1 : 0
+        prm.warm_start = warm_start ? 1 : 0
+        check(ccall((:pdcs_solve, libpdcs[]), Cint,
+            (Ptr{Cvoid}, Ref{PdcsParams}, Ptr{Float64}, Ptr{Float64}, Ref{PdcsResult}),
+            prob[], prm, warm_start ? primal_sol : C_NULL, warm_start ? dual_sol : C_NULL, res), ctx[])
+        check(ccall((:pdcs_get_solution, libpdcs[]), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+            prob[], x, y, s), ctx[])
+    finally
+        prob[] != C_NULL && ccall((:pdcs_problem_destroy, libpdcs[]), Cint, (Ptr{Cvoid},), prob[])
+        ccall((:pdcs_ctx_destroy, libpdcs[]), Cint, (Ptr{Cvoid},), ctx[])
+    end
+    info = (exit_code = Int(res.exit_code), exit_status = get(EXIT_STATUS, Int(res.exit_code), :unknown),
+            pObj = res.pobj, dObj = res.dobj, time = res.time_solve, iter = Int(res.iter), result = res)
+    return (x = (recovered_primal = (primal_sol = (x = x,),),),
+            y = (recovered_dual = (dual_sol = (y = y,),), slack = (primal_sol = (x = s,),)),
+            info = info)
+end
+
+# ---------------------------------------------------------------------------
+# batches of independent programs (pdcs_batch_*)
+
+struct PdcsBatchShape
+    n::Int64
+    m::Int64
+    nb::Int64
+    mzero::Int64
+    mnonneg::Int64
+    nsoc::Int64
+    nexp::Int64
+    ndualexp::Int64
+end
+
+struct PdcsBatchResult
+    exit_code::Int32
+    sequence::Int32
+    iter::Int64
+    iter_fp32::Int64
+    restarts::Int64
+    pobj::Float64
+    dobj::Float64
+    rel_primal_res::Float64
+    rel_dual_res::Float64
+    rel_gap::Float64
+    time::Float64
+end
+
+"""
+    batch_solve(problems; precision = :fp64, rel_tol = 1e-6, abs_tol = 1e-6, time_limit = Inf, device = 0)
+
+Solves independent programs in one persistent launch.  Each element of
+`problems` has fields `c, G, h, bl, bu, mzero, mnonneg, soc` (and optionally
+`nexp`, `ndualexp`; rows of G in that cone order, the first `length(bl)`
+variables boxed).  Returns `(results, xs, ys, time_total)` with one
+`PdcsBatchResult` and one unscaled solution per program.
+"""
+function batch_solve(problems; precision::Symbol = :fp64, rel_tol::Float64 = 1e-6, abs_tol::Float64 = 1e-6,
+                     time_limit::Float64 = Inf, device::Integer = 0)
+    count = length(problems)
+    shapes = PdcsBatchShape[]
+    c = Float64[]; h = Float64[]; bl = Float64[]; bu = Float64[]; vals = Float64[]
+    rowptr = Int32[]; colidx = Int32[]; soc = Int64[]
+    for P in problems
+        m, n = size(P.G)
+        rp, ci, v = csr0(P.G)
+        nexp = hasproperty(P, :nexp) ? P.nexp : 0
+        ndexp = hasproperty(P, :ndualexp) ? P.ndualexp : 0
+        push!(shapes, PdcsBatchShape(n, m, length(P.bl), P.mzero, P.mnonneg, length(P.soc), nexp, ndexp))
+        append!(c, P.c); append!(h, P.h); append!(bl, P.bl); append!(bu, P.bu)
+        append!(rowptr, rp); append!(colidx, ci); append!(vals, v); append!(soc, P.soc)
+    end
+    ctx = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:pdcs_ctx_create, libpdcs[]), Cint, (Cint, Ptr{Ptr{Cvoid}}), device, ctx), C_NULL)
+    bat = Ref{Ptr{Cvoid}}(C_NULL)
+    res = Vector{PdcsBatchResult}(undef, count)
+    total = Ref{Float64}(0.0)
+    xs = Vector{Vector{Float64}}(); ys = Vector{Vector{Float64}}()
+    try
+        check(ccall((:pdcs_batch_create_host, libpdcs[]), Cint,
+            (Ptr{Cvoid}, Int64, Ptr{PdcsBatchShape}, Ptr{Float64}, Ptr{Int32}, Ptr{Int32}, Ptr{Float64},
+             Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Int64}, Ptr{Ptr{Cvoid}}),
+            ctx[], count, shapes, c, rowptr, colidx, vals, h, bl, bu, soc, bat), ctx[])
+        prm = PdcsParams()
+        ccall((:pdcs_params_default, libpdcs[]), Cvoid, (Ref{PdcsParams},), prm)
+        prm.rel_tol = rel_tol; prm.abs_tol = abs_tol; prm.time_limit = time_limit
+        prm.precision = precision == :mixed ? 1 : 0
+        check(ccall((:pdcs_batch_solve, libpdcs[]), Cint,
+            (Ptr{Cvoid}, Ref{PdcsParams}, Ptr{PdcsBatchResult}, Ref{Float64}), bat[], prm, res, total), ctx[])
+        x = zeros(sum(s.n for s in shapes)); y = zeros(sum(s.m for s in shapes))
+        check(ccall((:pdcs_batch_get_solution, libpdcs[]), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
+            bat[], x, y), ctx[])
+        xo = 0; yo = 0
+        for s in shapes
+            push!(xs, x[xo+1:xo+s.n]); push!(ys, y[yo+1:yo+s.m])
+            xo += s.n; yo += s.m
+        end
+    finally
+        bat[] != C_NULL && ccall((:pdcs_batch_destroy, libpdcs[]), Cint, (Ptr{Cvoid},), bat[])
+        ccall((:pdcs_ctx_destroy, libpdcs[]), Cint, (Ptr{Cvoid},), ctx[])
+    end
+    return res, xs, ys, total[]
+end
+
+# ---------------------------------------------------------------------------
+# differentiable solve (pdcs_adjoint)
+
+"""A context and a GPU-resident problem kept alive for the reverse pass."""
+mutable struct ResidentProblem
+    ctx::Ptr{Cvoid}
+    prob::Ptr{Cvoid}
+    G::SparseMatrixCSC{Float64, Int}
+    function ResidentProblem(ctx, prob, G)
+        R = new(ctx, prob, G)
+        finalizer(R) do r
+            r.prob != C_NULL && ccall((:pdcs_problem_destroy, libpdcs[]), Cint, (Ptr{Cvoid},), r.prob)
+            r.ctx != C_NULL && ccall((:pdcs_ctx_destroy, libpdcs[]), Cint, (Ptr{Cvoid},), r.ctx)
+            r.prob = C_NULL; r.ctx = C_NULL
+        end
+        return R
+    end
+end
+
+"""
+    conic_layer_forward(c, G, h; bl, bu, mzero, mnonneg, soc, precision, rel_tol, abs_tol, device)
+
+Solves min c'x s.t. h - Gx in K (zero, nonnegative and SOC rows), bl <= x <= bu
+on the first `length(bl)` variables, and returns `(x, R)`; `R` keeps the
+solution on the GPU for `conic_layer_adjoint`.
+"""
+function conic_layer_forward(c::Vector{Float64}, G::SparseMatrixCSC{Float64}, h::Vector{Float64};
+                             bl::Vector{Float64}, bu::Vector{Float64}, mzero::Integer, mnonneg::Integer,
+                             soc::Vector{<:Integer}, precision::Symbol = :fp64, rel_tol::Float64 = 1e-8,
+                             abs_tol::Float64 = 1e-10, device::Integer = 0)
+    m, n = size(G)
+    rowptr, colidx, vals = csr0(G)
+    socv = Vector{Int64}(soc)
+    ctx = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:pdcs_ctx_create, libpdcs[]), Cint, (Cint, Ptr{Ptr{Cvoid}}), device, ctx), C_NULL)
+    prob = Ref{Ptr{Cvoid}}(C_NULL)
+    R = ResidentProblem(ctx[], C_NULL, SparseMatrixCSC{Float64, Int}(G))
+    GC.@preserve socv begin
+        cones = Ref(PdcsCones(mzero, mnonneg, length(socv), pointer(socv), 0, C_NULL, 0, 0))
+        check(ccall((:pdcs_problem_create_host, libpdcs[]), Cint,
+            (Ptr{Cvoid}, Int64, Int64, Int64, Ptr{Float64}, Ptr{Int32}, Ptr{Int32}, Ptr{Float64},
+             Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{PdcsCones}, Ptr{Ptr{Cvoid}}),
+            ctx[], n, m, length(bl), c, rowptr, colidx, vals, h, bl, bu, cones, prob), ctx[])
+    end
+    R.prob = prob[]
+    prm = PdcsParams()
+    ccall((:pdcs_params_default, libpdcs[]), Cvoid, (Ref{PdcsParams},), prm)
+    prm.rel_tol = rel_tol; prm.abs_tol = abs_tol
+    prm.precision = precision == :mixed ? 1 : 0
+    res = PdcsResult()
+    check(ccall((:pdcs_solve, libpdcs[]), Cint,
+        (Ptr{Cvoid}, Ref{PdcsParams}, Ptr{Float64}, Ptr{Float64}, Ref{PdcsResult}),
+        R.prob, prm, C_NULL, C_NULL, res), R.ctx)
+    res.exit_code == 0 || error("PDCS_GPU_AMD: layer solve ended with exit code $(res.exit_code)")
+    x = zeros(n); y = zeros(m); s = zeros(n)
+    check(ccall((:pdcs_get_solution, libpdcs[]), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+        R.prob, x, y, s), R.ctx)
+    return x, R
+end
+
+"""
+    conic_layer_adjoint(R, dx) -> (dc, dG, dh)
+
+Reverse-mode derivative of the layer at the solution held by `R` for the
+cotangent `dx` of x: implicit differentiation of the KKT fixed point on the GPU
+(`pdcs_adjoint`).  `dG` is sparse with the pattern of G.
+"""
+function conic_layer_adjoint(R::ResidentProblem, dx::AbstractVector{<:Real})
+    m, n = size(R.G)
+    nnz_ = nnz(R.G)
+    dxv = Vector{Float64}(dx)
+    dc = zeros(n); dh = zeros(m); dGcsr = zeros(nnz_)
+    check(ccall((:pdcs_adjoint, libpdcs[]), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+        R.prob, dxv, C_NULL, dc, dh, dGcsr), R.ctx)
+    # dGcsr follows the CSR order of G, i.e. the CSC order of G'
+    T = SparseMatrixCSC(transpose(R.G))
+    dG = SparseMatrixCSC(transpose(SparseMatrixCSC(T.m, T.n, T.colptr, T.rowval, dGcsr)))
+    return dc, dG, dh
+end
+
+"""x = conic_layer(c, G, h; kw...): the solution of the layer program (see `conic_layer_forward`)."""
+conic_layer(c, G, h; kw...) = first(conic_layer_forward(c, G, h; kw...))
+
+end # module
