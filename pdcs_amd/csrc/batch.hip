@@ -666,7 +666,8 @@ struct BSolve {
                 sigma = eta * omega;
                 const double eta_bar = (omega * dx2 + dy2 / omega) / (2.0 * fabs(dt) + 1e-20);
                 const double k1 = (double)(iter_stepsize + 1);
-                const double eta_prime = fmin((1.0 - pow(k1, -0.3)) * eta_bar, (1.0 + pow(k1, -0.6)) * eta);
+                const double p3 = exp(-0.3 * log(k1));   // k1^-0.3; k1^-0.6 = p3^2
+                const double eta_prime = fmin((1.0 - p3) * eta_bar, (1.0 + p3 * p3) * eta);
                 const bool accept = eta <= eta_bar;
                 if (!accept) count += 1;
                 const bool force = !accept && count > 1000000;

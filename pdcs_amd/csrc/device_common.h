@@ -61,10 +61,10 @@ __device__ __forceinline__ double block_sum_all(double v, double *lds) {
     return r;
 }
 
-// Block-wide reduction of N values in one LDS round (kBlock threads): entries
+// Block-wide reduction of N values in one LDS round (NT threads): entries
 // with ismax[q] take the maximum, the others the sum; results broadcast to all.
-template <int N>
-__device__ __forceinline__ void block_reduce_multi(double (&v)[N], const bool (&ismax)[N], double *lds /* 4*N */) {
+template <int N, int NT = kBlock>
+__device__ __forceinline__ void block_reduce_multi(double (&v)[N], const bool (&ismax)[N], double *lds /* NT/64*N */) {
 #pragma unroll
     for (int q = 0; q < N; ++q) v[q] = ismax[q] ? wave_max(v[q]) : wave_sum(v[q]);
     const int w = threadIdx.x >> 6;
@@ -75,7 +75,7 @@ __device__ __forceinline__ void block_reduce_multi(double (&v)[N], const bool (&
 #pragma unroll
     for (int q = 0; q < N; ++q) {
         double r = lds[q];
-        for (int i = 1; i < kBlock / 64; ++i) r = ismax[q] ? fmax(r, lds[i * N + q]) : r + lds[i * N + q];
+        for (int i = 1; i < NT / 64; ++i) r = ismax[q] ? fmax(r, lds[i * N + q]) : r + lds[i * N + q];
         v[q] = r;
     }
     __syncthreads();

@@ -22,12 +22,17 @@ struct Csr {
     int32_t grid;    // workgroup cap of the grid-stride attempt kernels (0 = kAttemptMaxGrid)
 };
 
+constexpr int kBigMaxStages = 8;
 // Per-cone scalar state of the multi-workgroup scaled-SOC root find.
 struct BigSocState {
     double s;          // head of the point being projected
     double t, tL, tR;  // root-find iterate and bracket of the projected head t
     double t_prev;     // projected head of the previous projection (warm start), NaN if none
     double t_closed;   // head for the s == 0 closed form
+    double s_in, t0_in; // head and expansion point of the last first moment pass
+    // multi-workgroup Newton stages of a robust attempt chain (ping-pong by stage parity)
+    double tn[2], tnL[2], tnR[2];
+    int32_t fin[kBigMaxStages + 1];   // fin[j]: projection finished before stage j
     int32_t cs;        // case: 0 interior, 1 polar, 2 s == 0, 3 root find
     int32_t done;
     int32_t iters;     // exact Newton steps taken by the fallback
@@ -38,7 +43,14 @@ struct BigSocState {
 };
 constexpr int kBigElems = 4;                       // tail elements per thread in a pass
 constexpr int kBigBlockElems = kBigElems * 256;    // tail elements per workgroup
-constexpr int kBigMoments = 12;                    // series terms M_0..M_{K}, K = kBigMoments - 1
+constexpr int kBigMoments = 16;                    // series terms M_0..M_{K}, K = kBigMoments - 1
+// series roots are accepted when the dropped terms are below this fraction of S:
+// the reference solves these projections to a relative tolerance of 1e-12
+// (proj_rel_tol, src/pdcs_gpu/PDCS_GPU.jl:29)
+constexpr double kBigSeriesTol = 1e-13;
+constexpr int kBigPoints = 3;                      // expansion points of the second moment pass
+constexpr int kBigStages = 8;                      // Newton stages of a robust chain
+static_assert(kBigStages <= kBigMaxStages, "BigSocState::fin holds kBigMaxStages + 1 flags");
 constexpr int kBigSums = 4 + kBigMoments + 1;      // case sums, moments, max e_i
 
 // Cone layout of the dual rows on the device.
@@ -56,8 +68,10 @@ struct ConeLayout {
     const int32_t *big_blk_first;// first tail element of each workgroup
     const int32_t *big_cone_blk0, *big_cone_nblk;
     BigSocState *big_state;      // per big cone
-    double *big_part;            // kBigSums partials per workgroup
-    unsigned int *big_count;     // arrival counters per big cone
+    double *big_part;            // kBigSums partials per workgroup, [q * nbig_blocks + block]
+    double *big_part2;           // second moment pass: [(point * kBigSums + q) * nbig_blocks + block]
+    double *big_part3;           // Newton stage sums: [(parity * 2 + q) * nbig_blocks + block]
+    int32_t big_stages;          // Newton stage kernels in this attempt chain (0: fast chain)
     int32_t nexp;                // exponential cones (code 12: y in Kexp*)
     int32_t ndexp;               // dual exponential cones (code 15: y in Kexp)
     int64_t exp_begin;           // first row of the exponential section
@@ -101,6 +115,11 @@ template <typename T>
 struct Iter {
     T *x, *xs, *xm, *xbar;   // candidate, series start, average, extrapolation
     T *y, *ys, *ym, *dy;     // candidate, series start, average, y - ys
+    // G' ys carried from attempt to attempt (FP64 in both paths): the reflection
+    // ys <- ys + a dy gives G' ys <- G' ys + a G' dy, and k_atdy computes G' dy
+    // anyway, so the primal step needs no SpMV.  Recomputed at every epoch
+    // start and residual check.
+    double *g, *gd;          // G' ys, G' dy of the last attempt
 };
 
 template <typename T>
@@ -115,8 +134,9 @@ void launch_attempt(hipStream_t s, int slot, const Csr<T> &G, const Csr<T> &Gt,
                     const ProbVec<T> &P, const ConeLayout &K, const double *d2,
                     Iter<T> it, StepState *st, double *part, int64_t part_stride);
 
-// the kernels of one attempt, individually: 0 primal (G'y + update), 1 dual (G xbar +
-// update), 2 cone projections, 3 G'dy dot, 4 step control, 5 reflection
+// the kernels of one attempt, individually: 0 primal (update from the carried G'ys),
+// 1 dual (G xbar + update), 2 cone projections, 3 G'dy (kept for the carried G'ys) and its dot, 4 step
+// control, 5 reflection (and the G'ys update)
 constexpr int kAttemptKernels = 6;
 constexpr unsigned kAttemptMaxGrid = 2048;   // default workgroup cap of the fused SpMV kernels (grid-stride beyond)
 constexpr int kTuneMaxGrid = 8192;           // largest cap the schedule autotuner tries

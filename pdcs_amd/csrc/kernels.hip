@@ -5,12 +5,14 @@
 // with every scalar kept in device memory (StepState) so attempts run back to
 // back without host synchronisation:
 //   k_primal   x = clamp(xs + tau*dec*(G'ys - c)); xbar = 2x - xs; sum (x-xs)^2
-//   k_dual     y = ys - sigma*dec*(G xbar - h); elementwise cones; sum dy^2
+//              (G'ys is carried, not recomputed: see Iter::g)
+//   k_dual     y = ys - sigma*dec*(G xbar - h); elementwise cones; sum dy^2;
+//              with one large SOC also that cone's first moment pass
 //   k_soc_*    scaled-SOC projections of the cone rows; dy; sum dy^2
-//   k_atdy     sum (x - xs) . (G' dy)
+//   k_atdy     G'dy (kept) and sum (x - xs) . (G' dy)
 //   k_step     eta_bar, eta', accept/reject, halt decisions (one workgroup)
 //   k_reflect  aggressive reflection + eta-weighted average (reflection_update,
-//              src/pdcs_gpu/cuda/utils.cu:38-57)
+//              src/pdcs_gpu/cuda/utils.cu:38-57); G'ys += a G'dy
 // Every kernel of attempt slot `slot` returns immediately once an earlier
 // k_step has set stop_at <= slot, so a pre-recorded chunk of attempts drains
 // to no-ops after a halt.
@@ -44,30 +46,160 @@ __device__ __forceinline__ bool halted(const StepState *st, int slot) {
     return slot >= __builtin_nontemporal_load(&st->stop_at);
 }
 
+// ------------------------------------------------ large scaled SOCs --
+// A cone longer than 1024 rows is split over workgroups of kBigBlockElems
+// tail rows.  k_bigsoc_moments computes, per workgroup, the case sums
+// (||d x||^2, ||x/d||^2, ||x||^2, the s == 0 closed form) and the moments of
+// S around the previous projection's head t0:
+//   w_i(t0 + h) = w_i0 (1 + h e_i),  w_i0 = t0 (1 + d_i^2) - s d_i^2,  e_i = (1 + d_i^2) / w_i0
+//   S(t0 + h) = sum_k (k+1) (-h)^k M_k,   M_k = sum_i a_i e_i^k / w_i0^2,
+// and max_i e_i.  k_bigsoc_apply: every workgroup reduces its cone's partials
+// in the same fixed order and solves phi = 0 on the truncated series
+// (safeguarded Newton, scalar), so all of them reach the same decision without
+// a launch of their own; with K = kBigMoments terms the relative truncation
+// error of S is at most (K+1) r^K / (1-r)^2, r = |h| max e_i, and the root is
+// accepted below kBigSeriesTol.  Accepted (the common case), each workgroup
+// writes z = x / (1 + lambda d^2) on its block.  Otherwise (no warm start, or
+// a far move such as the first attempts after a restart) each workgroup
+// computes its block's moments at kBigPoints expansion points (the clipped
+// series root and two points beyond it towards the far end of the bracket);
+// k_bigsoc_rare then decides the same way from those and projects.  A cone
+// none of whose series is accepted is finished by one workgroup (exact Newton
+// steps, then the whole cone) in the fast chain; a robust chain (chosen by the
+// host while such cones occur, mostly in the first attempts of an epoch)
+// follows with K.big_stages multi-workgroup Newton stages (k_bigsoc_stage),
+// each a no-op once the cone is projected.
+
+__device__ __forceinline__ void big_newton_update(BigSocState &S, double Ssum, double S1sum) {
+    bool done = false;
+    S.t = soc_newton_step(S.t, Ssum, S1sum, S.tL, S.tR, done);
+    S.iters += 1;
+    S.n_newton += 1;
+    S.done = done ? 1 : 0;
+}
+
+// Series solve of phi(t0 + h) = 0 from the moments; returns false if the
+// truncation bound is not met (the caller then falls back to exact Newton).
+__device__ __forceinline__ bool big_series_solve(BigSocState &L, double t0, const double *M, double emax) {
+    double c[kBigMoments];
+    for (int k = 0; k < kBigMoments; ++k) c[k] = (double)(k + 1) * ((k & 1) ? -M[k] : M[k]);
+    const double rad = 0.5 / emax;
+    double hL = fmax(L.tL - t0, -rad), hR = fmin(L.tR - t0, rad);
+    if (!(hL < hR)) return false;
+    double h = fmin(fmax(0.0, hL), hR);
+    bool ok = false;
+    for (int it = 0; it < 60; ++it) {
+        double S = c[kBigMoments - 1], dS = 0.0;
+        for (int k = kBigMoments - 2; k >= 0; --k) { dS = dS * h + S; S = S * h + c[k]; }
+        if (!(S > 0.0)) break;
+        const double isq = 1.0 / sqrt(S);
+        const double phi = isq - 1.0;
+        if (phi == 0.0) { ok = true; break; }
+        if (phi < 0.0) hL = fmax(hL, h); else hR = fmin(hR, h);
+        const double dphi = -0.5 * dS * isq * isq * isq;
+        double hn = h - phi / dphi;
+        bool newton = true;
+        if (!(dphi > 0.0) || !(hn >= hL && hn <= hR)) { hn = 0.5 * (hL + hR); newton = false; }
+        const bool conv = (newton && fabs(hn - h) <= 1e-15 * fabs(t0 + hn)) || (hR - hL) <= 4.5e-16 * fabs(t0 + hR);
+        h = hn;
+        if (conv) { ok = true; break; }
+    }
+    // the series root (clipped to the bracket and the convergence radius) is the
+    // expansion point of the next pass even when it is not accepted
+    L.t = fmin(fmax(t0 + h, L.tL), L.tR);
+    if (!ok) return false;
+    const double r = fabs(h) * emax;
+    if (!(r < 0.5)) return false;
+    // remainder of the dropped terms k >= kBigMoments: sum (k+1) r^k <= (kBigMoments+1) r^kBigMoments / (1-r)^2
+    double rk = 1.0;
+    for (int k = 0; k < kBigMoments; ++k) rk *= r;
+    if ((double)(kBigMoments + 1) * rk / ((1.0 - r) * (1.0 - r)) > kBigSeriesTol) return false;
+    L.t = t0 + h;
+    return true;
+}
+
+// Per-element terms of a moment pass: case sums (first pass only) and the
+// moments of S around t0 (skipped when t0 is NaN).
+__device__ __forceinline__ void big_accumulate(double (&v)[kBigSums], double xv, double dd, double s, double t0,
+                                               bool first) {
+    const double a = dd * xv * xv;
+    if (first) {
+        v[0] += a;
+        v[1] += xv * xv / dd;
+        v[2] += xv * xv;
+        const double qv = xv / (1.0 + dd);
+        v[3] += dd * qv * qv;
+    }
+    if (t0 == t0) {
+        const double bb = 1.0 + dd;
+        const double iw = 1.0 / (t0 * bb - s * dd);
+        const double ei = bb * iw;
+        double term = a * iw * iw;
+#pragma unroll
+        for (int m = 0; m < kBigMoments; ++m) { v[4 + m] += term; term *= ei; }
+        v[4 + kBigMoments] = fmax(v[4 + kBigMoments], ei);
+    }
+}
+__device__ __forceinline__ void big_init(double (&v)[kBigSums], bool (&ismax)[kBigSums]) {
+#pragma unroll
+    for (int q = 0; q < kBigSums; ++q) { v[q] = 0.0; ismax[q] = (q == kBigSums - 1); }
+}
+// warm start of a first pass: the previous projection's head when it lies
+// strictly right of max(s, 0), else none (NaN)
+__device__ __forceinline__ double big_t0(const BigSocState *S, double s) {
+    const double tp = S->t_prev;
+    return (isfinite(tp) && tp > fmax(s, 0.0)) ? tp : NAN;
+}
+
+// Case decision (first pass) and series solve of a cone's projection from the
+// reduced sums `tot` of a moment pass; updates L.  Scalar code run by one
+// thread (inline: an out-of-line call keeps L in scratch memory).
+__device__ __forceinline__ void big_decide(BigSocState &L, const double *tot, double s, double t0, bool refine) {
+    if (!refine) {
+        L.s = s;
+        L.iters = 0;
+        L.series = 0;
+        L.done = 1;
+        const SocSums z = {tot[0], tot[1], tot[2]};
+        L.cs = soc_case(s, z);
+        if (L.cs == 2) L.t_closed = sqrt(tot[3]);
+        if (L.cs == 3) {
+            L.tL = fmax(s, 0.0);
+            L.tR = sqrt(s * s + tot[2]);
+            L.done = 0;
+            L.t = L.tL;
+            L.n_root += 1;
+        }
+    }
+    if (L.cs == 3 && !L.done && t0 == t0 && t0 < L.tR && tot[4] > 0.0) {
+        if (big_series_solve(L, t0, tot + 4, tot[4 + kBigMoments])) {
+            L.done = 1;
+            L.series = 1;
+            if (refine) L.n_refine += 1; else L.n_first += 1;
+        }
+    }
+}
+
 // ------------------------------------------------------------- attempt --
-template <typename T, int L>
-__global__ __launch_bounds__(kBlock) void k_primal(Csr<T> Gt, ProbVec<T> P, const T *__restrict__ ys,
+// Elementwise: the SpMV G'ys of the primal update is the carried vector g.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_primal(ProbVec<T> P, const double *__restrict__ g,
                                                    const T *__restrict__ xs, T *__restrict__ x,
                                                    T *__restrict__ xbar, StepState *st, int slot,
                                                    double *__restrict__ part) {
     __shared__ double lds[kBlock / 64];
     if (halted(st, slot)) return;
     const double ps = st->eta / st->omega * st->dec;   // primal_step_size, scaling.jl:687-689
-    const int lane = (int)(threadIdx.x % L);
     double d2 = 0.0;
-    for (int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / L; row < P.n;
-         row += (int64_t)gridDim.x * (kBlock / L)) {
-    const double acc = row_dot<T, L>(Gt, row, lane, ys);
-    if (lane == 0) {
+    for (int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x; row < P.n; row += (int64_t)gridDim.x * kBlock) {
         const double xl = (double)xs[row];
-        double v = xl + ps * (acc - (double)P.c[row]);            // primal_update, utils.cu:8
+        double v = xl + ps * (g[row] - (double)P.c[row]);             // primal_update, utils.cu:8
         if (row < P.nb) v = fmin(fmax(v, (double)P.bl[row]), (double)P.bu[row]);  // code 18
         const T xv = (T)v;
         x[row] = xv;
         xbar[row] = (T)(2.0 * (double)xv - xl);                   // extrapolation_update, utils.cu:18
         const double d = (double)xv - xl;
         d2 += d * d;
-    }
     }
     d2 = block_sum(d2, lds);
     if (threadIdx.x == 0) part[blockIdx.x] = d2;
@@ -178,263 +310,84 @@ __global__ __launch_bounds__(kBlock) void k_soc_group(ConeLayout K, const double
     }
 }
 
-// ------------------------------------------------ large scaled SOCs --
-// A cone longer than 1024 rows is split over workgroups of kBigBlockElems
-// tail rows.  One pass (k_bigsoc_init) computes, per workgroup, the case sums
-// (||d x||^2, ||x/d||^2, ||x||^2, the s == 0 closed form) and the moments of
-// S around the previous projection's head t0:
-//   w_i(t0 + h) = w_i0 (1 + h e_i),  w_i0 = t0 (1 + d_i^2) - s d_i^2,  e_i = (1 + d_i^2) / w_i0
-//   S(t0 + h) = sum_k (k+1) (-h)^k M_k,   M_k = sum_i a_i e_i^k / w_i0^2,
-// and max_i e_i.  The last workgroup to arrive reduces the partials in fixed
-// order and solves phi = 0 on the truncated series (safeguarded Newton, all
-// scalar).  With K = kBigMoments terms the relative truncation error of S is
-// at most (K+1) r^K / (1-r)^2, r = |h| max e_i; below 1e-16 the root is accepted.  Otherwise (no warm
-// start, or a far move such as the first attempt after a restart) the
-// single-workgroup k_bigsoc_finish runs exact Newton steps to convergence.
-// k_bigsoc_apply then writes z = x / (1 + lambda d^2) and the head.
-
-// Last-arrival detection.  Thread 0 wrote this workgroup's partials; its
-// agent-scope acq_rel increment publishes them (release) and, in the last
-// workgroup, makes every other workgroup's partials visible (acquire; the
-// barrier then extends that to the whole workgroup).  Measured on MI355X this
-// costs ~4 us per pass against ~13 us for __threadfence(), which writes back L2.
-__device__ __forceinline__ bool big_last_arrival(unsigned int *count, unsigned int nblk, bool *lds_flag) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned int prev = __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        *lds_flag = (prev == nblk - 1);
-    }
-    __syncthreads();
-    return *lds_flag;
-}
-
-__device__ __forceinline__ void big_newton_update(BigSocState &S, double Ssum, double S1sum) {
-    bool done = false;
-    S.t = soc_newton_step(S.t, Ssum, S1sum, S.tL, S.tR, done);
-    S.iters += 1;
-    S.n_newton += 1;
-    S.done = done ? 1 : 0;
-}
-
-// Series solve of phi(t0 + h) = 0 from the moments; returns false if the
-// truncation bound is not met (the caller then falls back to exact Newton).
-__device__ __forceinline__ bool big_series_solve(BigSocState &L, double t0, const double *M, double emax) {
-    double c[kBigMoments];
-    for (int k = 0; k < kBigMoments; ++k) c[k] = (double)(k + 1) * ((k & 1) ? -M[k] : M[k]);
-    const double rad = 0.5 / emax;
-    double hL = fmax(L.tL - t0, -rad), hR = fmin(L.tR - t0, rad);
-    if (!(hL < hR)) return false;
-    double h = fmin(fmax(0.0, hL), hR);
-    bool ok = false;
-    for (int it = 0; it < 60; ++it) {
-        double S = c[kBigMoments - 1], dS = 0.0;
-        for (int k = kBigMoments - 2; k >= 0; --k) { dS = dS * h + S; S = S * h + c[k]; }
-        if (!(S > 0.0)) break;
-        const double isq = 1.0 / sqrt(S);
-        const double phi = isq - 1.0;
-        if (phi == 0.0) { ok = true; break; }
-        if (phi < 0.0) hL = fmax(hL, h); else hR = fmin(hR, h);
-        const double dphi = -0.5 * dS * isq * isq * isq;
-        double hn = h - phi / dphi;
-        bool newton = true;
-        if (!(dphi > 0.0) || !(hn >= hL && hn <= hR)) { hn = 0.5 * (hL + hR); newton = false; }
-        const bool conv = (newton && fabs(hn - h) <= 1e-15 * fabs(t0 + hn)) || (hR - hL) <= 4.5e-16 * fabs(t0 + hR);
-        h = hn;
-        if (conv) { ok = true; break; }
-    }
-    // the series root (clipped to the bracket and the convergence radius) is the
-    // expansion point of the next pass even when it is not accepted
-    L.t = fmin(fmax(t0 + h, L.tL), L.tR);
-    if (!ok) return false;
-    const double r = fabs(h) * emax;
-    if (!(r < 0.5)) return false;
-    // remainder of the dropped terms k >= kBigMoments: sum (k+1) r^k <= (kBigMoments+1) r^kBigMoments / (1-r)^2
-    double rk = 1.0;
-    for (int k = 0; k < kBigMoments; ++k) rk *= r;
-    if ((double)(kBigMoments + 1) * rk / ((1.0 - r) * (1.0 - r)) > 1e-16) return false;
-    L.t = t0 + h;
-    return true;
-}
-
-// Per-element terms of a moment pass: case sums (first pass only) and the
-// moments of S around t0 (skipped when t0 is NaN).
-__device__ __forceinline__ void big_accumulate(double (&v)[kBigSums], double xv, double dd, double s, double t0,
-                                               bool first) {
-    const double a = dd * xv * xv;
-    if (first) {
-        v[0] += a;
-        v[1] += xv * xv / dd;
-        v[2] += xv * xv;
-        const double qv = xv / (1.0 + dd);
-        v[3] += dd * qv * qv;
-    }
-    if (t0 == t0) {
-        const double bb = 1.0 + dd;
-        const double iw = 1.0 / (t0 * bb - s * dd);
-        const double ei = bb * iw;
-        double term = a * iw * iw;
-#pragma unroll
-        for (int m = 0; m < kBigMoments; ++m) { v[4 + m] += term; term *= ei; }
-        v[4 + kBigMoments] = fmax(v[4 + kBigMoments], ei);
-    }
-}
-__device__ __forceinline__ void big_init(double (&v)[kBigSums], bool (&ismax)[kBigSums]) {
-#pragma unroll
-    for (int q = 0; q < kBigSums; ++q) { v[q] = 0.0; ismax[q] = (q == kBigSums - 1); }
-}
-// warm start of a first pass: the previous projection's head when it lies
-// strictly right of max(s, 0), else none (NaN)
-__device__ __forceinline__ double big_t0(const BigSocState *S, double s) {
-    const double tp = S->t_prev;
-    return (isfinite(tp) && tp > fmax(s, 0.0)) ? tp : NAN;
-}
-
-// Case decision (first pass) and series solve of a cone's projection from the
-// reduced sums `tot` of a moment pass; updates L.
-__device__ __forceinline__ void big_decide(BigSocState &L, const double *tot, double s, double t0, bool refine) {
-    if (!refine) {
-        L.s = s;
-        L.iters = 0;
-        L.series = 0;
-        L.done = 1;
-        const SocSums z = {tot[0], tot[1], tot[2]};
-        L.cs = soc_case(s, z);
-        if (L.cs == 2) L.t_closed = sqrt(tot[3]);
-        if (L.cs == 3) {
-            L.tL = fmax(s, 0.0);
-            L.tR = sqrt(s * s + tot[2]);
-            L.done = 0;
-            L.t = L.tL;
-            L.n_root += 1;
-        }
-    }
-    if (L.cs == 3 && !L.done && t0 == t0 && t0 < L.tR && tot[4] > 0.0) {
-        if (big_series_solve(L, t0, tot + 4, tot[4 + kBigMoments])) {
-            L.done = 1;
-            L.series = 1;
-            if (refine) L.n_refine += 1; else L.n_first += 1;
-        }
-    }
-}
-
-// Block-reduced sums `v` of this workgroup go to parts[slot]; the last of the
-// `np` workgroups of `cone` to arrive reduces parts[0..np) in fixed order and
-// decides (big_decide).
-__device__ __forceinline__ void big_conclude(const ConeLayout &K, int cone, double (&v)[kBigSums], const bool (&ismax)[kBigSums],
-                             double *parts, int slot, int np, double s, double t0, bool refine, double *lds,
-                             bool *flag) {
-    block_reduce_multi<kBigSums>(v, ismax, lds);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int q = 0; q < kBigSums; ++q) parts[(int64_t)slot * kBigSums + q] = v[q];
-    if (!big_last_arrival(K.big_count + cone, (unsigned)np, flag)) return;
-    double tot[kBigSums];
-#pragma unroll
-    for (int q = 0; q < kBigSums; ++q) tot[q] = 0.0;
-    for (int j = threadIdx.x; j < np; j += kBlock) {
-#pragma unroll
-        for (int q = 0; q < kBigSums; ++q) {
-            const double pv = __builtin_nontemporal_load(&parts[(int64_t)j * kBigSums + q]);
-            tot[q] = ismax[q] ? fmax(tot[q], pv) : tot[q] + pv;
-        }
-    }
-    block_reduce_multi<kBigSums>(tot, ismax, lds);
-    if (threadIdx.x != 0) return;
-    K.big_count[cone] = 0u;
-    BigSocState *S = K.big_state + cone;
-    BigSocState L = *S;
-    big_decide(L, tot, s, t0, refine);
-    *S = L;
-}
-
-// One moment pass over the big cones.  refine == 0: case sums and
-// moments at the previous head t_prev; refine == 1: moments at the current
-// iterate L.t of a projection whose series solve was not accepted (no-op
-// once done).
+// First moment pass over the big cones at the warm-start head t_prev (case
+// sums and moments); kBigSums partials per workgroup, no decision.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bigsoc_pass(ConeLayout K, const double *__restrict__ d2,
-                                                        const T *__restrict__ y, StepState *st, int slot, int refine) {
+__global__ __launch_bounds__(kBlock) void k_bigsoc_moments(ConeLayout K, const double *__restrict__ d2,
+                                                           const T *__restrict__ y, StepState *st, int slot) {
     __shared__ double lds[(kBlock / 64) * kBigSums];
-    __shared__ bool flag;
     if (st != nullptr && halted(st, slot)) return;
     const int b = blockIdx.x;
     const int cone = K.big_blk_cone[b];
-    BigSocState *S = K.big_state + cone;
-    if (refine && S->done) return;
     const int64_t st0 = K.soc_big_start[cone];
     const int k = K.soc_big_dim[cone] - 1;
-    const double s = refine ? S->s : (double)y[st0];
-    const double t0 = refine ? S->t : big_t0(S, s);
+    const double s = (double)y[st0];
+    const double t0 = big_t0(K.big_state + cone, s);
     double v[kBigSums];
     bool ismax[kBigSums];
     big_init(v, ismax);
     const int e0 = K.big_blk_first[b];
     for (int q = 0; q < kBigElems; ++q) {
         const int e = e0 + q * kBlock + threadIdx.x;
-        if (e < k) big_accumulate(v, (double)y[st0 + 1 + e], d2[st0 + 1 + e], s, t0, !refine);
+        if (e < k) big_accumulate(v, (double)y[st0 + 1 + e], d2[st0 + 1 + e], s, t0, true);
     }
-    const int b0 = K.big_cone_blk0[cone];
-    big_conclude(K, cone, v, ismax, K.big_part + (int64_t)b0 * kBigSums, b - b0, K.big_cone_nblk[cone], s, t0,
-                 refine != 0, lds, &flag);
-}
-
-// Fallback: one workgroup finishes the exact Newton iteration (no-op when done).
-template <typename T>
-__global__ __launch_bounds__(1024) void k_bigsoc_finish(ConeLayout K, const double *__restrict__ d2,
-                                                        const T *__restrict__ y, StepState *st, int slot) {
-    __shared__ double lds[1024 / 64];
-    __shared__ BigSocState L;
-    if (st != nullptr && halted(st, slot)) return;
-    const int cone = blockIdx.x;
-    BigSocState *S = K.big_state + cone;
-    if (S->done) return;
-    const int64_t st0 = K.soc_big_start[cone];
-    const int k = K.soc_big_dim[cone] - 1;
+    block_reduce_multi<kBigSums>(v, ismax, lds);
     if (threadIdx.x == 0) {
-        L = *S;
-        L.n_fallback += 1;
+#pragma unroll
+        for (int q = 0; q < kBigSums; ++q) K.big_part[(int64_t)q * K.nbig_blocks + b] = v[q];
+        // the point the partials belong to, for k_bigsoc_apply (which may change y[st0] and t_prev)
+        if (b == K.big_cone_blk0[cone]) {
+            K.big_state[cone].s_in = s;
+            K.big_state[cone].t0_in = t0;
+        }
     }
-    __syncthreads();
-    const double s = L.s;
-    for (int it = 0; it < 200; ++it) {
-        if (L.done) break;
-        double A = 0.0, C = 0.0;
-        const double tt = L.t;
-        for (int e = threadIdx.x; e < k; e += 1024) soc_newton_terms(tt, s, (double)y[st0 + 1 + e], d2[st0 + 1 + e], A, C);
-        A = block_sum(A, lds);
-        C = block_sum(C, lds);
-        if (threadIdx.x == 0) big_newton_update(L, A, C);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *S = L;
 }
 
+constexpr int kDecideUnroll = 8;      // tail elements loaded per thread before use (loads in flight)
+
+// sum over the cone tail of f(y_i, d_i^2) by one workgroup of kBlock threads,
+// kDecideUnroll loads in flight per thread
+template <typename T, typename F>
+__device__ __forceinline__ void big_tail_loop(const T *__restrict__ y, const double *__restrict__ d2, int64_t st0,
+                                              int k, F f) {
+    for (int e0 = (int)threadIdx.x; e0 < k; e0 += kBlock * kDecideUnroll) {
+        double yv[kDecideUnroll], dv[kDecideUnroll];
+#pragma unroll
+        for (int u = 0; u < kDecideUnroll; ++u) {
+            const int e = e0 + u * kBlock;
+            yv[u] = e < k ? (double)y[st0 + 1 + e] : 0.0;
+            dv[u] = e < k ? d2[st0 + 1 + e] : 1.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kDecideUnroll; ++u)
+            if (e0 + u * kBlock < k) f(yv[u], dv[u]);
+    }
+}
+
+// Expansion points of the second moment pass: the clipped series root tr of
+// the first pass and two points beyond it towards the end of the bracket the
+// first pass moved to (the right end without a warm start).
+__device__ __forceinline__ double big_point(int p, double tr, double t0, double tL, double tR) {
+    const double far = (!(t0 == t0) || tr >= t0) ? tR : tL;
+    return p == 0 ? tr : tr + (p == 1 ? 0.3 : 0.7) * (far - tr);
+}
+
+// projection of this workgroup's block of a big cone with multiplier lam
+// (case cs); the head when the block is the cone's first; returns sum dy^2
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bigsoc_apply(ConeLayout K, const double *__restrict__ d2,
-                                                         T *__restrict__ y, const T *__restrict__ ys,
-                                                         T *__restrict__ dy, StepState *st, int slot,
-                                                         double *__restrict__ part) {
-    __shared__ double lds[kBlock / 64];
-    if (st != nullptr && halted(st, slot)) return;
-    const int b = blockIdx.x;
-    const int cone = K.big_blk_cone[b];
-    BigSocState *S = K.big_state + cone;
-    const int64_t st0 = K.soc_big_start[cone];
-    const int k = K.soc_big_dim[cone] - 1;
-    const int cs = S->cs;
-    double lam = 0.0, th = S->s;
-    if (cs == 2) { lam = 1.0; th = S->t_closed; }
-    if (cs == 3) { lam = 1.0 - S->s / S->t; th = S->t; }
+__device__ __forceinline__ double big_apply_block(const ConeLayout &K, const double *__restrict__ d2,
+                                                  T *__restrict__ y, const T *__restrict__ ys,
+                                                  T *__restrict__ dy, int64_t st0, int k, int e0, bool head,
+                                                  int cs, double lam, double th) {
     double my2 = 0.0;
-    const int e0 = K.big_blk_first[b];
     for (int q = 0; q < kBigElems; ++q) {
         const int e = e0 + q * kBlock + threadIdx.x;
         if (e < k) {
             const int64_t i = st0 + 1 + e;
-            double v = (double)y[i];
-            if (cs == 1) v = 0.0;
-            else if (cs >= 2) v = v / (1.0 + lam * d2[i]);
-            const T vt = (T)v;
+            double xv = (double)y[i];
+            if (cs == 1) xv = 0.0;
+            else if (cs >= 2) xv = xv / (1.0 + lam * d2[i]);
+            const T vt = (T)xv;
             y[i] = vt;
             if (dy != nullptr) {
                 const double d = (double)vt - (double)ys[i];
@@ -443,22 +396,336 @@ __global__ __launch_bounds__(kBlock) void k_bigsoc_apply(ConeLayout K, const dou
             }
         }
     }
-    if (b == K.big_cone_blk0[cone] && threadIdx.x == 0) {
-        const double hv = cs == 1 ? 0.0 : th;
-        const T ht = (T)hv;
+    if (head && threadIdx.x == 0) {
+        const T ht = (T)(cs == 1 ? 0.0 : th);
         y[st0] = ht;
         if (dy != nullptr) {
             const double d = (double)ht - (double)ys[st0];
             dy[st0] = (T)d;
             my2 += d * d;
         }
-        S->t_prev = (cs == 1) ? NAN : th;
+    }
+    return my2;
+}
+
+// Decision and projection of the big cones.  Each workgroup decides for itself
+// from s_in / t0_in and the first-pass partials (read-only here); only the
+// workgroup holding the head writes the cone state.  dy and sum dy^2 when `dy`
+// is given; part[b] is this workgroup's sum (0 when the cone is left to
+// k_bigsoc_rare).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bigsoc_apply(ConeLayout K, const double *__restrict__ d2,
+                                                         T *__restrict__ y, const T *__restrict__ ys,
+                                                         T *__restrict__ dy, StepState *st, int slot,
+                                                         double *__restrict__ part) {
+    __shared__ double lds[(kBlock / 64) * kBigSums];
+    __shared__ double sh_t, sh_lam, sh_th, sh_tL, sh_tR;
+    __shared__ int sh_cs, sh_done;
+    if (st != nullptr && halted(st, slot)) return;
+    const int b = blockIdx.x;
+    const int cone = K.big_blk_cone[b];
+    BigSocState *S = K.big_state + cone;
+    const int64_t st0 = K.soc_big_start[cone];
+    const int k = K.soc_big_dim[cone] - 1;
+    const int b0 = K.big_cone_blk0[cone], np = K.big_cone_nblk[cone];
+    const double s = S->s_in, t0 = S->t0_in;
+    double v[kBigSums];
+    bool ismax[kBigSums];
+    big_init(v, ismax);
+    for (int j = threadIdx.x; j < np; j += kBlock) {
+#pragma unroll
+        for (int q = 0; q < kBigSums; ++q) {
+            const double pv = K.big_part[(int64_t)q * K.nbig_blocks + b0 + j];
+            v[q] = ismax[q] ? fmax(v[q], pv) : v[q] + pv;
+        }
+    }
+    block_reduce_multi<kBigSums>(v, ismax, lds);
+    if (threadIdx.x == 0) {
+        BigSocState L{};
+        L.t_prev = NAN;
+        big_decide(L, v, s, t0, false);
+        double lam = 0.0, th = s;
+        if (L.cs == 2) { lam = 1.0; th = L.t_closed; }
+        if (L.cs == 3) { lam = 1.0 - s / L.t; th = L.t; }
+        sh_cs = L.cs;
+        sh_done = L.done;
+        sh_t = L.t;
+        sh_tL = L.tL;
+        sh_tR = L.tR;
+        sh_lam = lam;
+        sh_th = th;
+        if (b == b0) {
+            S->s = s;
+            S->cs = L.cs;
+            S->done = L.done;
+            S->series = L.series;
+            S->iters = 0;
+            S->t = L.t;
+            S->tL = L.tL;
+            S->tR = L.tR;
+            S->t_closed = L.t_closed;
+            S->n_root += L.n_root;
+            S->n_first += L.n_first;
+            if (L.done) S->t_prev = (L.cs == 1) ? NAN : th;
+        }
+    }
+    __syncthreads();
+    const int e0 = K.big_blk_first[b];
+    double my2 = 0.0;
+    if (sh_done) {
+        my2 = big_apply_block(K, d2, y, ys, dy, st0, k, e0, b == b0, sh_cs, sh_lam, sh_th);
+    } else {
+        // moments of this block at the second-pass expansion points for k_bigsoc_rare
+#pragma unroll 1
+        for (int p = 0; p < kBigPoints; ++p) {
+            const double tp = big_point(p, sh_t, t0, sh_tL, sh_tR);
+            big_init(v, ismax);
+            for (int q = 0; q < kBigElems; ++q) {
+                const int e = e0 + q * kBlock + threadIdx.x;
+                if (e < k) big_accumulate(v, (double)y[st0 + 1 + e], d2[st0 + 1 + e], s, tp, false);
+            }
+            block_reduce_multi<kBigSums>(v, ismax, lds);
+            if (threadIdx.x == 0)
+#pragma unroll
+                for (int q = 0; q < kBigSums; ++q)
+                    K.big_part2[((int64_t)p * kBigSums + q) * K.nbig_blocks + b] = v[q];
+        }
     }
     if (part != nullptr) {
         my2 = block_sum(my2, lds);
         if (threadIdx.x == 0) part[b] = my2;
     }
 }
+
+// One workgroup finishes a cone: exact Newton steps from t_start, then the
+// projection of the whole cone (its sum dy^2 into the first block's slot).
+template <typename T>
+__device__ void big_fallback(const ConeLayout &K, BigSocState *S, BigSocState &L, const double *__restrict__ d2,
+                             T *__restrict__ y, const T *__restrict__ ys, T *__restrict__ dy, int64_t st0, int k,
+                             double s, double t_start, int b0, double *__restrict__ part, double *lds) {
+    if (threadIdx.x == 0) {
+        L = *S;
+        L.t = t_start;
+        L.iters = 0;
+    }
+    __syncthreads();
+    for (int it = 0; it < 200; ++it) {
+        if (L.done) break;
+        double A = 0.0, C = 0.0;
+        const double tt = L.t;
+        big_tail_loop(y, d2, st0, k, [&](double yv, double dv) { soc_newton_terms(tt, s, yv, dv, A, C); });
+        A = block_sum(A, lds);
+        C = block_sum(C, lds);
+        if (threadIdx.x == 0) big_newton_update(L, A, C);
+        __syncthreads();
+    }
+    const double th = L.t, lam = 1.0 - s / th;
+    double my2 = 0.0;
+    for (int e = threadIdx.x; e < k; e += kBlock) {
+        const int64_t i = st0 + 1 + e;
+        const T vt = (T)((double)y[i] / (1.0 + lam * d2[i]));
+        y[i] = vt;
+        if (dy != nullptr) {
+            const double d = (double)vt - (double)ys[i];
+            dy[i] = (T)d;
+            my2 += d * d;
+        }
+    }
+    if (threadIdx.x == 0) {
+        const T ht = (T)th;
+        y[st0] = ht;
+        if (dy != nullptr) {
+            const double d = (double)ht - (double)ys[st0];
+            dy[st0] = (T)d;
+            my2 += d * d;
+        }
+    }
+    my2 = block_sum(my2, lds);
+    if (threadIdx.x == 0) {
+        if (part != nullptr) part[b0] = my2;
+        S->t_prev = th;
+        S->n_fallback += 1;
+        S->n_newton += L.iters;
+    }
+}
+
+// Newton sums (S, S') of this workgroup's block at t into big_part3 slot `par`
+template <typename T>
+__device__ double big_stage_sums(const ConeLayout &K, const double *__restrict__ d2, const T *__restrict__ y,
+                                 int64_t st0, int k, int e0, double s, double t, int b, int par, double *lds) {
+    double A = 0.0, C = 0.0;
+    for (int q = 0; q < kBigElems; ++q) {
+        const int e = e0 + q * kBlock + threadIdx.x;
+        if (e < k) soc_newton_terms(t, s, (double)y[st0 + 1 + e], d2[st0 + 1 + e], A, C);
+    }
+    A = block_sum(A, lds);
+    C = block_sum(C, lds);
+    if (threadIdx.x == 0) {
+        K.big_part3[((int64_t)par * 2 + 0) * K.nbig_blocks + b] = A;
+        K.big_part3[((int64_t)par * 2 + 1) * K.nbig_blocks + b] = C;
+    }
+    return A;
+}
+
+// Big cones whose first-pass series was not accepted (no-op otherwise).  Each
+// workgroup reduces the cone's second-pass partials point by point in fixed
+// order and takes the first accepted series root; all reach the same one and
+// project their own blocks.  If none is accepted the cone's first workgroup
+// runs exact Newton steps from the clipped root and projects the whole cone.
+// Reads only what k_bigsoc_apply wrote (done, s, t, tL, tR) and t0_in; writes
+// t_prev and the counters.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bigsoc_rare(ConeLayout K, const double *__restrict__ d2,
+                                                        T *__restrict__ y, const T *__restrict__ ys,
+                                                        T *__restrict__ dy, StepState *st, int slot,
+                                                        double *__restrict__ part) {
+    __shared__ double lds[(kBlock / 64) * kBigSums];
+    __shared__ double sh_th;
+    __shared__ int sh_acc;
+    __shared__ BigSocState L;
+    if (st != nullptr && halted(st, slot)) return;
+    const int b = blockIdx.x;
+    const int cone = K.big_blk_cone[b];
+    BigSocState *S = K.big_state + cone;
+    if (S->done) return;
+    const int64_t st0 = K.soc_big_start[cone];
+    const int k = K.soc_big_dim[cone] - 1;
+    const int b0 = K.big_cone_blk0[cone], np = K.big_cone_nblk[cone];
+    const double s = S->s, tr = S->t, tL = S->tL, tR = S->tR, t0 = S->t0_in;
+    if (threadIdx.x == 0) sh_acc = -1;
+    double v[kBigSums];
+    bool ismax[kBigSums];
+#pragma unroll 1
+    for (int p = 0; p < kBigPoints; ++p) {
+        const double tp = big_point(p, tr, t0, tL, tR);
+        big_init(v, ismax);
+        for (int j = threadIdx.x; j < np; j += kBlock) {
+#pragma unroll
+            for (int q = 0; q < kBigSums; ++q) {
+                const double pv = K.big_part2[((int64_t)p * kBigSums + q) * K.nbig_blocks + b0 + j];
+                v[q] = ismax[q] ? fmax(v[q], pv) : v[q] + pv;
+            }
+        }
+        block_reduce_multi<kBigSums>(v, ismax, lds);
+        if (threadIdx.x == 0) {
+            BigSocState Lp{};
+            Lp.cs = 3;
+            Lp.done = 0;
+            Lp.tL = tL;
+            Lp.tR = tR;
+            Lp.t = tr;
+            big_decide(Lp, v, s, tp, true);
+            if (Lp.done) {
+                sh_acc = p;
+                sh_th = Lp.t;
+            }
+        }
+        __syncthreads();
+        if (sh_acc >= 0) break;
+    }
+    if (sh_acc >= 0) {
+        const double th = sh_th;
+        const double my2 = big_apply_block(K, d2, y, ys, dy, st0, k, K.big_blk_first[b], b == b0, 3, 1.0 - s / th, th);
+        const double tot = part != nullptr ? block_sum(my2, lds) : 0.0;
+        if (threadIdx.x == 0) {
+            if (part != nullptr) part[b] = tot;
+            if (b == b0) {
+                S->t_prev = th;
+                S->n_refine += 1;
+                S->fin[0] = 1;   // robust chain: nothing left for the stages
+            }
+        }
+        return;
+    }
+    if (K.big_stages > 0) {
+        // robust chain: Newton sums of this block at tr for the first stage
+        big_stage_sums(K, d2, y, st0, k, K.big_blk_first[b], s, tr, b, 0, lds);
+        if (b == b0 && threadIdx.x == 0) {
+            S->tn[0] = tr;
+            S->tnL[0] = tL;
+            S->tnR[0] = tR;
+            S->fin[0] = 0;
+        }
+        return;
+    }
+    if (b != b0) return;
+    big_fallback(K, S, L, d2, y, ys, dy, st0, k, s, tr, b0, part, lds);
+}
+
+// Newton stage j of a robust chain: every workgroup reduces the stage sums at
+// t_j in fixed order and takes the same bracketed Newton step; converged, it
+// projects its block at the new head; otherwise it computes its block's sums
+// at the new point for stage j + 1.  The last stage leaves an unconverged cone
+// to its first workgroup (exact Newton, whole cone).  Stage j reads the
+// parity-j slots and fin[j]; it writes the other parity and fin[j + 1].
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bigsoc_stage(ConeLayout K, const double *__restrict__ d2,
+                                                         T *__restrict__ y, const T *__restrict__ ys,
+                                                         T *__restrict__ dy, StepState *st, int slot,
+                                                         double *__restrict__ part, int j) {
+    __shared__ double lds[(kBlock / 64) * kBigSums];
+    __shared__ double sh_t, sh_tL, sh_tR;
+    __shared__ int sh_done;
+    __shared__ BigSocState L;
+    if (st != nullptr && halted(st, slot)) return;
+    const int b = blockIdx.x;
+    const int cone = K.big_blk_cone[b];
+    BigSocState *S = K.big_state + cone;
+    const int b0 = K.big_cone_blk0[cone], np = K.big_cone_nblk[cone];
+    if (S->done || S->fin[j]) {
+        if (b == b0 && threadIdx.x == 0) S->fin[j + 1] = 1;
+        return;
+    }
+    const int64_t st0 = K.soc_big_start[cone];
+    const int k = K.soc_big_dim[cone] - 1;
+    const int pj = j & 1, pn = pj ^ 1;
+    const double s = S->s;
+    double A = 0.0, C = 0.0;
+    for (int i = threadIdx.x; i < np; i += kBlock) {
+        A += K.big_part3[((int64_t)pj * 2 + 0) * K.nbig_blocks + b0 + i];
+        C += K.big_part3[((int64_t)pj * 2 + 1) * K.nbig_blocks + b0 + i];
+    }
+    A = block_sum(A, lds);
+    C = block_sum(C, lds);
+    if (threadIdx.x == 0) {
+        double tL = S->tnL[pj], tR = S->tnR[pj];
+        bool done = false;
+        const double tn = soc_newton_step(S->tn[pj], A, C, tL, tR, done);
+        sh_t = tn;
+        sh_tL = tL;
+        sh_tR = tR;
+        sh_done = done ? 1 : 0;
+    }
+    __syncthreads();
+    const double tn = sh_t;
+    if (sh_done) {
+        const double my2 = big_apply_block(K, d2, y, ys, dy, st0, k, K.big_blk_first[b], b == b0, 3, 1.0 - s / tn, tn);
+        const double tot = part != nullptr ? block_sum(my2, lds) : 0.0;
+        if (threadIdx.x == 0) {
+            if (part != nullptr) part[b] = tot;
+            if (b == b0) {
+                S->t_prev = tn;
+                S->fin[j + 1] = 1;
+                S->n_fallback += 1;
+                S->n_newton += j + 1;
+            }
+        }
+        return;
+    }
+    if (j + 1 < K.big_stages) {
+        big_stage_sums(K, d2, y, st0, k, K.big_blk_first[b], s, tn, b, pn, lds);
+        if (b == b0 && threadIdx.x == 0) {
+            S->tn[pn] = tn;
+            S->tnL[pn] = sh_tL;
+            S->tnR[pn] = sh_tR;
+            S->fin[j + 1] = 0;
+        }
+        return;
+    }
+    if (b != b0) return;
+    big_fallback(K, S, L, d2, y, ys, dy, st0, k, s, tn, b0, part, lds);
+}
+
 
 // dual-side projection of the exponential section: the first nexp blocks
 // (expG rows, y in Kexp*, code 12) and then ndexp blocks (dual_expG rows,
@@ -496,7 +763,8 @@ __global__ __launch_bounds__(kBlock) void k_exp(ConeLayout K, T *__restrict__ y,
 template <typename T, int L>
 __global__ __launch_bounds__(kBlock) void k_atdy(Csr<T> Gt, int64_t n, const T *__restrict__ dy,
                                                  const T *__restrict__ x, const T *__restrict__ xs,
-                                                 StepState *st, int slot, double *__restrict__ part) {
+                                                 double *__restrict__ gd, StepState *st, int slot,
+                                                 double *__restrict__ part) {
     __shared__ double lds[kBlock / 64];
     if (halted(st, slot)) return;
     const int lane = (int)(threadIdx.x % L);
@@ -504,7 +772,10 @@ __global__ __launch_bounds__(kBlock) void k_atdy(Csr<T> Gt, int64_t n, const T *
     for (int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / L; row < n;
          row += (int64_t)gridDim.x * (kBlock / L)) {
         const double acc = row_dot<T, L>(Gt, row, lane, dy);
-        if (lane == 0) v += ((double)x[row] - (double)xs[row]) * acc;
+        if (lane == 0) {
+            gd[row] = acc;
+            v += ((double)x[row] - (double)xs[row]) * acc;
+        }
     }
     v = block_sum(v, lds);
     if (threadIdx.x == 0) part[blockIdx.x] = v;
@@ -566,7 +837,8 @@ __global__ __launch_bounds__(1024) void k_step(StepState *st, int slot, const do
     const double den = 2.0 * fabs(c);
     const double eta_bar = num / (den + 1e-20);
     const double k1 = (double)(st->iter_stepsize + 1);
-    const double eta_prime = fmin((1.0 - pow(k1, -0.3)) * eta_bar, (1.0 + pow(k1, -0.6)) * eta);
+    const double p3 = exp(-0.3 * log(k1));           // k1^-0.3; k1^-0.6 = p3^2
+    const double eta_prime = fmin((1.0 - p3) * eta_bar, (1.0 + p3 * p3) * eta);
     st->eta_bar = eta_bar;
     st->eta_prime = eta_prime;
     const bool accept = eta <= eta_bar;
@@ -596,6 +868,8 @@ __global__ __launch_bounds__(kBlock) void k_reflect(Iter<T> it, int64_t n, int64
         const T nx = (T)(tmp * ((1.0 + e) * xv - e * xl) + tail * xv);
         it.xs[i] = nx;
         it.xm[i] = (T)(((double)it.xm[i] * cum + (double)nx * eta) / (cum + eta));
+        // ys' = tmp ((1+e) y - e ys) + tail y = ys + (tmp (1+e) + tail) dy, as tmp + tail = 1
+        it.g[i] += (tmp * (1.0 + e) + tail) * it.gd[i];
     }
     if (i < m) {
         const double yv = (double)it.y[i], yl = (double)it.ys[i];
@@ -606,6 +880,11 @@ __global__ __launch_bounds__(kBlock) void k_reflect(Iter<T> it, int64_t n, int64
 }
 
 static inline unsigned nblocks(int64_t work) { return (unsigned)((work + kBlock - 1) / kBlock); }
+// the elementwise primal step: one element per thread up to 1024 workgroups
+static inline unsigned primal_grid(int64_t n) {
+    const unsigned b = nblocks(n);
+    return b < 1024u ? b : 1024u;
+}
 unsigned attempt_grid(int64_t rows, int lanes, int cap) {
     const unsigned b = nblocks(rows * lanes);
     const unsigned c = cap > 0 ? (unsigned)cap : kAttemptMaxGrid;
@@ -648,11 +927,11 @@ static void launch_cones(hipStream_t s, const ConeLayout &K, const double *d2, T
     }
     if (K.nsoc_big > 0) {
         const unsigned nb = (unsigned)K.nbig_blocks;
-        k_bigsoc_pass<T><<<nb, kBlock, 0, s>>>(K, d2, y, st, slot, 0);
-        k_bigsoc_pass<T><<<nb, kBlock, 0, s>>>(K, d2, y, st, slot, 1);
-        k_bigsoc_finish<T><<<K.nsoc_big, 1024, 0, s>>>(K, d2, y, st, slot);
-        k_bigsoc_apply<T><<<nb, kBlock, 0, s>>>(K, d2, y, ys, dy, st, slot,
-                                                part ? part + soc_blocks(K) : nullptr);
+        double *pb = part ? part + soc_blocks(K) : nullptr;
+        k_bigsoc_moments<T><<<nb, kBlock, 0, s>>>(K, d2, y, st, slot);
+        k_bigsoc_apply<T><<<nb, kBlock, 0, s>>>(K, d2, y, ys, dy, st, slot, pb);
+        k_bigsoc_rare<T><<<nb, kBlock, 0, s>>>(K, d2, y, ys, dy, st, slot, pb);
+        for (int j = 0; j < K.big_stages; ++j) k_bigsoc_stage<T><<<nb, kBlock, 0, s>>>(K, d2, y, ys, dy, st, slot, pb, j);
     }
     if (K.nexp + K.ndexp > 0)
         k_exp<T><<<exp_blocks(K), kBlock, 0, s>>>(
@@ -669,14 +948,14 @@ void launch_attempt_part(int part_id, hipStream_t s, int slot, const Csr<T> &G, 
                          double *part, int64_t stride, hipEvent_t ev0, hipEvent_t ev1) {
     const unsigned bp = attempt_grid(P.n, Gt.lanes, Gt.grid);
     const unsigned bd = attempt_grid(P.m, G.lanes, G.grid);
+    const unsigned pg = primal_grid(P.n);
     switch (part_id) {
     case 0:
-#define KP(L)                                                                                                \
-    if (ev0) hipExtLaunchKernelGGL(k_primal<T, L>, dim3(bp), dim3(kBlock), 0, s, ev0, ev1, 0, Gt, P, it.ys,    \
-                                   (const T *)it.xs, it.x, it.xbar, st, slot, part);                        \
-    else k_primal<T, L><<<bp, kBlock, 0, s>>>(Gt, P, it.ys, it.xs, it.x, it.xbar, st, slot, part)
-        PDCS_LANES(Gt.lanes, KP)
-#undef KP
+        if (ev0)
+            hipExtLaunchKernelGGL(k_primal<T>, dim3(pg), dim3(kBlock), 0, s, ev0, ev1, 0, P, (const double *)it.g,
+                                  (const T *)it.xs, it.x, it.xbar, st, slot, part);
+        else
+            k_primal<T><<<pg, kBlock, 0, s>>>(P, it.g, it.xs, it.x, it.xbar, st, slot, part);
         break;
     case 1:
 #define KD(L)                                                                                                \
@@ -692,8 +971,9 @@ void launch_attempt_part(int part_id, hipStream_t s, int slot, const Csr<T> &G, 
     case 3:
 #define KA(L)                                                                                                \
     if (ev0) hipExtLaunchKernelGGL(k_atdy<T, L>, dim3(bp), dim3(kBlock), 0, s, ev0, ev1, 0, Gt, P.n,          \
-                                   (const T *)it.dy, (const T *)it.x, (const T *)it.xs, st, slot, part + 2 * stride); \
-    else k_atdy<T, L><<<bp, kBlock, 0, s>>>(Gt, P.n, it.dy, it.x, it.xs, st, slot, part + 2 * stride)
+                                   (const T *)it.dy, (const T *)it.x, (const T *)it.xs, it.gd, st, slot,        \
+                                   part + 2 * stride);                                                         \
+    else k_atdy<T, L><<<bp, kBlock, 0, s>>>(Gt, P.n, it.dy, it.x, it.xs, it.gd, st, slot, part + 2 * stride)
         PDCS_LANES(Gt.lanes, KA)
 #undef KA
         break;
@@ -701,9 +981,9 @@ void launch_attempt_part(int part_id, hipStream_t s, int slot, const Csr<T> &G, 
         const int ncone = cone_part_blocks(K);
         if (ev0)
             hipExtLaunchKernelGGL(k_step, dim3(1), dim3(1024), 0, s, ev0, ev1, 0, st, slot, (const double *)part, stride,
-                                  (int)bp, (int)(bd + ncone), (int)bp);
+                                  (int)pg, (int)(bd + ncone), (int)bp);
         else
-            k_step<<<1, 1024, 0, s>>>(st, slot, part, stride, (int)bp, (int)(bd + ncone), (int)bp);
+            k_step<<<1, 1024, 0, s>>>(st, slot, part, stride, (int)pg, (int)(bd + ncone), (int)bp);
         break;
     }
     default:

@@ -99,8 +99,8 @@ struct pdcs_problem {
     int graph_chunk = 0;
 
     ~pdcs_problem() {
-        for (auto g : graphs64) hipGraphExecDestroy(g);
-        for (auto g : graphs32) hipGraphExecDestroy(g);
+        for (auto g : graphs64) if (g) hipGraphExecDestroy(g);
+        for (auto g : graphs32) if (g) hipGraphExecDestroy(g);
         if (h_st) hipHostFree(h_st);
         if (h_kout) hipHostFree(h_kout);
         if (h_dout) hipHostFree(h_dout);
@@ -246,8 +246,9 @@ int build_cone_layout(pdcs_ctx *ctx, DevBuf &M, int64_t m, int64_t mzero, int64_
     if (K.nsoc_big > 0) {
         K.big_state = M.alloc<BigSocState>(K.nsoc_big);
         K.big_part = M.alloc<double>((int64_t)K.nbig_blocks * kBigSums);
-        K.big_count = M.alloc<unsigned int>(K.nsoc_big);
-        if (!K.big_state || !K.big_part || !K.big_count) return fail(ctx, PDCS_ERR_HIP, "alloc failed");
+        K.big_part2 = M.alloc<double>((int64_t)K.nbig_blocks * kBigSums * kBigPoints);
+        K.big_part3 = M.alloc<double>((int64_t)K.nbig_blocks * 4);
+        if (!K.big_state || !K.big_part || !K.big_part2 || !K.big_part3) return fail(ctx, PDCS_ERR_HIP, "alloc failed");
         std::vector<BigSocState> init(K.nsoc_big);
         for (auto &b : init) { std::memset(&b, 0, sizeof(b)); b.t_prev = NAN; b.done = 1; }
         upload(ctx, K.big_state, init.data(), K.nsoc_big);
@@ -299,6 +300,7 @@ int alloc_iters(pdcs_problem *P) {
     auto &M = P->mem;
     I.it.x = M.alloc<T>(n); I.it.xs = M.alloc<T>(n); I.it.xm = M.alloc<T>(n); I.it.xbar = M.alloc<T>(n);
     I.it.y = M.alloc<T>(m); I.it.ys = M.alloc<T>(m); I.it.ym = M.alloc<T>(m); I.it.dy = M.alloc<T>(m);
+    I.it.g = M.alloc<double>(n); I.it.gd = M.alloc<double>(n);
     I.x0 = M.alloc<T>(n); I.y0 = M.alloc<T>(m); I.x0l = M.alloc<T>(n); I.y0l = M.alloc<T>(m);
     I.tn1 = M.alloc<T>(n); I.tn2 = M.alloc<T>(n); I.tm1 = M.alloc<T>(m); I.tm2 = M.alloc<T>(m);
     I.h1 = M.alloc<T>(n); I.h2 = M.alloc<T>(m); I.xd = M.alloc<T>(n); I.yd = M.alloc<T>(m);
@@ -314,7 +316,8 @@ int alloc_iters(pdcs_problem *P) {
         launch_convert(P->ctx->stream, (float *)I.vals, P->d_vals, P->nnz);
         launch_convert(P->ctx->stream, (float *)I.tvals, P->d_tvals, P->nnz);
     }
-    if (!I.it.x || !I.h2 || !I.bu || !I.yd || !I.rx || !I.ry) return fail(P->ctx, PDCS_ERR_HIP, "iterate allocation failed");
+    if (!I.it.x || !I.h2 || !I.bu || !I.yd || !I.rx || !I.ry || !I.it.g || !I.it.gd)
+        return fail(P->ctx, PDCS_ERR_HIP, "iterate allocation failed");
     // scaled data in precision T: c/Dr, h/Dl, bl*Dr, bu*Dr
     std::vector<double> hDr(P->n), hDl(P->m);
     HIPCHK(P->ctx, copy_sync(P->ctx, hDr.data(), P->Dr, sizeof(double) * P->n, hipMemcpyDeviceToHost));
@@ -419,8 +422,9 @@ int build_problem(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double 
     if (P->K.nsoc_big > 0) {
         P->Kaux.big_state = M.alloc<BigSocState>(P->K.nsoc_big);
         P->Kaux.big_part = M.alloc<double>((int64_t)P->K.nbig_blocks * kBigSums);
-        P->Kaux.big_count = M.alloc<unsigned int>(P->K.nsoc_big);
-        if (!P->Kaux.big_state || !P->Kaux.big_part || !P->Kaux.big_count) {
+        P->Kaux.big_part2 = M.alloc<double>((int64_t)P->K.nbig_blocks * kBigSums * kBigPoints);
+        P->Kaux.big_part3 = M.alloc<double>((int64_t)P->K.nbig_blocks * 4);
+        if (!P->Kaux.big_state || !P->Kaux.big_part || !P->Kaux.big_part2 || !P->Kaux.big_part3) {
             delete P;
             return fail(ctx, PDCS_ERR_HIP, "alloc failed");
         }
@@ -460,7 +464,8 @@ int build_problem(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double 
 // SpMV row-schedule autotuner.  For G and G' in precision T, every lanes-per-row
 // L in {1..64} and workgroup cap in {1024..8192} is timed with the attempt
 // kernels' access pattern (launch_spmv_sched: grid-stride, L lanes per row,
-// FP64 accumulation) and the fastest is kept for the problem's lifetime.  The
+// FP64 accumulation) and the first within 5% of the fastest is kept for the
+// problem's lifetime.  The
 // fused attempt kernels share that pattern, so the probe ranks their
 // schedules; the residual kernels keep the fixed heuristic.
 template <typename T>
@@ -472,8 +477,11 @@ int autotune(pdcs_problem *P, bool enable) {
     hipEvent_t e0, e1;
     HIPCHK(P->ctx, hipEventCreate(&e0));
     HIPCHK(P->ctx, hipEventCreate(&e1));
+    // the first schedule (lanes, then cap ascending) within 5% of the fastest:
+    // timing noise then rarely changes the choice, and with it the summation order
     auto tune = [&](Csr<T> A, const T *x, T *y, int &lanes, int &grid) -> int {
-        float best = INFINITY;
+        struct Cand { int L, g; float ms; };
+        std::vector<Cand> cand;
         for (int L = 1; L <= 64; L <<= 1) {
             const int64_t need = (A.rows * L + kBlock - 1) / kBlock;
             for (int g = 1024; g <= kTuneMaxGrid; g <<= 1) {
@@ -488,9 +496,13 @@ int autotune(pdcs_problem *P, bool enable) {
                 HIPCHK(P->ctx, hipEventSynchronize(e1));
                 float ms = 0.f;
                 HIPCHK(P->ctx, hipEventElapsedTime(&ms, e0, e1));
-                if (ms < best) { best = ms; lanes = L; grid = g; }
+                cand.push_back({L, g, ms});
             }
         }
+        float best = INFINITY;
+        for (const Cand &c : cand) best = std::min(best, c.ms);
+        for (const Cand &c : cand)
+            if (c.ms <= 1.05f * best) { lanes = c.L; grid = c.g; break; }
         return PDCS_OK;
     };
     int rc = tune(Csr<T>{P->d_rowptr, P->d_col, I.vals, P->m, 1, 0}, I.tn1, I.tm1, sc.lanesG, sc.gridG);
@@ -815,13 +827,39 @@ struct Solver {
         return om;
     }
 
+    // recompute the carried G'ys of the attempt kernels from ys
+    void refresh_gys() { launch_spmv_acc<T>(s, Gt, I.it.ys, I.it.g); }
+
     // graph of `len` attempt slots
-    hipGraphExec_t make_graph(int len) {
+    // The fast attempt chain, or the robust one with the large-SOC Newton
+    // stages: chosen per chunk, robust at an epoch start and while the last
+    // chunk had projections that the series could not finish.
+    bool robust_next = true;
+    int64_t fallbacks_seen = 0;
+    ConeLayout chain_layout(bool robust) const {
+        ConeLayout Kc = P->K;
+        Kc.big_stages = (robust && P->K.nsoc_big > 0) ? kBigStages : 0;
+        return Kc;
+    }
+    int update_chain_choice() {
+        if (P->K.nsoc_big == 0) return PDCS_OK;
+        std::vector<BigSocState> bs(P->K.nsoc_big);
+        HIPCHK(P->ctx, hipMemcpyAsync(bs.data(), P->K.big_state, sizeof(BigSocState) * bs.size(), hipMemcpyDeviceToHost, s));
+        if (int r = sync()) return r;
+        int64_t nf = 0;
+        for (const auto &b : bs) nf += b.n_fallback;
+        robust_next = nf > fallbacks_seen;
+        fallbacks_seen = nf;
+        return PDCS_OK;
+    }
+
+    hipGraphExec_t make_graph(int len, bool robust) {
         hipGraph_t g = nullptr;
         hipGraphExec_t ge = nullptr;
+        const ConeLayout Kc = chain_layout(robust);
         if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
         for (int k = 0; k < len; ++k)
-            launch_attempt<T>(s, k, G, Gt, PV, P->K, P->dsc2, I.it, P->st, P->apart, P->apart_stride);
+            launch_attempt<T>(s, k, G, Gt, PV, Kc, P->dsc2, I.it, P->st, P->apart, P->apart_stride);
         if (hipStreamEndCapture(s, &g) != hipSuccess) return nullptr;
         if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) ge = nullptr;
         hipGraphDestroy(g);
@@ -831,22 +869,25 @@ struct Solver {
 
     // enqueue `len` attempts starting at slot 0; stop_at reset to len first
     int run_chunk(int len) {
+        const bool robust = robust_next && P->K.nsoc_big > 0;
         P->h_st->stop_at = (int64_t)1 << 62;
         HIPCHK(P->ctx, hipMemcpyAsync(&P->st->stop_at, &P->h_st->stop_at, sizeof(int64_t), hipMemcpyHostToDevice, s));
         if (prm->use_graph && len == P->graph_chunk) {
             auto &gc = graph_cache();
-            if (gc.empty()) {
-                hipGraphExec_t ge = make_graph(len);
-                if (!ge) return fail(P->ctx, PDCS_ERR_HIP, "graph capture failed");
-                gc.push_back(ge);
+            if (gc.size() < 2) gc.resize(2, nullptr);
+            if (!gc[robust]) {
+                gc[robust] = make_graph(len, robust);
+                if (!gc[robust]) return fail(P->ctx, PDCS_ERR_HIP, "graph capture failed");
             }
-            HIPCHK(P->ctx, hipGraphLaunch(gc[0], s));
+            HIPCHK(P->ctx, hipGraphLaunch(gc[robust], s));
         } else {
+            const ConeLayout Kc = chain_layout(robust);
             for (int k = 0; k < len; ++k)
-                launch_attempt<T>(s, k, G, Gt, PV, P->K, P->dsc2, I.it, P->st, P->apart, P->apart_stride);
+                launch_attempt<T>(s, k, G, Gt, PV, Kc, P->dsc2, I.it, P->st, P->apart, P->apart_stride);
         }
         HIPCHK(P->ctx, hipGetLastError());
-        return read_state();
+        if (int r = read_state()) return r;
+        return update_chain_choice();
     }
 
     // the main loop; `iter0` attempts already done (polish); returns PDCS_OK
@@ -876,6 +917,8 @@ struct Solver {
             launch_copy<T>(s, I.x0l, I.x0, n); launch_copy<T>(s, I.y0l, I.y0, m);
             launch_copy<T>(s, I.x0, it.xm, n); launch_copy<T>(s, I.y0, it.ym, m);
             launch_copy<T>(s, it.xs, it.xm, n); launch_copy<T>(s, it.ys, it.ym, m);
+            refresh_gys();
+            robust_next = true;
             if (prm->use_kkt_restart) restart_threshold(false);
             else restart_threshold(restart_dg_flag);
             if (rc) return rc;
@@ -929,6 +972,7 @@ struct Solver {
                     if ((rc = write_state())) return rc;
                     launch_continue(s, slot, P->st);
                     launch_reflect<T>(s, slot, it, n, m, P->st);
+                    refresh_gys();   // drop the rounding the carried G'ys has collected
                     if ((rc = read_state())) return rc;
                     if (h.stop_at > slot + 1 || h.reason == 1) continue;  // rejected or no further halt
                     reason = h.reason;
@@ -1226,12 +1270,12 @@ int time_kernels(pdcs_problem *P, int reps, double *ms, double *bytes) {
     const double sv = sizeof(T), n = (double)P->n, m = (double)P->m, nnz = (double)P->nnz, nb = (double)P->nb;
     const double melem = (double)(P->mzero + P->mnonneg), mc = m - melem;
     const double b[kAttemptKernels] = {
-        nnz * (sv + 4) + 4 * (n + 1) + sv * (m + 4 * n + 2 * nb),
+        8 * n + sv * (4 * n + 2 * nb),
         nnz * (sv + 4) + 4 * (m + 1) + sv * (n + 3 * m + melem),
         mc * (4 * sv + 8) + 8.0 * (double)(P->K.nsoc_small + P->K.nsoc_big),
-        nnz * (sv + 4) + 4 * (n + 1) + sv * (m + 2 * n),
+        nnz * (sv + 4) + 4 * (n + 1) + sv * (m + 2 * n) + 8 * n,
         8.0 * 3 * (double)P->apart_stride,
-        sv * 5 * (n + m)};
+        sv * 5 * (n + m) + 8 * 3 * n};
     hipEvent_t e0, e1;
     HIPCHK(ctx, hipEventCreate(&e0));
     HIPCHK(ctx, hipEventCreate(&e1));
@@ -1292,10 +1336,10 @@ int pdcs_set_schedule(pdcs_problem *P, int32_t precision, const int32_t *sched) 
     auto &sc = P->sched[precision == PDCS_FP64 ? 0 : 1];
     sc = {sched[0], sched[1], sched[2], sched[3], true};
     if (precision == PDCS_FP64) {
-        for (auto g : P->graphs64) hipGraphExecDestroy(g);
+        for (auto g : P->graphs64) if (g) hipGraphExecDestroy(g);
         P->graphs64.clear();
     } else {
-        for (auto g : P->graphs32) hipGraphExecDestroy(g);
+        for (auto g : P->graphs32) if (g) hipGraphExecDestroy(g);
         P->graphs32.clear();
     }
     return PDCS_OK;
