@@ -408,95 +408,6 @@ __device__ __forceinline__ double big_apply_block(const ConeLayout &K, const dou
     return my2;
 }
 
-// Decision and projection of the big cones.  Each workgroup decides for itself
-// from s_in / t0_in and the first-pass partials (read-only here); only the
-// workgroup holding the head writes the cone state.  dy and sum dy^2 when `dy`
-// is given; part[b] is this workgroup's sum (0 when the cone is left to
-// k_bigsoc_rare).
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bigsoc_apply(ConeLayout K, const double *__restrict__ d2,
-                                                         T *__restrict__ y, const T *__restrict__ ys,
-                                                         T *__restrict__ dy, StepState *st, int slot,
-                                                         double *__restrict__ part) {
-    __shared__ double lds[(kBlock / 64) * kBigSums];
-    __shared__ double sh_t, sh_lam, sh_th, sh_tL, sh_tR;
-    __shared__ int sh_cs, sh_done;
-    if (st != nullptr && halted(st, slot)) return;
-    const int b = blockIdx.x;
-    const int cone = K.big_blk_cone[b];
-    BigSocState *S = K.big_state + cone;
-    const int64_t st0 = K.soc_big_start[cone];
-    const int k = K.soc_big_dim[cone] - 1;
-    const int b0 = K.big_cone_blk0[cone], np = K.big_cone_nblk[cone];
-    const double s = S->s_in, t0 = S->t0_in;
-    double v[kBigSums];
-    bool ismax[kBigSums];
-    big_init(v, ismax);
-    for (int j = threadIdx.x; j < np; j += kBlock) {
-#pragma unroll
-        for (int q = 0; q < kBigSums; ++q) {
-            const double pv = K.big_part[(int64_t)q * K.nbig_blocks + b0 + j];
-            v[q] = ismax[q] ? fmax(v[q], pv) : v[q] + pv;
-        }
-    }
-    block_reduce_multi<kBigSums>(v, ismax, lds);
-    if (threadIdx.x == 0) {
-        BigSocState L{};
-        L.t_prev = NAN;
-        big_decide(L, v, s, t0, false);
-        double lam = 0.0, th = s;
-        if (L.cs == 2) { lam = 1.0; th = L.t_closed; }
-        if (L.cs == 3) { lam = 1.0 - s / L.t; th = L.t; }
-        sh_cs = L.cs;
-        sh_done = L.done;
-        sh_t = L.t;
-        sh_tL = L.tL;
-        sh_tR = L.tR;
-        sh_lam = lam;
-        sh_th = th;
-        if (b == b0) {
-            S->s = s;
-            S->cs = L.cs;
-            S->done = L.done;
-            S->series = L.series;
-            S->iters = 0;
-            S->t = L.t;
-            S->tL = L.tL;
-            S->tR = L.tR;
-            S->t_closed = L.t_closed;
-            S->n_root += L.n_root;
-            S->n_first += L.n_first;
-            if (L.done) S->t_prev = (L.cs == 1) ? NAN : th;
-        }
-    }
-    __syncthreads();
-    const int e0 = K.big_blk_first[b];
-    double my2 = 0.0;
-    if (sh_done) {
-        my2 = big_apply_block(K, d2, y, ys, dy, st0, k, e0, b == b0, sh_cs, sh_lam, sh_th);
-    } else {
-        // moments of this block at the second-pass expansion points for k_bigsoc_rare
-#pragma unroll 1
-        for (int p = 0; p < kBigPoints; ++p) {
-            const double tp = big_point(p, sh_t, t0, sh_tL, sh_tR);
-            big_init(v, ismax);
-            for (int q = 0; q < kBigElems; ++q) {
-                const int e = e0 + q * kBlock + threadIdx.x;
-                if (e < k) big_accumulate(v, (double)y[st0 + 1 + e], d2[st0 + 1 + e], s, tp, false);
-            }
-            block_reduce_multi<kBigSums>(v, ismax, lds);
-            if (threadIdx.x == 0)
-#pragma unroll
-                for (int q = 0; q < kBigSums; ++q)
-                    K.big_part2[((int64_t)p * kBigSums + q) * K.nbig_blocks + b] = v[q];
-        }
-    }
-    if (part != nullptr) {
-        my2 = block_sum(my2, lds);
-        if (threadIdx.x == 0) part[b] = my2;
-    }
-}
-
 // One workgroup finishes a cone: exact Newton steps from t_start, then the
 // projection of the whole cone (its sum dy^2 into the first block's slot).
 template <typename T>
@@ -566,6 +477,105 @@ __device__ double big_stage_sums(const ConeLayout &K, const double *__restrict__
     }
     return A;
 }
+
+// Decision and projection of the big cones.  Each workgroup decides for itself
+// from s_in / t0_in and the first-pass partials (read-only here); only the
+// workgroup holding the head writes the cone state.  dy and sum dy^2 when `dy`
+// is given; part[b] is this workgroup's sum.  A series that is not accepted:
+// in a robust chain (K.big_stages > 0) every workgroup computes the second
+// moment pass for k_bigsoc_rare; in the fast chain the head's workgroup
+// finishes the cone alone (exact Newton, whole cone), which the host answers
+// by switching to the robust chain.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bigsoc_apply(ConeLayout K, const double *__restrict__ d2,
+                                                         T *__restrict__ y, const T *__restrict__ ys,
+                                                         T *__restrict__ dy, StepState *st, int slot,
+                                                         double *__restrict__ part) {
+    __shared__ double lds[(kBlock / 64) * kBigSums];
+    __shared__ double sh_t, sh_lam, sh_th, sh_tL, sh_tR;
+    __shared__ int sh_cs, sh_done;
+    __shared__ BigSocState L1;
+    if (st != nullptr && halted(st, slot)) return;
+    const int b = blockIdx.x;
+    const int cone = K.big_blk_cone[b];
+    BigSocState *S = K.big_state + cone;
+    const int64_t st0 = K.soc_big_start[cone];
+    const int k = K.soc_big_dim[cone] - 1;
+    const int b0 = K.big_cone_blk0[cone], np = K.big_cone_nblk[cone];
+    const double s = S->s_in, t0 = S->t0_in;
+    double v[kBigSums];
+    bool ismax[kBigSums];
+    big_init(v, ismax);
+    for (int j = threadIdx.x; j < np; j += kBlock) {
+#pragma unroll
+        for (int q = 0; q < kBigSums; ++q) {
+            const double pv = K.big_part[(int64_t)q * K.nbig_blocks + b0 + j];
+            v[q] = ismax[q] ? fmax(v[q], pv) : v[q] + pv;
+        }
+    }
+    block_reduce_multi<kBigSums>(v, ismax, lds);
+    if (threadIdx.x == 0) {
+        BigSocState L{};
+        L.t_prev = NAN;
+        big_decide(L, v, s, t0, false);
+        double lam = 0.0, th = s;
+        if (L.cs == 2) { lam = 1.0; th = L.t_closed; }
+        if (L.cs == 3) { lam = 1.0 - s / L.t; th = L.t; }
+        sh_cs = L.cs;
+        sh_done = L.done;
+        sh_t = L.t;
+        sh_tL = L.tL;
+        sh_tR = L.tR;
+        sh_lam = lam;
+        sh_th = th;
+        if (b == b0) {
+            S->s = s;
+            S->cs = L.cs;
+            S->done = L.done;
+            S->series = L.series;
+            S->iters = 0;
+            S->t = L.t;
+            S->tL = L.tL;
+            S->tR = L.tR;
+            S->t_closed = L.t_closed;
+            S->n_root += L.n_root;
+            S->n_first += L.n_first;
+            if (L.done) S->t_prev = (L.cs == 1) ? NAN : th;
+        }
+    }
+    __syncthreads();
+    const int e0 = K.big_blk_first[b];
+    double my2 = 0.0;
+    if (sh_done) {
+        my2 = big_apply_block(K, d2, y, ys, dy, st0, k, e0, b == b0, sh_cs, sh_lam, sh_th);
+    } else if (K.big_stages == 0) {
+        if (b == b0) {
+            big_fallback(K, S, L1, d2, y, ys, dy, st0, k, s, sh_t, b0, part, lds);
+            return;
+        }
+    } else {
+        // moments of this block at the second-pass expansion points for k_bigsoc_rare
+#pragma unroll 1
+        for (int p = 0; p < kBigPoints; ++p) {
+            const double tp = big_point(p, sh_t, t0, sh_tL, sh_tR);
+            big_init(v, ismax);
+            for (int q = 0; q < kBigElems; ++q) {
+                const int e = e0 + q * kBlock + threadIdx.x;
+                if (e < k) big_accumulate(v, (double)y[st0 + 1 + e], d2[st0 + 1 + e], s, tp, false);
+            }
+            block_reduce_multi<kBigSums>(v, ismax, lds);
+            if (threadIdx.x == 0)
+#pragma unroll
+                for (int q = 0; q < kBigSums; ++q)
+                    K.big_part2[((int64_t)p * kBigSums + q) * K.nbig_blocks + b] = v[q];
+        }
+    }
+    if (part != nullptr) {
+        my2 = block_sum(my2, lds);
+        if (threadIdx.x == 0) part[b] = my2;
+    }
+}
+
 
 // Big cones whose first-pass series was not accepted (no-op otherwise).  Each
 // workgroup reduces the cone's second-pass partials point by point in fixed
@@ -930,7 +940,7 @@ static void launch_cones(hipStream_t s, const ConeLayout &K, const double *d2, T
         double *pb = part ? part + soc_blocks(K) : nullptr;
         k_bigsoc_moments<T><<<nb, kBlock, 0, s>>>(K, d2, y, st, slot);
         k_bigsoc_apply<T><<<nb, kBlock, 0, s>>>(K, d2, y, ys, dy, st, slot, pb);
-        k_bigsoc_rare<T><<<nb, kBlock, 0, s>>>(K, d2, y, ys, dy, st, slot, pb);
+        if (K.big_stages > 0) k_bigsoc_rare<T><<<nb, kBlock, 0, s>>>(K, d2, y, ys, dy, st, slot, pb);
         for (int j = 0; j < K.big_stages; ++j) k_bigsoc_stage<T><<<nb, kBlock, 0, s>>>(K, d2, y, ys, dy, st, slot, pb, j);
     }
     if (K.nexp + K.ndexp > 0)
@@ -1114,7 +1124,11 @@ template <typename T>
 void launch_project_dual(hipStream_t s, const ConeLayout &K, const double *d, const double *d2, T *y) {
     (void)d;
     if (K.mnonneg > 0) k_nonneg<T><<<nblocks(K.mnonneg), kBlock, 0, s>>>(y, K.mzero, K.mzero + K.mnonneg);
-    launch_cones<T>(s, K, d2, y, nullptr, nullptr, nullptr, 0, nullptr);
+    // standalone projections start from arbitrary points (no useful warm start):
+    // robust chain, multi-workgroup Newton stages for the large cones
+    ConeLayout Kr = K;
+    Kr.big_stages = kBigStages;
+    launch_cones<T>(s, Kr, d2, y, nullptr, nullptr, nullptr, 0, nullptr);
 }
 template void launch_project_dual<double>(hipStream_t, const ConeLayout &, const double *, const double *, double *);
 template void launch_project_dual<float>(hipStream_t, const ConeLayout &, const double *, const double *, float *);
