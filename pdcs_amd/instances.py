@@ -95,22 +95,24 @@ def _soc_interior(rng, dims):
 
 
 def soc_program(n: int, m: int, density: float, cone_dim: int, seed: int, box: float = 10.0,
-                name: str = "soc") -> ConicProblem:
+                name: str = "soc", dims: list | None = None) -> ConicProblem:
     """SOC program with a box-bounded primal (workload 0 and, per layer, workload 4).
 
     G (m x n) has a uniform pattern of round(density*m*n) entries with N(0,1)
-    values; K_d is m/cone_dim SOC blocks.  x0 ~ U(-5,5); s0 strictly inside
-    K_d (tail ~ N(0,1), head = ||tail|| + U(0.1,1)); h = G x0 - s0; y0 in K_d*
-    built the same way; c = G' y0 + N(0, 0.1^2).
+    values; K_d is m/cone_dim SOC blocks (or the blocks `dims`, summing to m).
+    x0 ~ U(-5,5); s0 strictly inside K_d (tail ~ N(0,1), head = ||tail|| +
+    U(0.1,1)); h = G x0 - s0; y0 in K_d* built the same way; c = G' y0 + N(0, 0.1^2).
     """
-    assert m % cone_dim == 0
+    if dims is None:
+        assert m % cone_dim == 0
+        dims = [cone_dim] * (m // cone_dim)
+    assert sum(dims) == m
     rng = np.random.default_rng(seed)
     nnz = int(round(density * m * n))
     r, cidx = random_pattern(rng, m, n, nnz)
     v = rng.standard_normal(nnz)
     rowptr, colidx, vals = csr_from_coo(m, r, cidx, v)
     x0 = rng.uniform(-5.0, 5.0, n)
-    dims = [cone_dim] * (m // cone_dim)
     s0 = _soc_interior(rng, dims)
     y0 = _soc_interior(rng, dims)
     P = ConicProblem(n=n, m=m, nb=n, c=np.zeros(n), rowptr=rowptr, colidx=colidx, vals=vals,

@@ -245,6 +245,25 @@ def test_solve_workload0_fp64(ctx, graph):
     assert res["iter"] == ref["iter"]
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_solve_two_large_socs(ctx, precision):
+    """Two SOCs above the 1024-row split (2500 and 1500 rows) beside 100 small
+    ones: per-cone moment partials, decisions, robust stages and fallbacks."""
+    P = instances.soc_program(300, 5000, 0.01, 0, seed=21, dims=[2500, 1500] + [10] * 100, name="soc_two_big")
+    ref, _, _ = oracle.solve(P, rel_tol=1e-8, abs_tol=1e-12)
+    assert ref["exit_code"] == 0
+    prob = Problem.from_instance(P, ctx)
+    res = prob.solve(precision=precision, time_limit=100.0)
+    x, y, _ = prob.solution()
+    assert res["exit_code"] == 0, res
+    assert max(res["rel_primal_res"], res["rel_dual_res"], res["rel_gap"]) < 1e-6
+    # two 1e-6 solutions can differ by ~2e-6 (1 + |obj|); the reference is solved to 1e-8
+    for k in ("pobj", "dobj"):
+        assert abs(res[k] - ref[k]) <= 2e-6 * (1 + abs(ref[k])), (k, res[k], ref[k])
+    chk = oracle.kkt(P, x, y)
+    assert max(chk["rel_primal_res"], chk["rel_dual_res"], chk["rel_gap"]) < 1e-6, chk
+
+
 def test_trajectory_tracks_oracle(ctx):
     """The first PDHG attempts match the oracle iterate for iterate (same
     scaling, step-size, extrapolation and projection arithmetic in FP64)."""
