@@ -136,7 +136,9 @@ void launch_attempt(hipStream_t s, int slot, const Csr<T> &G, const Csr<T> &Gt,
 
 // the kernels of one attempt, individually: 0 primal (update from the carried G'ys),
 // 1 dual (G xbar + update), 2 cone projections, 3 G'dy (kept for the carried G'ys) and its dot, 4 step
-// control, 5 reflection (and the G'ys update)
+// control, 5 reflection (and the G'ys update) fused with the next attempt's primal step.
+// launch_attempt enqueues parts 1-5: part 0 of an attempt is the previous
+// attempt's part 5, or a standalone launch at the start of a chunk.
 constexpr int kAttemptKernels = 6;
 constexpr unsigned kAttemptMaxGrid = 2048;   // default workgroup cap of the fused SpMV kernels (grid-stride beyond)
 constexpr int kTuneMaxGrid = 8192;           // largest cap the schedule autotuner tries

@@ -889,6 +889,60 @@ __global__ __launch_bounds__(kBlock) void k_reflect(Iter<T> it, int64_t n, int64
     }
 }
 
+// End of attempt `slot` and start of attempt slot + 1 in one launch: the
+// reflection of slot (when k_step accepted it, as k_reflect) and the primal
+// step of slot + 1 (when slot + 1 is not halted, as k_primal).  Both are
+// elementwise and the same thread handles element j of each, so the primal
+// step reads the new xs and G'ys from registers.  The m part of the reflection
+// takes one element per thread over the whole grid; the n part runs
+// grid-stride over the first pg workgroups, which write the primal partials.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_reflect_primal(Iter<T> it, ProbVec<T> P, StepState *st, int slot,
+                                                           int pg, double *__restrict__ part) {
+    __shared__ double lds[kBlock / 64];
+    const bool refl = st->reflect_slot == slot;
+    const bool prim = !halted(st, slot + 1);
+    if (!refl && !prim) return;
+    const double k = st->reflect_k, e = st->extra, eta = st->reflect_eta, cum = st->reflect_cum;
+    const double tmp = (k + 1.0) / (k + 2.0), tail = 1.0 / (k + 2.0);
+    const double ga = tmp * (1.0 + e) + tail;   // ys' = ys + ga dy
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (refl && i < P.m) {
+        const double yv = (double)it.y[i], yl = (double)it.ys[i];
+        const T ny = (T)(tmp * ((1.0 + e) * yv - e * yl) + tail * yv);
+        it.ys[i] = ny;
+        it.ym[i] = (T)(((double)it.ym[i] * cum + (double)ny * eta) / (cum + eta));
+    }
+    if ((int)blockIdx.x >= pg) return;
+    const double ps = st->eta / st->omega * st->dec;   // primal_step_size of slot + 1, scaling.jl:687-689
+    double d2 = 0.0;
+    for (int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x; row < P.n; row += (int64_t)pg * kBlock) {
+        double xl = (double)it.xs[row], gv = it.g[row];
+        if (refl) {
+            const double xv = (double)it.x[row];
+            const T nx = (T)(tmp * ((1.0 + e) * xv - e * xl) + tail * xv);
+            it.xs[row] = nx;
+            it.xm[row] = (T)(((double)it.xm[row] * cum + (double)nx * eta) / (cum + eta));
+            gv += ga * it.gd[row];
+            it.g[row] = gv;
+            xl = (double)nx;
+        }
+        if (prim) {
+            double v = xl + ps * (gv - (double)P.c[row]);              // primal_update, utils.cu:8
+            if (row < P.nb) v = fmin(fmax(v, (double)P.bl[row]), (double)P.bu[row]);  // code 18
+            const T xv = (T)v;
+            it.x[row] = xv;
+            it.xbar[row] = (T)(2.0 * (double)xv - xl);              // extrapolation_update, utils.cu:18
+            const double d = (double)xv - xl;
+            d2 += d * d;
+        }
+    }
+    if (prim) {
+        d2 = block_sum(d2, lds);
+        if (threadIdx.x == 0) part[blockIdx.x] = d2;
+    }
+}
+
 static inline unsigned nblocks(int64_t work) { return (unsigned)((work + kBlock - 1) / kBlock); }
 // the elementwise primal step: one element per thread up to 1024 workgroups
 static inline unsigned primal_grid(int64_t n) {
@@ -949,7 +1003,7 @@ static void launch_cones(hipStream_t s, const ConeLayout &K, const double *d2, T
             part ? part + soc_blocks(K) + (K.nsoc_big > 0 ? K.nbig_blocks : 0) : nullptr);
 }
 
-// ev0/ev1 (optional): for the single-kernel parts 0, 1, 3 and 4 the events are
+// ev0/ev1 (optional): for the single-kernel parts 0, 1, 3, 4 and 5 the events are
 // recorded at the kernel's own start and end (hipExtLaunchKernelGGL), so their
 // elapsed time is the kernel's execution time, as a profiler reports it
 template <typename T>
@@ -996,9 +1050,15 @@ void launch_attempt_part(int part_id, hipStream_t s, int slot, const Csr<T> &G, 
             k_step<<<1, 1024, 0, s>>>(st, slot, part, stride, (int)pg, (int)(bd + ncone), (int)bp);
         break;
     }
-    default:
-        k_reflect<T><<<nblocks(P.n > P.m ? P.n : P.m), kBlock, 0, s>>>(it, P.n, P.m, st, slot);
+    default: {
+        const unsigned gr = nblocks(P.m) > pg ? nblocks(P.m) : pg;
+        if (ev0)
+            hipExtLaunchKernelGGL(k_reflect_primal<T>, dim3(gr), dim3(kBlock), 0, s, ev0, ev1, 0, it, P, st, slot,
+                                  (int)pg, part);
+        else
+            k_reflect_primal<T><<<gr, kBlock, 0, s>>>(it, P, st, slot, (int)pg, part);
         break;
+    }
     }
 }
 template void launch_attempt_part<double>(int, hipStream_t, int, const Csr<double> &, const Csr<double> &,
@@ -1012,7 +1072,9 @@ template <typename T>
 void launch_attempt(hipStream_t s, int slot, const Csr<T> &G, const Csr<T> &Gt, const ProbVec<T> &P,
                     const ConeLayout &K, const double *d2, Iter<T> it, StepState *st, double *part,
                     int64_t stride) {
-    for (int k = 0; k < kAttemptKernels; ++k) launch_attempt_part<T>(k, s, slot, G, Gt, P, K, d2, it, st, part, stride);
+    // part 0 (the primal step) is the previous attempt's k_reflect_primal, or
+    // launched by the host at the start of a chunk that follows a halt
+    for (int k = 1; k < kAttemptKernels; ++k) launch_attempt_part<T>(k, s, slot, G, Gt, P, K, d2, it, st, part, stride);
 }
 template void launch_attempt<double>(hipStream_t, int, const Csr<double> &, const Csr<double> &,
                                      const ProbVec<double> &, const ConeLayout &, const double *,

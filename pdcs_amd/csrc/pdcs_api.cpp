@@ -868,8 +868,15 @@ struct Solver {
     std::vector<hipGraphExec_t> &graph_cache();
 
     // enqueue `len` attempts starting at slot 0; stop_at reset to len first
+    // the primal step of a chunk's first attempt is still to be launched (after
+    // a halt, or a host change of the iterates); a chunk that ran all its slots
+    // ends with the primal step of the next one
+    bool primal_pending = true;
+
     int run_chunk(int len) {
         const bool robust = robust_next && P->K.nsoc_big > 0;
+        if (primal_pending)
+            launch_attempt_part<T>(0, s, 0, G, Gt, PV, P->K, P->dsc2, I.it, P->st, P->apart, P->apart_stride);
         P->h_st->stop_at = (int64_t)1 << 62;
         HIPCHK(P->ctx, hipMemcpyAsync(&P->st->stop_at, &P->h_st->stop_at, sizeof(int64_t), hipMemcpyHostToDevice, s));
         if (prm->use_graph && len == P->graph_chunk) {
@@ -887,6 +894,7 @@ struct Solver {
         }
         HIPCHK(P->ctx, hipGetLastError());
         if (int r = read_state()) return r;
+        primal_pending = P->h_st->stop_at <= len;
         return update_chain_choice();
     }
 
@@ -919,6 +927,7 @@ struct Solver {
             launch_copy<T>(s, it.xs, it.xm, n); launch_copy<T>(s, it.ys, it.ym, m);
             refresh_gys();
             robust_next = true;
+            primal_pending = true;
             if (prm->use_kkt_restart) restart_threshold(false);
             else restart_threshold(restart_dg_flag);
             if (rc) return rc;
@@ -1275,7 +1284,7 @@ int time_kernels(pdcs_problem *P, int reps, double *ms, double *bytes) {
         mc * (4 * sv + 8) + 8.0 * (double)(P->K.nsoc_small + P->K.nsoc_big),
         nnz * (sv + 4) + 4 * (n + 1) + sv * (m + 2 * n) + 8 * n,
         8.0 * 3 * (double)P->apart_stride,
-        sv * 5 * (n + m) + 8 * 3 * n};
+        sv * 5 * (n + m) + 8 * 3 * n + sv * (3 * n + 2 * nb)};   // reflection + next primal step
     hipEvent_t e0, e1;
     HIPCHK(ctx, hipEventCreate(&e0));
     HIPCHK(ctx, hipEventCreate(&e1));
@@ -1298,7 +1307,7 @@ int time_kernels(pdcs_problem *P, int reps, double *ms, double *bytes) {
         for (int r = 0; r <= reps; ++r) {
             for (int q = 0; q < kAttemptKernels; ++q) {
                 const bool timed = q == k;
-                const bool single = k != 2 && k != 5;
+                const bool single = k != 2;
                 if (timed && !single) HIPCHK(ctx, hipEventRecord(e0, s));
                 launch_attempt_part<T>(q, s, 0, G, Gt, PV, P->K, P->dsc2, I.it, P->st, P->apart, P->apart_stride,
                                        timed && single ? e0 : nullptr, timed && single ? e1 : nullptr);

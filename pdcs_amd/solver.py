@@ -164,7 +164,7 @@ class Problem:
         _lib.check(self.ctx.L.pdcs_get_schedule(self.h, precision, a), self.ctx.h)
         return list(a)
 
-    KERNELS = ["primal_step", "dual_step", "cone_projection", "atdy_dot", "step_control", "reflection"]
+    KERNELS = ["primal_step", "dual_step", "cone_projection", "atdy_dot", "step_control", "reflection_next_primal"]
 
     def time_kernels(self, precision: int = 0, reps: int = 50):
         """Mean ms and algorithmic bytes per launch of each attempt kernel (after a solve)."""
