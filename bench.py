@@ -200,7 +200,7 @@ def main():
     dom = max(single, key=lambda k: kt_mixed[k]["ms"])
     ach = kt_mixed[dom]["bytes"] / (kt_mixed[dom]["ms"] * 1e-3) / 1e9
     traffic, tsrc = None, None
-    tfile = os.path.join(ROOT, "profiles", "traffic_soc_regression_200000x50000.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_soc_regression_{a.rows}x{a.cols}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             tj = json.load(f)

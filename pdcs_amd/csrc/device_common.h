@@ -122,6 +122,11 @@ __device__ __forceinline__ int soc_case(double s, const SocSums &z) {
 // at t (phi' = S1 / S^(3/2)).  Convergence: phi == 0 keeps t; a Newton step
 // shorter than 1e-8 t leaves an error ~ (1e-8)^2 t by quadratic convergence, so
 // its result is final; a bracket narrower than 4 ulps ends a bisection tail.
+// phi is concave: a step from the right of the root lands left of it, and
+// Newton from any point left of the root stays left and increases to it.  A
+// step from the right that leaves the bracket on the left therefore restarts
+// at tL (phi(tL) < 0) instead of bisecting, which would need ~50 steps when the
+// root lies near tL.
 __device__ __forceinline__ double soc_newton_step(double t, double S, double S1, double &tL,
                                                   double &tR, bool &done) {
     const double isq = 1.0 / sqrt(S);
@@ -131,7 +136,8 @@ __device__ __forceinline__ double soc_newton_step(double t, double S, double S1,
     const double dphi = S1 * isq * isq * isq;
     double tn = t - phi / dphi;
     bool newton = true;
-    if (!(tn >= tL && tn <= tR)) { tn = 0.5 * (tL + tR); newton = false; }
+    if (phi > 0.0 && dphi > 0.0 && tn < tL && tL < t) { tn = tL; newton = false; }
+    else if (!(tn >= tL && tn <= tR)) { tn = 0.5 * (tL + tR); newton = false; }
     done = (newton && fabs(tn - t) <= 1e-8 * fabs(tn)) || (tR - tL) <= 4.5e-16 * fabs(tR);
     return tn;
 }

@@ -8,7 +8,6 @@
 namespace pdcs {
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wavefronts)
-constexpr int64_t kKktStrideHost = 1 << 18;  // partial slots per residual quantity
 constexpr int kKktQuantities = 13;
 
 // CSR matrix on the device. `val` is float or double depending on the path.
@@ -41,8 +40,8 @@ struct BigSocState {
     // in the first / refine pass, exact fallbacks and their Newton steps
     int64_t n_root, n_first, n_refine, n_fallback, n_newton;
 };
-constexpr int kBigElems = 4;                       // tail elements per thread in a pass
-constexpr int kBigBlockElems = kBigElems * 256;    // tail elements per workgroup
+constexpr int kBigBlockElems = 1024;               // tail elements per workgroup (at least)
+constexpr int kBigMaxBlocks = 256;                 // workgroups per big cone (at most)
 constexpr int kBigMoments = 16;                    // series terms M_0..M_{K}, K = kBigMoments - 1
 // series roots are accepted when the dropped terms are below this fraction of S:
 // the reference solves these projections to a relative tolerance of 1e-12
@@ -67,6 +66,7 @@ struct ConeLayout {
     const int32_t *big_blk_cone; // cone of each workgroup
     const int32_t *big_blk_first;// first tail element of each workgroup
     const int32_t *big_cone_blk0, *big_cone_nblk;
+    const int32_t *big_cone_elems;  // tail elements per workgroup of each big cone
     BigSocState *big_state;      // per big cone
     double *big_part;            // kBigSums partials per workgroup, [q * nbig_blocks + block]
     double *big_part2;           // second moment pass: [(point * kBigSums + q) * nbig_blocks + block]
@@ -198,6 +198,7 @@ struct KktData {
     ConeLayout K;
     double *rtmp;          // m scratch
     double *scratch_big;   // nbig_blocks scratch (large-cone norms)
+    int64_t qstride;       // partial slots per residual quantity (>= kkt_partial_slots)
 };
 struct KktOut {
     double pobj, dobj, atyInf, atyN1, sInf, sN1, d2, dInf, axInf, axN1, p2, pInf, projInf;
@@ -205,6 +206,8 @@ struct KktOut {
 template <typename T>
 void launch_kkt(hipStream_t s, const KktData &D, const T *x, const T *y, double *part,
                 KktOut *out);
+// block partials one residual quantity needs for this problem (all residual kernels)
+int64_t kkt_partial_slots(const KktData &D);
 
 // scaling (Ruiz l_inf / Pock-Chambolle alpha=1) on the FP64 CSR values
 void launch_row_colmax(hipStream_t s, const Csr<double> &G, const Csr<double> &Gt, double *rowv,

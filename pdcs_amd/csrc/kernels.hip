@@ -48,7 +48,7 @@ __device__ __forceinline__ bool halted(const StepState *st, int slot) {
 
 // ------------------------------------------------ large scaled SOCs --
 // A cone longer than 1024 rows is split over workgroups of kBigBlockElems
-// tail rows.  k_bigsoc_moments computes, per workgroup, the case sums
+// tail rows (longer blocks when that would exceed kBigMaxBlocks workgroups).  k_bigsoc_moments computes, per workgroup, the case sums
 // (||d x||^2, ||x/d||^2, ||x||^2, the s == 0 closed form) and the moments of
 // S around the previous projection's head t0:
 //   w_i(t0 + h) = w_i0 (1 + h e_i),  w_i0 = t0 (1 + d_i^2) - s d_i^2,  e_i = (1 + d_i^2) / w_i0
@@ -69,6 +69,34 @@ __device__ __forceinline__ bool halted(const StepState *st, int slot) {
 // host while such cones occur, mostly in the first attempts of an epoch)
 // follows with K.big_stages multi-workgroup Newton stages (k_bigsoc_stage),
 // each a no-op once the cone is projected.
+
+// end (exclusive) of block b's tail elements: blocks hold big_cone_elems[cone]
+// elements each (1024, or more for cones split over kBigMaxBlocks workgroups)
+__device__ __forceinline__ int big_blk_end(const ConeLayout &K, int b, int k) {
+    const int e1 = K.big_blk_first[b] + K.big_cone_elems[K.big_blk_cone[b]];
+    return e1 < k ? e1 : k;
+}
+
+// f(y_i, d_i^2) over the tail elements [e0, e1) of a block: each thread takes
+// e0 + tid + j kBlock and loads kBigUnroll of them before using any, so a
+// block of thousands of rows (large cones) keeps loads in flight
+constexpr int kBigUnroll = 4;
+template <typename T, typename F>
+__device__ __forceinline__ void big_block_loop(const T *__restrict__ y, const double *__restrict__ d2, int64_t st0,
+                                               int e0, int e1, F f) {
+    for (int eb = e0 + (int)threadIdx.x; eb < e1; eb += kBlock * kBigUnroll) {
+        double yv[kBigUnroll], dv[kBigUnroll];
+#pragma unroll
+        for (int u = 0; u < kBigUnroll; ++u) {
+            const int e = eb + u * kBlock;
+            yv[u] = e < e1 ? (double)y[st0 + 1 + e] : 0.0;
+            dv[u] = e < e1 ? d2[st0 + 1 + e] : 1.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kBigUnroll; ++u)
+            if (eb + u * kBlock < e1) f(yv[u], dv[u]);
+    }
+}
 
 __device__ __forceinline__ void big_newton_update(BigSocState &S, double Ssum, double S1sum) {
     bool done = false;
@@ -326,11 +354,8 @@ __global__ __launch_bounds__(kBlock) void k_bigsoc_moments(ConeLayout K, const d
     double v[kBigSums];
     bool ismax[kBigSums];
     big_init(v, ismax);
-    const int e0 = K.big_blk_first[b];
-    for (int q = 0; q < kBigElems; ++q) {
-        const int e = e0 + q * kBlock + threadIdx.x;
-        if (e < k) big_accumulate(v, (double)y[st0 + 1 + e], d2[st0 + 1 + e], s, t0, true);
-    }
+    const int e0 = K.big_blk_first[b], e1 = big_blk_end(K, b, k);
+    big_block_loop(y, d2, st0, e0, e1, [&](double yv, double dv) { big_accumulate(v, yv, dv, s, t0, true); });
     block_reduce_multi<kBigSums>(v, ismax, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
@@ -377,20 +402,32 @@ __device__ __forceinline__ double big_point(int p, double tr, double t0, double 
 template <typename T>
 __device__ __forceinline__ double big_apply_block(const ConeLayout &K, const double *__restrict__ d2,
                                                   T *__restrict__ y, const T *__restrict__ ys,
-                                                  T *__restrict__ dy, int64_t st0, int k, int e0, bool head,
+                                                  T *__restrict__ dy, int64_t st0, int e0, int e1, bool head,
                                                   int cs, double lam, double th) {
     double my2 = 0.0;
-    for (int q = 0; q < kBigElems; ++q) {
-        const int e = e0 + q * kBlock + threadIdx.x;
-        if (e < k) {
+    for (int eb = e0 + (int)threadIdx.x; eb < e1; eb += kBlock * kBigUnroll) {
+        double yv[kBigUnroll], dv[kBigUnroll], sv[kBigUnroll];
+#pragma unroll
+        for (int u = 0; u < kBigUnroll; ++u) {
+            const int e = eb + u * kBlock;
             const int64_t i = st0 + 1 + e;
-            double xv = (double)y[i];
+            const bool in = e < e1;
+            yv[u] = in ? (double)y[i] : 0.0;
+            dv[u] = in ? d2[i] : 1.0;
+            sv[u] = (in && dy != nullptr) ? (double)ys[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kBigUnroll; ++u) {
+            const int e = eb + u * kBlock;
+            if (e >= e1) break;
+            const int64_t i = st0 + 1 + e;
+            double xv = yv[u];
             if (cs == 1) xv = 0.0;
-            else if (cs >= 2) xv = xv / (1.0 + lam * d2[i]);
+            else if (cs >= 2) xv = xv / (1.0 + lam * dv[u]);
             const T vt = (T)xv;
             y[i] = vt;
             if (dy != nullptr) {
-                const double d = (double)vt - (double)ys[i];
+                const double d = (double)vt - sv[u];
                 dy[i] = (T)d;
                 my2 += d * d;
             }
@@ -463,12 +500,9 @@ __device__ void big_fallback(const ConeLayout &K, BigSocState *S, BigSocState &L
 // Newton sums (S, S') of this workgroup's block at t into big_part3 slot `par`
 template <typename T>
 __device__ double big_stage_sums(const ConeLayout &K, const double *__restrict__ d2, const T *__restrict__ y,
-                                 int64_t st0, int k, int e0, double s, double t, int b, int par, double *lds) {
+                                 int64_t st0, int e0, int e1, double s, double t, int b, int par, double *lds) {
     double A = 0.0, C = 0.0;
-    for (int q = 0; q < kBigElems; ++q) {
-        const int e = e0 + q * kBlock + threadIdx.x;
-        if (e < k) soc_newton_terms(t, s, (double)y[st0 + 1 + e], d2[st0 + 1 + e], A, C);
-    }
+    big_block_loop(y, d2, st0, e0, e1, [&](double yv, double dv) { soc_newton_terms(t, s, yv, dv, A, C); });
     A = block_sum(A, lds);
     C = block_sum(C, lds);
     if (threadIdx.x == 0) {
@@ -544,10 +578,10 @@ __global__ __launch_bounds__(kBlock) void k_bigsoc_apply(ConeLayout K, const dou
         }
     }
     __syncthreads();
-    const int e0 = K.big_blk_first[b];
+    const int e0 = K.big_blk_first[b], e1 = big_blk_end(K, b, k);
     double my2 = 0.0;
     if (sh_done) {
-        my2 = big_apply_block(K, d2, y, ys, dy, st0, k, e0, b == b0, sh_cs, sh_lam, sh_th);
+        my2 = big_apply_block(K, d2, y, ys, dy, st0, e0, e1, b == b0, sh_cs, sh_lam, sh_th);
     } else if (K.big_stages == 0) {
         if (b == b0) {
             big_fallback(K, S, L1, d2, y, ys, dy, st0, k, s, sh_t, b0, part, lds);
@@ -559,10 +593,7 @@ __global__ __launch_bounds__(kBlock) void k_bigsoc_apply(ConeLayout K, const dou
         for (int p = 0; p < kBigPoints; ++p) {
             const double tp = big_point(p, sh_t, t0, sh_tL, sh_tR);
             big_init(v, ismax);
-            for (int q = 0; q < kBigElems; ++q) {
-                const int e = e0 + q * kBlock + threadIdx.x;
-                if (e < k) big_accumulate(v, (double)y[st0 + 1 + e], d2[st0 + 1 + e], s, tp, false);
-            }
+            big_block_loop(y, d2, st0, e0, e1, [&](double yv, double dv) { big_accumulate(v, yv, dv, s, tp, false); });
             block_reduce_multi<kBigSums>(v, ismax, lds);
             if (threadIdx.x == 0)
 #pragma unroll
@@ -635,7 +666,8 @@ __global__ __launch_bounds__(kBlock) void k_bigsoc_rare(ConeLayout K, const doub
     }
     if (sh_acc >= 0) {
         const double th = sh_th;
-        const double my2 = big_apply_block(K, d2, y, ys, dy, st0, k, K.big_blk_first[b], b == b0, 3, 1.0 - s / th, th);
+        const double my2 = big_apply_block(K, d2, y, ys, dy, st0, K.big_blk_first[b], big_blk_end(K, b, k), b == b0, 3,
+                                           1.0 - s / th, th);
         const double tot = part != nullptr ? block_sum(my2, lds) : 0.0;
         if (threadIdx.x == 0) {
             if (part != nullptr) part[b] = tot;
@@ -649,7 +681,7 @@ __global__ __launch_bounds__(kBlock) void k_bigsoc_rare(ConeLayout K, const doub
     }
     if (K.big_stages > 0) {
         // robust chain: Newton sums of this block at tr for the first stage
-        big_stage_sums(K, d2, y, st0, k, K.big_blk_first[b], s, tr, b, 0, lds);
+        big_stage_sums(K, d2, y, st0, K.big_blk_first[b], big_blk_end(K, b, k), s, tr, b, 0, lds);
         if (b == b0 && threadIdx.x == 0) {
             S->tn[0] = tr;
             S->tnL[0] = tL;
@@ -709,7 +741,8 @@ __global__ __launch_bounds__(kBlock) void k_bigsoc_stage(ConeLayout K, const dou
     __syncthreads();
     const double tn = sh_t;
     if (sh_done) {
-        const double my2 = big_apply_block(K, d2, y, ys, dy, st0, k, K.big_blk_first[b], b == b0, 3, 1.0 - s / tn, tn);
+        const double my2 = big_apply_block(K, d2, y, ys, dy, st0, K.big_blk_first[b], big_blk_end(K, b, k), b == b0, 3,
+                                           1.0 - s / tn, tn);
         const double tot = part != nullptr ? block_sum(my2, lds) : 0.0;
         if (threadIdx.x == 0) {
             if (part != nullptr) part[b] = tot;
@@ -723,7 +756,7 @@ __global__ __launch_bounds__(kBlock) void k_bigsoc_stage(ConeLayout K, const dou
         return;
     }
     if (j + 1 < K.big_stages) {
-        big_stage_sums(K, d2, y, st0, k, K.big_blk_first[b], s, tn, b, pn, lds);
+        big_stage_sums(K, d2, y, st0, K.big_blk_first[b], big_blk_end(K, b, k), s, tn, b, pn, lds);
         if (b == b0 && threadIdx.x == 0) {
             S->tn[pn] = tn;
             S->tnL[pn] = sh_tL;
@@ -1308,8 +1341,8 @@ template void launch_convert<float, double>(hipStream_t, float *, const double *
 template void launch_convert<double, float>(hipStream_t, double *, const float *, int64_t);
 
 // ------------------------------------------------------------------ KKT --
-// Partial layout: quantity q of block b at part[q * kKktStride + b].
-constexpr int64_t kKktStride = kKktStrideHost;
+// Partial layout: quantity q of block b at part[q * qs + b], qs = KktData::qstride
+// (sized per problem from its block counts).
 enum { Q_POBJ, Q_ATYN1, Q_DOBJ, Q_SN1, Q_D2, Q_ATYINF, Q_SINF, Q_DINF, Q_AXN1, Q_P2, Q_AXINF, Q_PINF, Q_PROJINF, Q_N };
 
 // n side: G'y, dual residual, objective pieces (converge_info_calculation_diagonal!,
@@ -1351,11 +1384,11 @@ __global__ __launch_bounds__(kBlock) void k_kkt_n(KktData D, const T *__restrict
     const int qm[3] = {Q_ATYINF, Q_SINF, Q_DINF};
     for (int q = 0; q < 5; ++q) {
         const double v = block_sum(s_[q], lds);
-        if (threadIdx.x == 0) part[qs[q] * kKktStride + bstride + blockIdx.x] = v;
+        if (threadIdx.x == 0) part[qs[q] * D.qstride + bstride + blockIdx.x] = v;
     }
     for (int q = 0; q < 3; ++q) {
         const double v = block_max(m_[q], lds);
-        if (threadIdx.x == 0) part[qm[q] * kKktStride + bstride + blockIdx.x] = v;
+        if (threadIdx.x == 0) part[qm[q] * D.qstride + bstride + blockIdx.x] = v;
     }
 }
 
@@ -1395,18 +1428,18 @@ __global__ __launch_bounds__(kBlock) void k_kkt_m(KktData D, const T *__restrict
     const int qm[3] = {Q_AXINF, Q_PINF, Q_PROJINF};
     for (int q = 0; q < 3; ++q) {
         const double v = block_sum(s_[q], lds);
-        if (threadIdx.x == 0) part[qs[q] * kKktStride + (q == 0 ? dobj_off : bstride) + blockIdx.x] = v;
+        if (threadIdx.x == 0) part[qs[q] * D.qstride + (q == 0 ? dobj_off : bstride) + blockIdx.x] = v;
     }
     for (int q = 0; q < 3; ++q) {
         const double v = block_max(m_[q], lds);
-        if (threadIdx.x == 0) part[qm[q] * kKktStride + bstride + blockIdx.x] = v;
+        if (threadIdx.x == 0) part[qm[q] * D.qstride + bstride + blockIdx.x] = v;
     }
 }
 
 // cone rows: plain SOC projection of r = Gx - h in place on a copy, residual sums
 __global__ __launch_bounds__(kBlock) void k_kkt_soc(const int32_t *start, const int32_t *dim, int64_t ncone,
                                                     const double *__restrict__ r, double *__restrict__ part,
-                                                    int64_t bstride) {
+                                                    int64_t bstride, int64_t qs) {
     __shared__ double lds[kBlock / 64];
     const int64_t cone = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / 64;
     const int lane = threadIdx.x & 63;
@@ -1433,15 +1466,15 @@ __global__ __launch_bounds__(kBlock) void k_kkt_soc(const int32_t *start, const 
     pinf = block_max(pinf, lds);
     projinf = block_max(projinf, lds);
     if (threadIdx.x == 0) {
-        part[Q_P2 * kKktStride + bstride + blockIdx.x] = p2;
-        part[Q_PINF * kKktStride + bstride + blockIdx.x] = pinf;
-        part[Q_PROJINF * kKktStride + bstride + blockIdx.x] = projinf;
+        part[Q_P2 * qs + bstride + blockIdx.x] = p2;
+        part[Q_PINF * qs + bstride + blockIdx.x] = pinf;
+        part[Q_PROJINF * qs + bstride + blockIdx.x] = projinf;
     }
 }
 
 // exponential rows: residual of r = Gx - h against its projection onto K
 __global__ __launch_bounds__(kBlock) void k_kkt_exp(ConeLayout K, const double *__restrict__ r,
-                                                    double *__restrict__ part, int64_t bstride) {
+                                                    double *__restrict__ part, int64_t bstride, int64_t qs) {
     __shared__ double lds[kBlock / 64];
     const int64_t b = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     double p2 = 0.0, pinf = 0.0, projinf = 0.0;
@@ -1461,9 +1494,9 @@ __global__ __launch_bounds__(kBlock) void k_kkt_exp(ConeLayout K, const double *
     pinf = block_max(pinf, lds);
     projinf = block_max(projinf, lds);
     if (threadIdx.x == 0) {
-        part[Q_P2 * kKktStride + bstride + blockIdx.x] = p2;
-        part[Q_PINF * kKktStride + bstride + blockIdx.x] = pinf;
-        part[Q_PROJINF * kKktStride + bstride + blockIdx.x] = projinf;
+        part[Q_P2 * qs + bstride + blockIdx.x] = p2;
+        part[Q_PINF * qs + bstride + blockIdx.x] = pinf;
+        part[Q_PROJINF * qs + bstride + blockIdx.x] = projinf;
     }
 }
 
@@ -1481,17 +1514,16 @@ __global__ __launch_bounds__(kBlock) void k_kkt_big_norm(ConeLayout K, const dou
     const int64_t st0 = K.soc_big_start[cone];
     const int k = K.soc_big_dim[cone] - 1;
     double acc = 0.0;
-    const int e0 = K.big_blk_first[b];
-    for (int q = 0; q < kBigElems; ++q) {
-        const int e = e0 + q * kBlock + threadIdx.x;
-        if (e < k) { const double v = r[st0 + 1 + e]; acc += v * v; }
+    const int e0 = K.big_blk_first[b], e1 = big_blk_end(K, b, k);
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += kBlock) {
+        { const double v = r[st0 + 1 + e]; acc += v * v; }
     }
     acc = block_sum(acc, lds);
     if (threadIdx.x == 0) scratch[b] = acc;
 }
 __global__ __launch_bounds__(kBlock) void k_kkt_big_res(ConeLayout K, const double *__restrict__ r,
                                                         const double *__restrict__ scratch, double *__restrict__ part,
-                                                        int64_t bstride) {
+                                                        int64_t bstride, int64_t qs) {
     __shared__ double lds[kBlock / 64];
     const int b = blockIdx.x;
     const int cone = K.big_blk_cone[b];
@@ -1508,10 +1540,9 @@ __global__ __launch_bounds__(kBlock) void k_kkt_big_res(ConeLayout K, const doub
     if (nx <= -t) { c = 0.0; th = 0.0; }
     else if (nx > t) { c = (1.0 + t / nx) / 2.0; th = nx * c; }
     double p2 = 0.0, pinf = 0.0, projinf = 0.0;
-    const int e0 = K.big_blk_first[b];
-    for (int q = 0; q < kBigElems; ++q) {
-        const int e = e0 + q * kBlock + threadIdx.x;
-        if (e < k) {
+    const int e0 = K.big_blk_first[b], e1 = big_blk_end(K, b, k);
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += kBlock) {
+        {
             const double v = r[st0 + 1 + e], p = v * c, pr = p - v;
             p2 += pr * pr;
             pinf = fmax(pinf, fabs(pr));
@@ -1528,20 +1559,20 @@ __global__ __launch_bounds__(kBlock) void k_kkt_big_res(ConeLayout K, const doub
     pinf = block_max(pinf, lds);
     projinf = block_max(projinf, lds);
     if (threadIdx.x == 0) {
-        part[Q_P2 * kKktStride + bstride + b] = p2;
-        part[Q_PINF * kKktStride + bstride + b] = pinf;
-        part[Q_PROJINF * kKktStride + bstride + b] = projinf;
+        part[Q_P2 * qs + bstride + b] = p2;
+        part[Q_PINF * qs + bstride + b] = pinf;
+        part[Q_PROJINF * qs + bstride + b] = projinf;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_kkt_final(const double *part, KktCounts counts, KktOut *out) {
+__global__ __launch_bounds__(kBlock) void k_kkt_final(const double *part, KktCounts counts, int64_t qs, KktOut *out) {
     __shared__ double lds[kBlock / 64];
     const bool ismax[Q_N] = {false, false, false, false, false, true, true, true, false, false, true, true, true};
     double res[Q_N];
     for (int q = 0; q < Q_N; ++q) {
         double v = 0.0;
         for (int i = threadIdx.x; i < counts.c[q]; i += kBlock)
-            v = ismax[q] ? fmax(v, part[q * kKktStride + i]) : v + part[q * kKktStride + i];
+            v = ismax[q] ? fmax(v, part[q * qs + i]) : v + part[q * qs + i];
         res[q] = ismax[q] ? block_max(v, lds) : block_sum(v, lds);
     }
     if (threadIdx.x == 0) {
@@ -1559,6 +1590,13 @@ __global__ __launch_bounds__(kBlock) void k_kkt_final(const double *part, KktCou
         out->pInf = res[Q_PINF];
         out->projInf = res[Q_PROJINF];
     }
+}
+
+int64_t kkt_partial_slots(const KktData &D) {
+    const int64_t bn = nblocks(D.n * D.Gt.lanes), bm = nblocks(D.m * D.G.lanes);
+    const int64_t bs1 = D.K.nsoc_small > 0 ? nblocks((int64_t)D.K.nsoc_small * 64) : 0;
+    const int64_t bs2 = D.K.nsoc_big > 0 ? D.K.nbig_blocks : 0;
+    return bn + bm + bs1 + bs2 + exp_blocks(D.K);
 }
 
 template <typename T>
@@ -1584,13 +1622,15 @@ void launch_kkt(hipStream_t s, const KktData &D, const T *x, const T *y, double 
 #define KM(L) k_kkt_m<T, L><<<bm, kBlock, 0, s>>>(D, x, y, part, 0, (int64_t)bn)
     PDCS_LANES(D.G.lanes, KM)
 #undef KM
-    if (bs1) k_kkt_soc<<<bs1, kBlock, 0, s>>>(D.K.soc_small_start, D.K.soc_small_dim, D.K.nsoc_small, D.rtmp, part, bm);
+    if (bs1)
+        k_kkt_soc<<<bs1, kBlock, 0, s>>>(D.K.soc_small_start, D.K.soc_small_dim, D.K.nsoc_small, D.rtmp, part, bm,
+                                         D.qstride);
     if (bs2) {
         k_kkt_big_norm<<<bs2, kBlock, 0, s>>>(D.K, D.rtmp, D.scratch_big);
-        k_kkt_big_res<<<bs2, kBlock, 0, s>>>(D.K, D.rtmp, D.scratch_big, part, bm + bs1);
+        k_kkt_big_res<<<bs2, kBlock, 0, s>>>(D.K, D.rtmp, D.scratch_big, part, bm + bs1, D.qstride);
     }
-    if (be) k_kkt_exp<<<be, kBlock, 0, s>>>(D.K, D.rtmp, part, bm + bs1 + bs2);
-    k_kkt_final<<<1, kBlock, 0, s>>>(part, kc, out);
+    if (be) k_kkt_exp<<<be, kBlock, 0, s>>>(D.K, D.rtmp, part, bm + bs1 + bs2, D.qstride);
+    k_kkt_final<<<1, kBlock, 0, s>>>(part, kc, D.qstride, out);
 }
 template void launch_kkt<double>(hipStream_t, const KktData &, const double *, const double *, double *, KktOut *);
 template void launch_kkt<float>(hipStream_t, const KktData &, const float *, const float *, double *, KktOut *);

@@ -73,6 +73,7 @@ struct pdcs_problem {
     double *Dr = nullptr, *Dl = nullptr, *dsc = nullptr, *dsc2 = nullptr;
     double *kkt_big = nullptr;   // large-cone norm partials of the residual check
     double *rtmp = nullptr, *kpart = nullptr, *dpart = nullptr, *dout = nullptr, *apart = nullptr;
+    int64_t kqstride = 0;   // residual partial slots per quantity
     int64_t apart_stride = 0;
     KktOut *kout = nullptr;
     StepState *st = nullptr;
@@ -193,7 +194,7 @@ int upload(pdcs_ctx *ctx, T *dst, const T *src, int64_t n) {
 // longer ones are split over workgroups of kBigBlockElems tail rows.
 int build_cone_layout(pdcs_ctx *ctx, DevBuf &M, int64_t m, int64_t mzero, int64_t mnonneg,
                       const std::vector<int64_t> &soc, int64_t nexp, int64_t ndexp, ConeLayout &K) {
-    std::vector<int32_t> ss_start, ss_dim, sb_start, sb_dim, blk_cone, blk_first, cone_blk0, cone_nblk;
+    std::vector<int32_t> ss_start, ss_dim, sb_start, sb_dim, blk_cone, blk_first, cone_blk0, cone_nblk, cone_elems;
     int64_t off = mzero + mnonneg;
     int maxsmall = 0;
     for (int64_t q : soc) {
@@ -205,12 +206,21 @@ int build_cone_layout(pdcs_ctx *ctx, DevBuf &M, int64_t m, int64_t mzero, int64_
             const int c = (int)sb_start.size();
             sb_start.push_back((int32_t)off);
             sb_dim.push_back((int32_t)q);
-            const int64_t nb = (q - 1 + kBigBlockElems - 1) / kBigBlockElems;
+            // blocks of kBigBlockElems tail rows, at most kBigMaxBlocks per cone (then
+            // longer blocks, a multiple of the workgroup size): the decision kernels
+            // reduce a cone's block partials in every workgroup
+            int64_t per = kBigBlockElems;
+            int64_t nb = (q - 1 + per - 1) / per;
+            if (nb > kBigMaxBlocks) {
+                per = ((q - 1 + kBigMaxBlocks - 1) / kBigMaxBlocks + kBlock - 1) / kBlock * kBlock;
+                nb = (q - 1 + per - 1) / per;
+            }
             cone_blk0.push_back((int32_t)blk_cone.size());
             cone_nblk.push_back((int32_t)nb);
+            cone_elems.push_back((int32_t)per);
             for (int64_t b = 0; b < nb; ++b) {
                 blk_cone.push_back(c);
-                blk_first.push_back((int32_t)(b * kBigBlockElems));
+                blk_first.push_back((int32_t)(b * per));
             }
         }
         off += q;
@@ -242,6 +252,7 @@ int build_cone_layout(pdcs_ctx *ctx, DevBuf &M, int64_t m, int64_t mzero, int64_
     K.big_blk_cone = up32(blk_cone);
     K.big_blk_first = up32(blk_first);
     K.big_cone_blk0 = up32(cone_blk0);
+    K.big_cone_elems = up32(cone_elems);
     K.big_cone_nblk = up32(cone_nblk);
     if (K.nsoc_big > 0) {
         K.big_state = M.alloc<BigSocState>(K.nsoc_big);
@@ -336,6 +347,8 @@ int alloc_iters(pdcs_problem *P) {
     return PDCS_OK;
 }
 
+KktData kkt_data(pdcs_problem *P);
+
 int build_problem(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double *c, const int32_t *rowptr,
                   const int32_t *colidx, const double *vals, const double *h, const double *bl, const double *bu,
                   const pdcs_cones *cones, pdcs_problem **out) {
@@ -390,7 +403,6 @@ int build_problem(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double 
     P->blf = M.alloc<double>(nb); P->buf = M.alloc<double>(nb); P->laml = M.alloc<double>(nb); P->lamu = M.alloc<double>(nb);
     P->Dr = M.alloc<double>(n); P->Dl = M.alloc<double>(m); P->dsc = M.alloc<double>(m); P->dsc2 = M.alloc<double>(m);
     P->rtmp = M.alloc<double>(m);
-    P->kpart = M.alloc<double>((kKktQuantities + 1) * kKktStrideHost);
     P->dpart = M.alloc<double>(4 * 512);
     P->dout = M.alloc<double>(8);
     P->kout = M.alloc<KktOut>(1);
@@ -405,17 +417,20 @@ int build_problem(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double 
     P->lanesG = lanes_for(nnz, m);
     P->lanesGt = lanes_for(nnz, n);
     for (auto &sc : P->sched) sc = {P->lanesG, P->lanesGt, (int)kAttemptMaxGrid, (int)kAttemptMaxGrid, false};
-    // residual partial arrays hold kKktStrideHost blocks per quantity
-    {
-        const int64_t bn = (n * P->lanesGt + kBlock - 1) / kBlock, bm = (m * P->lanesG + kBlock - 1) / kBlock;
-        if (bn + bm + (m + 3) / 4 + 1024 >= kKktStrideHost) {
-            delete P;
-            return fail(ctx, PDCS_ERR_UNSUPPORTED, "problem too large for the residual reduction layout");
-        }
-    }
     if ((rc = build_cone_layout(ctx, M, m, cones->mzero, cones->mnonneg, P->soc, cones->nexp, cones->ndualexp, P->K))) {
         delete P;
         return rc;
+    }
+    // residual partials: one slot per block of every residual kernel, per quantity
+    P->kqstride = kkt_partial_slots(kkt_data(P)) + 64;
+    if (P->kqstride >= (int64_t)1 << 31) {
+        delete P;
+        return fail(ctx, PDCS_ERR_UNSUPPORTED, "problem too large for the residual reduction layout");
+    }
+    P->kpart = M.alloc<double>((kKktQuantities + 1) * P->kqstride);
+    if (!P->kpart) {
+        delete P;
+        return fail(ctx, PDCS_ERR_HIP, "device allocation failed");
     }
     P->Kaux = P->K;
     P->kkt_big = M.alloc<double>(P->K.nbig_blocks + 1);
@@ -543,6 +558,7 @@ KktData kkt_data(pdcs_problem *P) {
     D.K = P->K;
     D.rtmp = P->rtmp;
     D.scratch_big = P->kkt_big;
+    D.qstride = P->kqstride;
     return D;
 }
 
@@ -832,10 +848,11 @@ struct Solver {
 
     // graph of `len` attempt slots
     // The fast attempt chain, or the robust one with the large-SOC Newton
-    // stages: chosen per chunk, robust at an epoch start and while the last
-    // chunk had projections that the series could not finish.
+    // stages: chosen per chunk, robust at an epoch start and until two chunks
+    // in a row had no projection that the series could not finish.
     bool robust_next = true;
     int64_t fallbacks_seen = 0;
+    int clean_chunks = 0;
     ConeLayout chain_layout(bool robust) const {
         ConeLayout Kc = P->K;
         Kc.big_stages = (robust && P->K.nsoc_big > 0) ? kBigStages : 0;
@@ -848,7 +865,8 @@ struct Solver {
         if (int r = sync()) return r;
         int64_t nf = 0;
         for (const auto &b : bs) nf += b.n_fallback;
-        robust_next = nf > fallbacks_seen;
+        clean_chunks = nf > fallbacks_seen ? 0 : clean_chunks + 1;
+        robust_next = clean_chunks < 2;
         fallbacks_seen = nf;
         return PDCS_OK;
     }
@@ -927,6 +945,7 @@ struct Solver {
             launch_copy<T>(s, it.xs, it.xm, n); launch_copy<T>(s, it.ys, it.ym, m);
             refresh_gys();
             robust_next = true;
+            clean_chunks = 0;
             primal_pending = true;
             if (prm->use_kkt_restart) restart_threshold(false);
             else restart_threshold(restart_dg_flag);
