@@ -77,11 +77,19 @@ typedef struct {
                                          the device at the first solve of each precision and keep the
                                          fastest; 0: fixed heuristic schedule */
     int32_t reserved;
+    /* infeasibility detection thresholds (rpdhg_alg_gpu_gen.jl:422-425) */
+    double eps_primal_infeasible_low_acc;   /* 1e-12 */
+    double eps_dual_infeasible_low_acc;     /* 1e-12 */
+    double eps_primal_infeasible_high_acc;  /* 1e-16 */
+    double eps_dual_infeasible_high_acc;    /* 1e-16 */
 } pdcs_params;
 
 /* Result of a solve; exit codes as the reference's check_termination_criteria
  * (src/pdcs_gpu/termination.jl:181-260): 0 optimal, 1 iteration limit,
- * 6 time limit, 8 NaN, 9 Inf. */
+ * 2/3 primal infeasible (low/high accuracy), 4/5 dual infeasible (low/high
+ * accuracy), 6 time limit, 8 NaN, 9 Inf.  For 2-5, pobj/dobj are the primal
+ * and dual ray objectives of the certificate and `sequence` names the ray
+ * (0: current iterate, 1: current minus average). */
 typedef struct {
     int32_t exit_code;
     int32_t sequence;        /* 0 = current iterate met the test, 1 = averaged iterate */

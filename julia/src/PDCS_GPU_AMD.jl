@@ -57,6 +57,10 @@ mutable struct PdcsParams
     mixed_switch_tol::Float64
     autotune::Int32
     reserved::Int32
+    eps_primal_infeasible_low_acc::Float64
+    eps_dual_infeasible_low_acc::Float64
+    eps_primal_infeasible_high_acc::Float64
+    eps_dual_infeasible_high_acc::Float64
     PdcsParams() = new()
 end
 
@@ -156,6 +160,10 @@ function rpdhg_gpu_solve(;
         prm.use_duality_gap_restart = use_duality_gap_restart
         prm.duality_gap_restart_freq = duality_gap_restart_freq
         prm.verbose = verbose > 1 ? 1 : 0
+        prm.eps_primal_infeasible_low_acc = eps_primal_infeasible_low_acc
+        prm.eps_dual_infeasible_low_acc = eps_dual_infeasible_low_acc
+        prm.eps_primal_infeasible_high_acc = eps_primal_infeasible_high_acc
+        prm.eps_dual_infeasible_high_acc = eps_dual_infeasible_high_acc
         prm.precision = precision == :mixed ? 1 : 0
         prm.warm_start = warm_start ? 1 : 0
         check(ccall((:pdcs_solve, libpdcs[]), Cint,

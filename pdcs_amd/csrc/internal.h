@@ -8,7 +8,7 @@
 namespace pdcs {
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wavefronts)
-constexpr int kKktQuantities = 13;
+constexpr int kKktQuantities = 15;
 
 // CSR matrix on the device. `val` is float or double depending on the path.
 template <typename T>
@@ -202,6 +202,7 @@ struct KktData {
 };
 struct KktOut {
     double pobj, dobj, atyInf, atyN1, sInf, sN1, d2, dInf, axInf, axN1, p2, pInf, projInf;
+    double xInf, yInf;     // l_inf norms of the unscaled x and y (ray norms of the infeasibility test)
 };
 template <typename T>
 void launch_kkt(hipStream_t s, const KktData &D, const T *x, const T *y, double *part,

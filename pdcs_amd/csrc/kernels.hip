@@ -1343,7 +1343,9 @@ template void launch_convert<double, float>(hipStream_t, double *, const float *
 // ------------------------------------------------------------------ KKT --
 // Partial layout: quantity q of block b at part[q * qs + b], qs = KktData::qstride
 // (sized per problem from its block counts).
-enum { Q_POBJ, Q_ATYN1, Q_DOBJ, Q_SN1, Q_D2, Q_ATYINF, Q_SINF, Q_DINF, Q_AXN1, Q_P2, Q_AXINF, Q_PINF, Q_PROJINF, Q_N };
+enum { Q_POBJ, Q_ATYN1, Q_DOBJ, Q_SN1, Q_D2, Q_ATYINF, Q_SINF, Q_DINF, Q_AXN1, Q_P2, Q_AXINF, Q_PINF, Q_PROJINF,
+       Q_XINF, Q_YINF, Q_N };
+static_assert(Q_N == kKktQuantities, "residual quantity count");
 
 // n side: G'y, dual residual, objective pieces (converge_info_calculation_diagonal!,
 // rpdhg_alg_gpu_gen_scaling.jl:128-140,160-166)
@@ -1354,7 +1356,7 @@ __global__ __launch_bounds__(kBlock) void k_kkt_n(KktData D, const T *__restrict
     const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t j = gid / L;
     const int lane = (int)(gid % L);
-    double s_[5] = {0, 0, 0, 0, 0}, m_[3] = {0, 0, 0};
+    double s_[5] = {0, 0, 0, 0, 0}, m_[4] = {0, 0, 0, 0};
     // row dot with double matrix and T vector
     double at = 0.0;
     if (j < D.n) {
@@ -1366,6 +1368,7 @@ __global__ __launch_bounds__(kBlock) void k_kkt_n(KktData D, const T *__restrict
         const double aty = at * D.Dr[j];              // G' y_raw = Dr .* (Gs' ys)
         const double xr = (double)x[j] / D.Dr[j];
         const double lam = D.c_raw[j] - aty;
+        m_[3] = fabs(xr);
         s_[0] = D.c_raw[j] * xr;
         s_[1] = fabs(aty);
         m_[0] = fabs(aty);
@@ -1381,12 +1384,12 @@ __global__ __launch_bounds__(kBlock) void k_kkt_n(KktData D, const T *__restrict
         m_[2] = fabs(r);
     }
     const int qs[5] = {Q_POBJ, Q_ATYN1, Q_DOBJ, Q_SN1, Q_D2};
-    const int qm[3] = {Q_ATYINF, Q_SINF, Q_DINF};
+    const int qm[4] = {Q_ATYINF, Q_SINF, Q_DINF, Q_XINF};
     for (int q = 0; q < 5; ++q) {
         const double v = block_sum(s_[q], lds);
         if (threadIdx.x == 0) part[qs[q] * D.qstride + bstride + blockIdx.x] = v;
     }
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < 4; ++q) {
         const double v = block_max(m_[q], lds);
         if (threadIdx.x == 0) part[qm[q] * D.qstride + bstride + blockIdx.x] = v;
     }
@@ -1406,10 +1409,11 @@ __global__ __launch_bounds__(kBlock) void k_kkt_m(KktData D, const T *__restrict
         for (int32_t k = s + lane; k < e; k += L) ax += D.G.val[k] * (double)x[D.G.col[k]];
     }
     if (L > 1) ax = group_sum<L>(ax);
-    double s_[3] = {0, 0, 0}, m_[3] = {0, 0, 0};
+    double s_[3] = {0, 0, 0}, m_[4] = {0, 0, 0, 0};
     if (i < D.m && lane == 0) {
         const double axr = ax * D.Dl[i];             // G x_raw = Dl .* (Gs xs)
         const double yr = (double)y[i] / D.Dl[i];
+        m_[3] = fabs(yr);
         s_[0] = D.h_raw[i] * yr;
         s_[1] = fabs(axr);
         m_[0] = fabs(axr);
@@ -1425,12 +1429,12 @@ __global__ __launch_bounds__(kBlock) void k_kkt_m(KktData D, const T *__restrict
         }
     }
     const int qs[3] = {Q_DOBJ, Q_AXN1, Q_P2};
-    const int qm[3] = {Q_AXINF, Q_PINF, Q_PROJINF};
+    const int qm[4] = {Q_AXINF, Q_PINF, Q_PROJINF, Q_YINF};
     for (int q = 0; q < 3; ++q) {
         const double v = block_sum(s_[q], lds);
         if (threadIdx.x == 0) part[qs[q] * D.qstride + (q == 0 ? dobj_off : bstride) + blockIdx.x] = v;
     }
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < 4; ++q) {
         const double v = block_max(m_[q], lds);
         if (threadIdx.x == 0) part[qm[q] * D.qstride + bstride + blockIdx.x] = v;
     }
@@ -1567,7 +1571,8 @@ __global__ __launch_bounds__(kBlock) void k_kkt_big_res(ConeLayout K, const doub
 
 __global__ __launch_bounds__(kBlock) void k_kkt_final(const double *part, KktCounts counts, int64_t qs, KktOut *out) {
     __shared__ double lds[kBlock / 64];
-    const bool ismax[Q_N] = {false, false, false, false, false, true, true, true, false, false, true, true, true};
+    const bool ismax[Q_N] = {false, false, false, false, false, true, true, true, false, false, true, true, true,
+                             true, true};
     double res[Q_N];
     for (int q = 0; q < Q_N; ++q) {
         double v = 0.0;
@@ -1589,6 +1594,8 @@ __global__ __launch_bounds__(kBlock) void k_kkt_final(const double *part, KktCou
         out->p2 = res[Q_P2];
         out->pInf = res[Q_PINF];
         out->projInf = res[Q_PROJINF];
+        out->xInf = res[Q_XINF];
+        out->yInf = res[Q_YINF];
     }
 }
 
@@ -1613,6 +1620,8 @@ void launch_kkt(hipStream_t s, const KktData &D, const T *x, const T *y, double 
     counts[Q_ATYINF] = counts[Q_SINF] = counts[Q_DINF] = (int)bn;
     counts[Q_DOBJ] = (int)(bn + bm);
     counts[Q_AXN1] = counts[Q_AXINF] = (int)bm;
+    counts[Q_XINF] = (int)bn;
+    counts[Q_YINF] = (int)bm;
     const unsigned be = (unsigned)exp_blocks(D.K);
     counts[Q_P2] = counts[Q_PINF] = counts[Q_PROJINF] = (int)(bm + bs1 + bs2 + be);
 #define KN(L) k_kkt_n<T, L><<<bn, kBlock, 0, s>>>(D, x, y, part, 0)

@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <deque>
 #include <vector>
 
 #include "../../include/pdcs.h"
@@ -53,6 +54,7 @@ struct ConvInfo {
     double pobj, dobj, abs_gap, rel_gap;
     double l2_abs_p, l2_rel_p, linf_abs_p, linf_rel_p, linf_rel_p_code;
     double l2_abs_d, l2_rel_d, linf_abs_d, linf_rel_d;
+    double xinf, yinf;   // l_inf norms of the unscaled x and y
 };
 
 }  // namespace
@@ -135,6 +137,10 @@ void pdcs_params_default(pdcs_params *p) {
     p->polish_check_freq = 200;
     p->mixed_switch_tol = 1e-5;
     p->autotune = 1;
+    p->eps_primal_infeasible_low_acc = 1e-12;
+    p->eps_dual_infeasible_low_acc = 1e-12;
+    p->eps_primal_infeasible_high_acc = 1e-16;
+    p->eps_dual_infeasible_high_acc = 1e-16;
 }
 
 int pdcs_ctx_create(int device, pdcs_ctx **out) {
@@ -583,6 +589,8 @@ int conv_info(pdcs_problem *P, const T *x, const T *y, ConvInfo &ci) {
     ci.l2_rel_d = ci.l2_abs_d / (1 + std::max(P->cN1, o.atyN1));
     ci.linf_abs_d = o.dInf;
     ci.linf_rel_d = o.dInf / (1 + std::max(P->cNinf, o.atyInf));
+    ci.xinf = o.xInf;
+    ci.yinf = o.yInf;
     return PDCS_OK;
 }
 
@@ -733,6 +741,88 @@ struct Solver {
     double best_kkt = INFINITY;
     int n_checks = 0;
 
+    // Infeasibility certificates (infeasibility_info_calculation_diagonal!,
+    // rpdhg_alg_gpu_gen_scaling.jl:185-246; criteria termination.jl:96-174).
+    // Ray 0 is the current iterate, ray 1 the current minus the average
+    // iterate.  Their objectives and residuals are the residual-check
+    // quantities of those vectors: the primal ray residual is
+    // l_inf(proj_K(Gx - h) - (Gx - h)) (counted only when c'x < 0), the dual
+    // ray objective h'y + bl'(c - G'y)^+ + bu'(c - G'y)^- and residual
+    // l_inf(proj_Lambda(c - G'y) - (c - G'y)) (counted only when the objective
+    // is positive).  The objective trends take the current iterate's pair at
+    // every check (the reference samples with rand() < 0.25).
+    static constexpr size_t kTrendLen = 20;   // trend_len, src/pdcs_gpu/def_struct.jl:724
+    struct RayInfo {
+        double pobj = 0.0, dobj = 0.0, max_pinf = 1e10, max_dinf = 1e10, pnorm = 0.0, dnorm = 0.0;
+        std::deque<double> ptrend, dtrend;
+        int status = 0;   // 0 continue; 2/3 primal infeasible low/high; 4/5 dual infeasible low/high
+    } rays[2];
+    int ray_seq = 0;
+    // is_monotonically_increasing/decreasing, src/pdcs_gpu/def_rpdhg_gen.jl:831-853
+    static bool monotone(const std::deque<double> &b, bool increasing) {
+        if (b.size() < kTrendLen) return false;
+        for (size_t i = 1; i < kTrendLen; ++i)
+            if (increasing ? b[i] < b[i - 1] : b[i] > b[i - 1]) return false;
+        return true;
+    }
+    static void push_trend(std::deque<double> &b, double v) {
+        b.push_back(v);
+        if (b.size() > kTrendLen) b.pop_front();
+    }
+    static void set_ray(RayInfo &r, const ConvInfo &c, const ConvInfo &cur) {
+        r.pobj = c.pobj;
+        r.dobj = c.dobj;
+        r.max_pinf = r.pobj < 0.0 ? c.linf_abs_p : 1e10;
+        r.max_dinf = r.dobj > 0.0 ? c.linf_abs_d : 1e10;
+        r.pnorm = c.xinf;
+        r.dnorm = c.yinf;
+        push_trend(r.ptrend, cur.pobj);
+        push_trend(r.dtrend, cur.dobj);
+    }
+    // primal_infeasibility_criteria_met, termination.jl:96-133
+    void primal_criteria(RayInfo &r) const {
+        if (r.dobj <= 0.0) return;
+        if ((r.dobj - r.pobj) / (1 + std::fabs(r.dobj) + std::fabs(r.pobj)) > 0.99999 && r.dobj > 0.0 &&
+            r.max_pinf > 1e3 && monotone(r.dtrend, true) && r.max_dinf > 1e4 && r.dobj > 1e14 && r.dnorm > 1e8)
+            r.status = 2;
+        const double res = std::max(r.max_dinf, 1.0) / r.dobj;
+        if (res <= prm->eps_primal_infeasible_low_acc && r.dobj > 0.0) {
+            if (res <= prm->eps_primal_infeasible_high_acc) r.status = 3;
+            if (r.max_dinf > 1e8) r.status = 2;
+            if (monotone(r.dtrend, true)) r.status = 2;
+        }
+    }
+    // dual_infeasibility_criteria_met, termination.jl:138-174
+    void dual_criteria(RayInfo &r) const {
+        if (r.pobj >= 0.0) return;
+        if ((r.pobj - r.dobj) / (1 + std::fabs(r.pobj) + std::fabs(r.dobj)) > 0.99999 && r.pobj < 0.0 &&
+            r.max_dinf > 1e3 && monotone(r.ptrend, false) && r.max_pinf > 1e4 && -r.pobj > 1e14 && r.pnorm > 1e8)
+            r.status = 4;
+        const double res = std::max(r.max_pinf, 1.0) / (-r.pobj);
+        if (res <= prm->eps_dual_infeasible_low_acc && r.pobj < 0.0) {
+            if (res <= prm->eps_dual_infeasible_high_acc) r.status = 5;
+            if (r.max_pinf > 1e8) r.status = 4;
+            if (monotone(r.ptrend, false)) r.status = 4;
+        }
+    }
+    // the infeasibility part of check_termination_criteria (termination.jl:205-244)
+    bool infeasibility_exit(const T *x, const T *y, int64_t iter) {
+        ConvInfo cd{};
+        launch_sub<T>(s, I.tn1, x, I.it.xm, n);
+        launch_sub<T>(s, I.tm1, y, I.it.ym, m);
+        if ((rc = conv_info<T>(P, I.tn1, I.tm1, cd))) return true;
+        set_ray(rays[0], ci[0], ci[0]);
+        set_ray(rays[1], cd, ci[0]);
+        for (int q = 0; q < 2; ++q) {
+            RayInfo &r = rays[q];
+            primal_criteria(r);
+            if ((r.status == 2 && iter > 2000) || (r.status == 3 && iter > 500)) { exit_code = r.status; ray_seq = q; return true; }
+            dual_criteria(r);
+            if ((r.status == 4 && iter > 2000) || (r.status == 5 && iter > 500)) { exit_code = r.status; ray_seq = q; return true; }
+        }
+        return false;
+    }
+
     // exit_condition_check_diagonal! + check_termination_criteria
     bool exit_check(const T *x, const T *y, int64_t iter) {
         const double t0 = now_s();
@@ -761,6 +851,13 @@ struct Solver {
             if (std::isnan(c.rel_gap)) { exit_code = 8; exit_seq = q; return true; }
             if (std::isinf(c.rel_gap)) { exit_code = 9; exit_seq = q; return true; }
             if (opt) { exit_code = 0; exit_seq = q; return true; }
+        }
+        {
+            const double t1 = now_s();
+            const bool infeasible = infeasibility_exit(x, y, iter);
+            t_kkt += now_s() - t1;
+            if (rc) return true;
+            if (infeasible) { exit_seq = 0; return true; }
         }
         if (iter >= prm->max_outer_iter * prm->max_inner_iter) { exit_code = 1; exit_seq = 0; return true; }
         if (now_s() - t_start > time_limit) { exit_code = 6; exit_seq = 0; return true; }
@@ -1038,6 +1135,14 @@ struct Solver {
         return rc;
     }
     double eta_init = 1.0;
+
+    // infeasibility exits report the certificate's ray objectives (termination.jl:208-242)
+    void fill_ray(pdcs_result *r) const {
+        if (exit_code < 2 || exit_code > 5) return;
+        r->pobj = rays[ray_seq].pobj;
+        r->dobj = rays[ray_seq].dobj;
+        r->sequence = ray_seq;
+    }
 };
 
 template <>
@@ -1236,6 +1341,7 @@ int pdcs_solve(pdcs_problem *P, const pdcs_params *prm_in, const double *x0, con
             fill_result(c, res);
             res->exit_code = S32.exit_code;
             res->sequence = S32.exit_seq;
+            S32.fill_ray(res);
             res->iter = iters_total;
             res->restarts = S32.restart_used;
             res->omega = S32.omega;
@@ -1271,6 +1377,7 @@ int pdcs_solve(pdcs_problem *P, const pdcs_params *prm_in, const double *x0, con
     fill_result(c, res);
     res->exit_code = S64.exit_code;
     res->sequence = S64.exit_seq;
+    S64.fill_ray(res);
     res->iter = P->h_st->iter;
     res->restarts = S64.restart_used;
     res->omega = S64.omega;

@@ -323,6 +323,10 @@ def rpdhg_gpu_solve(*, n, m, nb, c, G, h, mGzero, mGnonnegative, socG, rsocG=(),
     prm.use_duality_gap_restart = int(use_duality_gap_restart)
     prm.duality_gap_restart_freq = duality_gap_restart_freq
     prm.verbose = int(verbose > 1)
+    prm.eps_primal_infeasible_low_acc = eps_primal_infeasible_low_acc
+    prm.eps_dual_infeasible_low_acc = eps_dual_infeasible_low_acc
+    prm.eps_primal_infeasible_high_acc = eps_primal_infeasible_high_acc
+    prm.eps_dual_infeasible_high_acc = eps_dual_infeasible_high_acc
     prm.precision = _lib.PDCS_MIXED if precision == "mixed" else _lib.PDCS_FP64
     x0 = y0 = None
     if warm_start:
