@@ -36,7 +36,11 @@ typedef struct pdcs_problem pdcs_problem;
 
 /* Cone layout of the constraint rows (the reference's mGzero, mGnonnegative,
  * socG, rsocG, expG, dual_expG arguments, rpdhg_alg_gpu_gen.jl:1219-1234).
- * Rotated SOC rows (nrsoc > 0) are rejected with PDCS_ERR_UNSUPPORTED. */
+ * Rotated SOC blocks {(u, w, z): 2uw >= ||z||^2, u, w >= 0} follow the SOC
+ * blocks.  pdcs_problem_create_* map each onto an SOC by rotating its first two
+ * rows ((u + w)/sqrt2, (u - w)/sqrt2) of G and h; solutions come back in the
+ * caller's coordinates.  The standalone projections, the batched solver and
+ * the adjoint reject rotated SOC rows (PDCS_ERR_UNSUPPORTED). */
 typedef struct {
     int64_t mzero;
     int64_t mnonneg;

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <string>
 #include <deque>
+#include <map>
 #include <vector>
 
 #include "../../include/pdcs.h"
@@ -68,6 +69,13 @@ struct pdcs_problem {
     std::vector<double> vals, c, h, bl, bu;
     int64_t mzero = 0, mnonneg = 0, nexp = 0, ndexp = 0;
     std::vector<int64_t> soc;
+    // Rotated-SOC blocks (first row of each, in the user's row order).  The
+    // library stores G, h and the iterates with the first two rows (u, w) of
+    // each such block rotated to ((u + w)/sqrt2, (u - w)/sqrt2): that maps
+    // {2uw >= ||z||^2, u, w >= 0} onto the SOC {u' >= ||(w', z)||}, and the
+    // rotation is orthogonal and its own inverse, so residuals, objectives and
+    // G'y are unchanged; y goes back through the same rotation on output.
+    std::vector<int64_t> rsoc_rows;
     // device structure
     int32_t *d_rowptr = nullptr, *d_col = nullptr, *d_trowptr = nullptr, *d_tcol = nullptr, *d_tmap = nullptr;
     double *d_vals = nullptr, *d_tvals = nullptr;     // scaled FP64
@@ -173,14 +181,19 @@ const char *pdcs_last_error(pdcs_ctx *ctx) { return ctx ? ctx->err.c_str() : g_t
 
 namespace {
 
-int validate_cones(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const pdcs_cones *k) {
+int validate_cones(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const pdcs_cones *k, bool allow_rsoc = false) {
     if (!k) return fail(ctx, PDCS_ERR_ARG, "cones is NULL");
-    if (k->nrsoc > 0) return fail(ctx, PDCS_ERR_UNSUPPORTED, "rotated SOC rows are not supported");
+    if (k->nrsoc > 0 && !allow_rsoc) return fail(ctx, PDCS_ERR_UNSUPPORTED, "rotated SOC rows are not supported here");
     if (n <= 0 || m <= 0 || nb < 0 || nb > n) return fail(ctx, PDCS_ERR_ARG, "bad sizes n=%ld m=%ld nb=%ld", (long)n, (long)m, (long)nb);
     int64_t s = k->mzero + k->mnonneg + 3 * (k->nexp + k->ndualexp);
     for (int64_t b = 0; b < k->nsoc; ++b) {
         if (k->soc[b] < 2) return fail(ctx, PDCS_ERR_ARG, "The size of the SOC must be greater than 1");
         s += k->soc[b];
+    }
+    for (int64_t b = 0; b < (allow_rsoc ? k->nrsoc : 0); ++b) {
+        // rpdhg_alg_gpu_gen.jl:91-94
+        if (k->rsoc[b] < 3) return fail(ctx, PDCS_ERR_ARG, "The size of the rotated SOC must be greater than 2");
+        s += k->rsoc[b];
     }
     // rpdhg_alg_gpu_gen.jl:1333-1336
     if (s != m) return fail(ctx, PDCS_ERR_ARG, "m does not match the sum of mGzero, mGnonnegative, socG, rsocG, and (expG + dual_expG) * 3");
@@ -355,11 +368,16 @@ int alloc_iters(pdcs_problem *P) {
 
 KktData kkt_data(pdcs_problem *P);
 
+int build_problem_rotated(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double *c, const int32_t *rowptr,
+                          const int32_t *colidx, const double *vals, const double *h, const double *bl,
+                          const double *bu, const pdcs_cones *cones, pdcs_problem **out);
+
 int build_problem(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double *c, const int32_t *rowptr,
                   const int32_t *colidx, const double *vals, const double *h, const double *bl, const double *bu,
                   const pdcs_cones *cones, pdcs_problem **out) {
-    int rc = validate_cones(ctx, n, m, nb, cones);
+    int rc = validate_cones(ctx, n, m, nb, cones, true);
     if (rc) return rc;
+    if (cones->nrsoc > 0) return build_problem_rotated(ctx, n, m, nb, c, rowptr, colidx, vals, h, bl, bu, cones, out);
     if (!c || !rowptr || !colidx || !vals || !h || (nb > 0 && (!bl || !bu)) || !out)
         return fail(ctx, PDCS_ERR_ARG, "NULL input array");
     const int64_t nnz = rowptr[m];
@@ -551,6 +569,74 @@ int finish_scaling(pdcs_problem *P) {
     for (int64_t i = 0; i < P->m; ++i) d2[i] = d[i] * d[i];
     HIPCHK(P->ctx, copy_sync(P->ctx, P->dsc, d.data(), sizeof(double) * P->m, hipMemcpyHostToDevice));
     HIPCHK(P->ctx, copy_sync(P->ctx, P->dsc2, d2.data(), sizeof(double) * P->m, hipMemcpyHostToDevice));
+    return PDCS_OK;
+}
+
+// In-place rotation of the first two rows of every rotated-SOC block (its own inverse).
+void rsoc_rotate(const pdcs_problem *P, double *y) {
+    const double r = std::sqrt(0.5);
+    for (int64_t r0 : P->rsoc_rows) {
+        const double a = y[r0], b = y[r0 + 1];
+        y[r0] = r * (a + b);
+        y[r0 + 1] = r * (a - b);
+    }
+}
+
+// Rotated-SOC blocks: rotate rows (r0, r0 + 1) of G and h of every block, then
+// build the problem with the blocks as SOCs (they follow the SOC rows, so the
+// SOC list is socG then rsocG).
+int build_problem_rotated(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double *c, const int32_t *rowptr,
+                          const int32_t *colidx, const double *vals, const double *h, const double *bl,
+                          const double *bu, const pdcs_cones *cones, pdcs_problem **out) {
+    if (!rowptr || !colidx || !vals || !h) return fail(ctx, PDCS_ERR_ARG, "NULL input array");
+    std::vector<int64_t> rows;
+    int64_t off = cones->mzero + cones->mnonneg;
+    for (int64_t b = 0; b < cones->nsoc; ++b) off += cones->soc[b];
+    for (int64_t b = 0; b < cones->nrsoc; ++b) {
+        rows.push_back(off);
+        off += cones->rsoc[b];
+    }
+    std::vector<char> first(m, 0);
+    for (int64_t r0 : rows) first[r0] = 1;
+    const double rs = std::sqrt(0.5);
+    std::vector<int32_t> rp(m + 1, 0), ci;
+    std::vector<double> va, hh(h, h + m);
+    ci.reserve(rowptr[m]);
+    va.reserve(rowptr[m]);
+    for (int64_t i = 0; i < m; ++i) {
+        if (first[i]) {
+            // union of the two rows' patterns, in column order
+            std::map<int32_t, std::pair<double, double>> u;
+            for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) u[colidx[k]].first += vals[k];
+            for (int32_t k = rowptr[i + 1]; k < rowptr[i + 2]; ++k) u[colidx[k]].second += vals[k];
+            for (const auto &e : u) { ci.push_back(e.first); va.push_back(rs * (e.second.first + e.second.second)); }
+            rp[i + 1] = (int32_t)ci.size();
+            for (const auto &e : u) { ci.push_back(e.first); va.push_back(rs * (e.second.first - e.second.second)); }
+            rp[i + 2] = (int32_t)ci.size();
+            const double a = h[i], b = h[i + 1];
+            hh[i] = rs * (a + b);
+            hh[i + 1] = rs * (a - b);
+            ++i;
+            continue;
+        }
+        for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) { ci.push_back(colidx[k]); va.push_back(vals[k]); }
+        rp[i + 1] = (int32_t)ci.size();
+    }
+    std::vector<int64_t> soc(cones->soc, cones->soc + cones->nsoc);
+    soc.insert(soc.end(), cones->rsoc, cones->rsoc + cones->nrsoc);
+    pdcs_cones k2 = *cones;
+    k2.soc = soc.data();
+    k2.nsoc = (int64_t)soc.size();
+    k2.rsoc = nullptr;
+    k2.nrsoc = 0;
+    const int rc = build_problem(ctx, n, m, nb, c, rp.data(), ci.data(), va.data(), hh.data(), bl, bu, &k2, out);
+    if (rc) return rc;
+    pdcs_problem *P = *out;
+    P->rsoc_rows = rows;
+    // the README denominators use the caller's h
+    P->hN1 = 0.0;
+    P->hNinf = 0.0;
+    for (int64_t i = 0; i < m; ++i) { P->hN1 += std::fabs(h[i]); P->hNinf = std::max(P->hNinf, std::fabs(h[i])); }
     return PDCS_OK;
 }
 
@@ -1160,8 +1246,10 @@ int init_iterates(pdcs_problem *P, const pdcs_params *prm, const double *x0, con
         std::vector<double> Dr(n), Dl(m);
         HIPCHK(P->ctx, copy_sync(P->ctx, Dr.data(), P->Dr, sizeof(double) * n, hipMemcpyDeviceToHost));
         HIPCHK(P->ctx, copy_sync(P->ctx, Dl.data(), P->Dl, sizeof(double) * m, hipMemcpyDeviceToHost));
+        std::vector<double> y0r(y0, y0 + m);
+        rsoc_rotate(P, y0r.data());
         for (int64_t j = 0; j < n; ++j) hx[j] = (T)(x0[j] * Dr[j]);
-        for (int64_t i = 0; i < m; ++i) hy[i] = (T)(y0[i] * Dl[i]);
+        for (int64_t i = 0; i < m; ++i) hy[i] = (T)(y0r[i] * Dl[i]);
     }
     T *xs[] = {I.it.x, I.it.xs, I.it.xm, I.x0, I.x0l};
     T *ys[] = {I.it.y, I.it.ys, I.it.ym, I.y0, I.y0l};
@@ -1494,7 +1582,10 @@ int pdcs_get_solution(pdcs_problem *P, double *x, double *y, double *s) {
     if (!P) return fail(nullptr, PDCS_ERR_ARG, "problem is NULL");
     if (!P->solved) return fail(P->ctx, PDCS_ERR_STATE, "no solution: call pdcs_solve first");
     if (x) std::memcpy(x, P->sol_x.data(), sizeof(double) * P->n);
-    if (y) std::memcpy(y, P->sol_y.data(), sizeof(double) * P->m);
+    if (y) {
+        std::memcpy(y, P->sol_y.data(), sizeof(double) * P->m);
+        rsoc_rotate(P, y);
+    }
     if (s) {
         // s = c - G'y on the original data
         for (int64_t j = 0; j < P->n; ++j) s[j] = P->c[j];
@@ -1509,7 +1600,8 @@ int pdcs_adjoint(pdcs_problem *P, const double *dL_dx, const double *dL_dy, doub
     if (!P || !dL_dx || !dL_dc || !dL_dh || !dL_dG) return fail(P ? P->ctx : nullptr, PDCS_ERR_ARG, "NULL argument");
     pdcs_ctx *ctx = P->ctx;
     if (!P->solved) return fail(ctx, PDCS_ERR_STATE, "no solution: call pdcs_solve first");
-    if (P->nexp + P->ndexp > 0) return fail(ctx, PDCS_ERR_UNSUPPORTED, "the adjoint supports zero, nonnegative and SOC rows");
+    if (P->nexp + P->ndexp > 0 || !P->rsoc_rows.empty())
+        return fail(ctx, PDCS_ERR_UNSUPPORTED, "the adjoint supports zero, nonnegative and SOC rows");
     const int64_t n = P->n, m = P->m, N = n + m;
     if (N > 16384) return fail(ctx, PDCS_ERR_UNSUPPORTED, "the dense adjoint solve is limited to n + m <= 16384");
     HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -1580,8 +1672,10 @@ int pdcs_evaluate(pdcs_problem *P, const double *x, const double *y, pdcs_result
     std::vector<double> Dr(P->n), Dl(P->m), xs(P->n), ys(P->m);
     HIPCHK(P->ctx, copy_sync(P->ctx, Dr.data(), P->Dr, sizeof(double) * P->n, hipMemcpyDeviceToHost));
     HIPCHK(P->ctx, copy_sync(P->ctx, Dl.data(), P->Dl, sizeof(double) * P->m, hipMemcpyDeviceToHost));
+    std::vector<double> yr(y, y + P->m);
+    rsoc_rotate(P, yr.data());
     for (int64_t j = 0; j < P->n; ++j) xs[j] = x[j] * Dr[j];
-    for (int64_t i = 0; i < P->m; ++i) ys[i] = y[i] * Dl[i];
+    for (int64_t i = 0; i < P->m; ++i) ys[i] = yr[i] * Dl[i];
     HIPCHK(P->ctx, copy_sync(P->ctx, P->f64.tn1, xs.data(), sizeof(double) * P->n, hipMemcpyHostToDevice));
     HIPCHK(P->ctx, copy_sync(P->ctx, P->f64.tm1, ys.data(), sizeof(double) * P->m, hipMemcpyHostToDevice));
     ConvInfo c{};

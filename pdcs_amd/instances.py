@@ -36,6 +36,7 @@ class ConicProblem:
     mzero: int = 0
     mnonneg: int = 0
     soc: list = field(default_factory=list)
+    rsoc: list = field(default_factory=list)   # rotated SOC blocks, after the SOC rows
     nexp: int = 0
     ndexp: int = 0
     name: str = ""

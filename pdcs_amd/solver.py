@@ -109,7 +109,7 @@ class Problem:
     @classmethod
     def from_instance(cls, P, ctx: Context | None = None):
         ctx = ctx or default_context()
-        cones = make_cones(P.mzero, P.mnonneg, P.soc, (), P.nexp, P.ndexp)
+        cones = make_cones(P.mzero, P.mnonneg, P.soc, getattr(P, "rsoc", ()), P.nexp, P.ndexp)
         return cls(ctx, P.n, P.m, P.nb, P.c, P.rowptr, P.colidx, P.vals, P.h, P.bl, P.bu, cones)
 
     def solve(self, params: _lib.Params | None = None, x0=None, y0=None, **kw):
@@ -295,8 +295,6 @@ def rpdhg_gpu_solve(*, n, m, nb, c, G, h, mGzero, mGnonnegative, socG, rsocG=(),
     """
     if len(soc_x) or len(rsoc_x) or exp_x or dual_exp_x:
         raise ValueError("primal-side cones are not supported (the reference's are undefined, def_rpdhg_gen.jl:153)")
-    if len(rsocG):
-        raise ValueError("rotated SOC constraints are not supported")
     if rescaling_method != "ruiz_pock_chambolle" or not use_preconditioner:
         raise ValueError("only rescaling_method=:ruiz_pock_chambolle with the preconditioner is supported")
     if not (use_adaptive_restart and use_adaptive_step_size_weight and use_resolving and use_aggressive
@@ -309,7 +307,7 @@ def rpdhg_gpu_solve(*, n, m, nb, c, G, h, mGzero, mGnonnegative, socG, rsocG=(),
     else:
         rowptr, colidx, vals = G
     ctx = ctx or default_context()
-    cones = make_cones(mGzero, mGnonnegative, socG, (), expG, dual_expG)
+    cones = make_cones(mGzero, mGnonnegative, socG, rsocG, expG, dual_expG)
     prob = Problem(ctx, n, m, nb, c, rowptr, colidx, vals, h, bl, bu, cones)
     prm = _lib.default_params()
     prm.max_outer_iter = max_outer_iter
