@@ -86,6 +86,12 @@ typedef struct {
     double eps_dual_infeasible_low_acc;     /* 1e-12 */
     double eps_primal_infeasible_high_acc;  /* 1e-16 */
     double eps_dual_infeasible_high_acc;    /* 1e-16 */
+    /* outer iteration at which the pessimistic re-solve
+     * (resolving_pessimistic_step!, rpdhg_alg_gpu_gen_scaling.jl:511-638) starts
+     * whatever its triggers (rpdhg_alg_gpu_gen_scaling.jl:811-836) say; 0 = only
+     * on the triggers.  Not a parameter of the reference: it makes the re-solve
+     * reachable on problems that do not trip the triggers. */
+    int64_t pessimistic_at;                 /* 0 */
 } pdcs_params;
 
 /* Result of a solve; exit codes as the reference's check_termination_criteria
@@ -113,6 +119,8 @@ typedef struct {
     double time_restart;     /* seconds spent in restart checks */
     int32_t sched[4];        /* SpMV schedule of the final phase: lanes per row of G, of G',
                                 workgroup cap of G, of G' */
+    int64_t pessimistic_iter;  /* attempts when the pessimistic re-solve started, -1 if it did not */
+    double pessimistic_norm;   /* its power-method estimate of ||G||_2 (scaled G), 0 if it did not run */
 } pdcs_result;
 
 void pdcs_params_default(pdcs_params *p);

@@ -61,6 +61,7 @@ mutable struct PdcsParams
     eps_dual_infeasible_low_acc::Float64
     eps_primal_infeasible_high_acc::Float64
     eps_dual_infeasible_high_acc::Float64
+    pessimistic_at::Int64
     PdcsParams() = new()
 end
 
@@ -88,6 +89,8 @@ mutable struct PdcsResult
     time_kkt::Float64
     time_restart::Float64
     sched::NTuple{4, Int32}
+    pessimistic_iter::Int64
+    pessimistic_norm::Float64
     PdcsResult() = new()
 end
 

@@ -35,7 +35,7 @@ class OracleParams(C.Structure):
                 ("use_duality_gap_restart", C.c_int32), ("use_preconditioner", C.c_int32),
                 ("ruiz_iters", C.c_int32), ("pock_chambolle", C.c_int32),
                 ("warm_start", C.c_int32), ("verbose", C.c_int32), ("bisect_intent", C.c_int32),
-                ("code_primal_denominator", C.c_int32)]
+                ("code_primal_denominator", C.c_int32), ("pessimistic_at", c_i64)]
 
 
 class OracleResult(C.Structure):
@@ -46,7 +46,8 @@ class OracleResult(C.Structure):
                 ("abs_dual_res", C.c_double), ("abs_gap", C.c_double),
                 ("l2_rel_primal_res", C.c_double), ("l2_rel_dual_res", C.c_double),
                 ("rel_primal_res_code", C.c_double), ("omega", C.c_double),
-                ("time_solve", C.c_double), ("pessimistic_flag", C.c_int32)]
+                ("time_solve", C.c_double), ("pessimistic_flag", C.c_int32),
+                ("pessimistic_norm", C.c_double), ("pessimistic_iter", c_i64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -11,9 +11,9 @@
  * Deliberate deviations (documented, all deterministic):
  *   - the wall-clock `% 3179` extra checks and the rand()-fed infeasibility
  *     trend buffers (rpdhg_alg_gpu_gen_scaling.jl:241,598,717) are dropped;
- *   - infeasibility certificates and the pessimistic re-solve
- *     (rpdhg_alg_gpu_gen_scaling.jl:511-638) are not restated: a trigger is
- *     reported in oracle_result.pessimistic_flag instead.
+ *   - infeasibility certificates are not restated;
+ *   - the power method of the pessimistic re-solve starts from a fixed
+ *     pseudo-random vector (the reference draws rand(n)).
  */
 #include "pdcs_oracle.h"
 
@@ -910,6 +910,118 @@ static void reflection(solver *S, double extra, int64_t inner, double eta_cum, d
     }
 }
 
+/* Deterministic stand-in for rand(n): uniform on (0,1) from the splitmix64
+ * finaliser of the index (Steele, Lea, Flood, OOPSLA 2014). */
+static double unit_uniform(uint64_t j) {
+    uint64_t z = j + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+/* power_method!, src/pdcs_gpu/def_rpdhg_gen.jl:891-917: sqrt of the largest
+ * eigenvalue of G'G for the scaled G; stops when the eigenvalue estimate moves
+ * by less than 1e-3, at most 1000 rounds.  *flag = 0 converged, 1 not. */
+static double power_norm(solver *S, int *flag) {
+    const int64_t n = S->n;
+    double *b = S->tn1, *atab = S->tn2, *ab = S->tm1;
+    for (int64_t j = 0; j < n; ++j) b[j] = unit_uniform((uint64_t)j);
+    double nb = nrm2(b, n);
+    for (int64_t j = 0; j < n; ++j) b[j] /= nb;
+    double lam = 0.0, lam_old = 0.0;
+    for (int it = 0; it < 1000; ++it) {
+        Gx(&S->G, b, ab);
+        Gty(&S->G, ab, atab);
+        nb = nrm2(atab, n);
+        if (!(nb > 0.0)) break;   /* G b = 0: no estimate */
+        for (int64_t j = 0; j < n; ++j) b[j] = atab[j] / nb;
+        Gx(&S->G, b, ab);
+        Gty(&S->G, ab, atab);
+        lam = dot(b, atab, n);
+        if (fabs(lam - lam_old) < 1e-3) { *flag = 0; return sqrt(lam); }
+        lam_old = lam;
+    }
+    *flag = 1;
+    return lam > 0.0 ? sqrt(lam) : 1.0;
+}
+
+/* pdhg_one_iter_diagonal_rescaling!, rpdhg_alg_gpu_gen_scaling.jl:250-262:
+ * one constant-step PDHG iteration, projections at abs 1e-16 / rel 1e-12 */
+static void constant_step(solver *S, double tau, double sigma) {
+    const int64_t n = S->n, m = S->m;
+    double *aty = S->tn1, *xbar = S->tn2, *gx = S->tm1;
+    memcpy(S->xl, S->x, sizeof(double) * (size_t)n);
+    Gty(&S->G, S->y, aty);
+    for (int64_t j = 0; j < n; ++j) S->x[j] = (S->x[j] - tau * S->c[j]) + tau * aty[j];
+    proj_primal(S, S->x);
+    for (int64_t j = 0; j < n; ++j) xbar[j] = 2.0 * S->x[j] - S->xl[j];
+    Gx(&S->G, xbar, gx);
+    for (int64_t i = 0; i < m; ++i) S->y[i] -= sigma * (gx[i] - S->h[i]);
+    oracle_proj_dual(S->P, S->y, S->dsc, S->dsc2, S->warm_dual, 1e-16, 1e-12);
+}
+
+/* average_seq, src/pdcs_gpu/cuda/utils.cu:80-92 */
+static void average_seq(solver *S, int64_t inner) {
+    const double k = (double)inner;
+    for (int64_t j = 0; j < S->n; ++j) S->xm[j] = (S->xm[j] * k + S->x[j]) / (k + 1.0);
+    for (int64_t i = 0; i < S->m; ++i) S->ym[i] = (S->ym[i] * k + S->y[i]) / (k + 1.0);
+}
+
+static double clamp_weight(double w) { return dmin(dmax(w, 0.8), 1.2); }
+
+/* resolving_pessimistic_step!, rpdhg_alg_gpu_gen_scaling.jl:511-638: constant
+ * steps 0.9/||G||_2 (0.8 if the power method did not converge), primal weight
+ * clamped to [0.8, 1.2], uniform averages, the same restart and exit checks
+ * (the exit check first), started from the best iterates.  Always ends the
+ * solve (exit_code set). */
+static void pessimistic_resolve(solver *S, const double *xbest, const double *ybest,
+                                double *xdiff, double *ydiff, oracle_result *res) {
+    const int64_t n = S->n, m = S->m;
+    S->iter_stepsize = 0;
+    int flag = 0;
+    const double lam = power_norm(S, &flag);
+    S->tau = S->sigma = (flag == 0 ? 0.9 : 0.8) / lam;
+    S->omega = clamp_weight(init_primal_weight(S));
+    res->pessimistic_flag = 1;
+    res->pessimistic_norm = lam;
+    res->pessimistic_iter = S->iter;
+    memcpy(S->xm, xbest, sizeof(double) * (size_t)n);
+    memcpy(S->ym, ybest, sizeof(double) * (size_t)m);
+    for (int64_t outer = 1; outer <= S->prm.max_outer_iter; ++outer) {
+        memcpy(S->x0l, S->x0, sizeof(double) * (size_t)n); memcpy(S->y0l, S->y0, sizeof(double) * (size_t)m);
+        memcpy(S->x0, S->xm, sizeof(double) * (size_t)n); memcpy(S->y0, S->ym, sizeof(double) * (size_t)m);
+        memcpy(S->x, S->xm, sizeof(double) * (size_t)n); memcpy(S->y, S->ym, sizeof(double) * (size_t)m);
+        if (S->prm.use_kkt_restart) restart_threshold(S, 0);
+        else restart_threshold(S, S->restart_dg_flag);
+        memcpy(xdiff, S->x, sizeof(double) * (size_t)n); memcpy(ydiff, S->y, sizeof(double) * (size_t)m);
+        for (int64_t inner = 1;; ++inner) {
+            S->iter++;
+            constant_step(S, S->tau / S->omega, S->sigma * S->omega);
+            average_seq(S, inner);
+            /* exit check before the restart checks, as in the main loop: the
+             * reference checks after them and skips the check whenever one
+             * restarts, so a re-solve restarting at every interval never exits */
+            if (S->iter % S->prm.check_terminate_freq == 0) {
+                if (S->prm.verbose)
+                    fprintf(stderr, "[oracle pessimistic] iter %ld\n", (long)S->iter);
+                if (exit_check(S)) return;
+            }
+            int brk = 0;
+            if (S->prm.use_kkt_restart && S->iter % S->prm.kkt_restart_freq == 0)
+                brk = restart_check(S, inner, 0) || outer == 1;
+            if (!brk && S->prm.use_duality_gap_restart && S->iter % S->prm.duality_gap_restart_freq == 0)
+                brk = restart_check(S, inner, S->restart_dg_flag) || outer == 1;
+            if (brk) break;
+        }
+        for (int64_t j = 0; j < n; ++j) xdiff[j] -= S->x[j];
+        for (int64_t i = 0; i < m; ++i) ydiff[i] -= S->y[i];
+        S->omega = clamp_weight(primal_weight_update(S, xdiff, ydiff, S->omega));
+    }
+    S->exit_code = 1;
+    S->exit_seq = 0;
+}
+
 static int check_cones(const oracle_problem *P) {
     int64_t s = P->mzero + P->mnonneg + 3 * (P->nexp + P->ndexp);
     for (int64_t b = 0; b < P->nsoc; ++b) {
@@ -995,9 +1107,22 @@ static int solve_impl(const oracle_problem *P, const oracle_params *prm, const d
     double ginf = nrminf(S->G.vals, S->G.nnz);
     double eta_prime = 1.0 / ginf, eta = 0.0;
     double *xdiff = dalloc(n), *ydiff = dalloc(m);
+    /* x_best / y_best and their l2 residuals, def_struct.jl:846-858 */
+    double *xbest = dalloc(n), *ybest = dalloc(m);
+    memcpy(xbest, S->x, sizeof(double) * (size_t)n); memcpy(ybest, S->y, sizeof(double) * (size_t)m);
+    double pres_best = 1e30, dres_best = 1e30;
     int done = 0;
     S->t_loop = now_s();
     for (int64_t outer = 1; outer <= prm->max_outer_iter && !done; ++outer) {
+        /* update_best_sol!, rpdhg_alg_gpu_gen_scaling.jl:640-649 */
+        if (S->ci[0].l2_rel_p < pres_best) {
+            pres_best = S->ci[0].l2_rel_p;
+            memcpy(xbest, S->x, sizeof(double) * (size_t)n);
+        }
+        if (S->ci[0].l2_rel_d < dres_best) {
+            dres_best = S->ci[0].l2_rel_d;
+            memcpy(ybest, S->y, sizeof(double) * (size_t)m);
+        }
         S->max_kkt = max_l2_kkt(&S->ci[0]);
         S->min_kkt = dmin(dmin(dmin(S->ci[0].l2_rel_p, S->ci[0].l2_rel_d), S->ci[0].rel_gap), S->min_kkt);
         double extra = -0.1 * log10(S->max_kkt) + 0.2;
@@ -1011,7 +1136,11 @@ static int solve_impl(const oracle_problem *P, const oracle_params *prm, const d
         int f2 = outer > 80 && (ratio > 1e10 || ratio < 1e-10);
         int f3 = outer > 10 && S->tau < 1e-6 && S->sigma < 1e-6;
         int f4 = S->iter > 1000000 && S->ci[0].rel_gap > 0.98 && S->ci[0].l2_rel_p > 1.0 && S->ci[0].l2_rel_d > 1.0;
-        if (f1 || f2 || f3 || f4) res->pessimistic_flag = 1;
+        if (f1 || f2 || f3 || f4 || outer == prm->pessimistic_at) {
+            pessimistic_resolve(S, xbest, ybest, xdiff, ydiff, res);
+            done = 1;
+            break;
+        }
         int64_t inner = 0;
         memcpy(xdiff, S->x, sizeof(double) * (size_t)n); memcpy(ydiff, S->y, sizeof(double) * (size_t)m);
         double eta_cum = 0.0;
@@ -1057,7 +1186,7 @@ static int solve_impl(const oracle_problem *P, const oracle_params *prm, const d
     res->time_solve = now_s() - (attempt_limit > 0 ? S->t_loop : S->t_start);
     if (x_out) memcpy(x_out, S->exit_seq ? S->xmr : S->xr, sizeof(double) * (size_t)n);
     if (y_out) memcpy(y_out, S->exit_seq ? S->ymr : S->yr, sizeof(double) * (size_t)m);
-    free(xdiff); free(ydiff);
+    free(xdiff); free(ydiff); free(xbest); free(ybest);
     double *bufs[] = {S->c, S->h, S->bl, S->bu, S->Drp, S->Dlp, S->dsc, S->dsc2, S->x, S->xl, S->xm,
                       S->y, S->yl, S->ym, S->x0, S->y0, S->x0l, S->y0l, S->tn1, S->tn2, S->tn3,
                       S->tm1, S->tm2, S->tm3, S->tm4, S->h1, S->h2, S->xr, S->yr, S->xmr, S->ymr,

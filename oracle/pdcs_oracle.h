@@ -57,6 +57,11 @@ typedef struct {
      * 1+max(||h||,||Gx||,||proj_K(Gx-h)||); 1 = code variant
      * 1+max(||proj_Lambda(c-G'y)||,||h||,||Gx||) (rpdhg_alg_gpu_gen_scaling.jl:153-162). */
     int32_t code_primal_denominator;
+    /* outer iteration at which the pessimistic re-solve
+     * (rpdhg_alg_gpu_gen_scaling.jl:511-638) starts whatever its triggers say;
+     * 0 = only on the triggers (rpdhg_alg_gpu_gen_scaling.jl:811-836).  Not a
+     * parameter of the reference: it makes the re-solve reachable in tests. */
+    int64_t pessimistic_at;
 } oracle_params;
 
 typedef struct {
@@ -71,7 +76,9 @@ typedef struct {
     double rel_primal_res_code;                             /* code-variant denominator */
     double omega;
     double time_solve;
-    int32_t pessimistic_flag;  /* 1 if a pessimistic-resolve trigger fired (not implemented here) */
+    int32_t pessimistic_flag;  /* 1 if the pessimistic re-solve ran */
+    double pessimistic_norm;   /* its power-method estimate of ||G||_2 (scaled G), 0 if it did not run */
+    int64_t pessimistic_iter;  /* attempts when it started */
 } oracle_result;
 
 void oracle_default_params(oracle_params *p);

@@ -36,7 +36,8 @@ class Params(C.Structure):
                 ("polish_check_freq", c_i64), ("mixed_switch_tol", C.c_double),
                 ("autotune", c_i32), ("reserved", c_i32),
                 ("eps_primal_infeasible_low_acc", C.c_double), ("eps_dual_infeasible_low_acc", C.c_double),
-                ("eps_primal_infeasible_high_acc", C.c_double), ("eps_dual_infeasible_high_acc", C.c_double)]
+                ("eps_primal_infeasible_high_acc", C.c_double), ("eps_dual_infeasible_high_acc", C.c_double),
+                ("pessimistic_at", c_i64)]
 
 
 class Result(C.Structure):
@@ -48,7 +49,7 @@ class Result(C.Structure):
                 ("rel_primal_res_code", C.c_double), ("omega", C.c_double),
                 ("time_setup", C.c_double), ("time_solve", C.c_double), ("time_fp32", C.c_double),
                 ("time_kkt", C.c_double), ("time_restart", C.c_double),
-                ("sched", c_i32 * 4)]
+                ("sched", c_i32 * 4), ("pessimistic_iter", c_i64), ("pessimistic_norm", C.c_double)]
 
     def as_dict(self):
         out = {}

@@ -173,6 +173,16 @@ void launch_axpy(hipStream_t s, T *out, const T *a, double alpha, const T *b, in
 template <typename T>
 void launch_project_dual(hipStream_t s, const ConeLayout &K, const double *d, const double *d2,
                          T *y);
+// one constant-step PDHG iteration of the pessimistic re-solve on (x, y) with
+// step sizes tau, sigma; xbar is scratch; xm, ym are the running averages
+// updated as the inner-th term
+template <typename T>
+void launch_constant_step(hipStream_t s, const Csr<T> &G, const Csr<T> &Gt, const ProbVec<T> &P,
+                          const ConeLayout &K, const double *d, const double *d2, T *x, T *y, T *xbar,
+                          T *xm, T *ym, double tau, double sigma, int64_t inner);
+// out = a / div
+template <typename T>
+void launch_div(hipStream_t s, T *out, const T *a, double div, int64_t n);
 // in-place projection onto K (unscaled, codes 2/3/5/13/16), FP64
 void launch_project_primal_cone(hipStream_t s, const ConeLayout &K, double *r);
 

@@ -1228,6 +1228,83 @@ void launch_project_dual(hipStream_t s, const ConeLayout &K, const double *d, co
 template void launch_project_dual<double>(hipStream_t, const ConeLayout &, const double *, const double *, double *);
 template void launch_project_dual<float>(hipStream_t, const ConeLayout &, const double *, const double *, float *);
 
+// ------------------------------------------------ pessimistic re-solve --
+// One constant-step PDHG iteration (pdhg_one_iter_diagonal_rescaling!,
+// rpdhg_alg_gpu_gen_scaling.jl:250-262) as two SpMV passes; the running
+// average of average_seq (src/pdcs_gpu/cuda/utils.cu:80-92),
+// mean <- (mean k + v) / (k + 1), is fused into the primal pass for x and
+// runs after the cone projection for y.
+template <typename T, int L>
+__global__ __launch_bounds__(kBlock) void k_cs_primal(Csr<T> Gt, ProbVec<T> P, const T *__restrict__ y,
+                                                      T *__restrict__ x, T *__restrict__ xbar,
+                                                      T *__restrict__ xm, double tau, double k) {
+    const int lane = (int)(threadIdx.x % L);
+    for (int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / L; row < P.n;
+         row += (int64_t)gridDim.x * (kBlock / L)) {
+        const double aty = row_dot<T, L>(Gt, row, lane, y);
+        if (lane == 0) {
+            const double xl = (double)x[row];
+            double v = (xl - tau * (double)P.c[row]) + tau * aty;
+            if (row < P.nb) v = fmin(fmax(v, (double)P.bl[row]), (double)P.bu[row]);   // code 18
+            const T xv = (T)v;
+            x[row] = xv;
+            xbar[row] = (T)(2.0 * (double)xv - xl);
+            xm[row] = (T)(((double)xm[row] * k + (double)xv) / (k + 1.0));
+        }
+    }
+}
+template <typename T, int L>
+__global__ __launch_bounds__(kBlock) void k_cs_dual(Csr<T> G, ProbVec<T> P, const T *__restrict__ xbar,
+                                                    T *__restrict__ y, double sigma) {
+    const int lane = (int)(threadIdx.x % L);
+    for (int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / L; row < P.m;
+         row += (int64_t)gridDim.x * (kBlock / L)) {
+        const double acc = row_dot<T, L>(G, row, lane, xbar);
+        if (lane == 0) y[row] = (T)((double)y[row] - sigma * (acc - (double)P.h[row]));
+    }
+}
+template <typename T>
+__global__ void k_cs_average(T *__restrict__ ym, const T *__restrict__ y, int64_t m, double k) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < m) ym[i] = (T)(((double)ym[i] * k + (double)y[i]) / (k + 1.0));
+}
+template <typename T>
+void launch_constant_step(hipStream_t s, const Csr<T> &G, const Csr<T> &Gt, const ProbVec<T> &P,
+                          const ConeLayout &K, const double *d, const double *d2, T *x, T *y, T *xbar,
+                          T *xm, T *ym, double tau, double sigma, int64_t inner) {
+    const double k = (double)inner;
+    const unsigned bp = attempt_grid(Gt.rows, Gt.lanes, Gt.grid);
+#define KP(L) k_cs_primal<T, L><<<bp, kBlock, 0, s>>>(Gt, P, y, x, xbar, xm, tau, k)
+    PDCS_LANES(Gt.lanes, KP)
+#undef KP
+    const unsigned bd = attempt_grid(G.rows, G.lanes, G.grid);
+#define KD(L) k_cs_dual<T, L><<<bd, kBlock, 0, s>>>(G, P, xbar, y, sigma)
+    PDCS_LANES(G.lanes, KD)
+#undef KD
+    launch_project_dual<T>(s, K, d, d2, y);
+    k_cs_average<T><<<nblocks(P.m), kBlock, 0, s>>>(ym, y, P.m, k);
+}
+template void launch_constant_step<double>(hipStream_t, const Csr<double> &, const Csr<double> &,
+                                           const ProbVec<double> &, const ConeLayout &, const double *,
+                                           const double *, double *, double *, double *, double *, double *,
+                                           double, double, int64_t);
+template void launch_constant_step<float>(hipStream_t, const Csr<float> &, const Csr<float> &,
+                                          const ProbVec<float> &, const ConeLayout &, const double *,
+                                          const double *, float *, float *, float *, float *, float *, double,
+                                          double, int64_t);
+
+template <typename T>
+__global__ void k_div(T *out, const T *a, double div, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) out[i] = (T)((double)a[i] / div);
+}
+template <typename T>
+void launch_div(hipStream_t s, T *out, const T *a, double div, int64_t n) {
+    k_div<T><<<nblocks(n), kBlock, 0, s>>>(out, a, div, n);
+}
+template void launch_div<double>(hipStream_t, double *, const double *, double, int64_t);
+template void launch_div<float>(hipStream_t, float *, const float *, double, int64_t);
+
 // plain SOC projection (soc_proj!, soc_rsoc_proj.jl:4-18), wave per cone
 __global__ __launch_bounds__(kBlock) void k_soc_plain(const int32_t *start, const int32_t *dim, int64_t ncone,
                                                       double *r) {

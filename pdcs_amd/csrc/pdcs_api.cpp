@@ -46,6 +46,7 @@ struct IterSet {
     T *tn1 = nullptr, *tn2 = nullptr, *tm1 = nullptr, *tm2 = nullptr;  // scratch
     T *h1 = nullptr, *h2 = nullptr, *xd = nullptr, *yd = nullptr;   // xd/yd: epoch start, then its difference
     T *rx = nullptr, *ry = nullptr;                                   // restart-check differences
+    T *xb = nullptr, *yb = nullptr;                                   // best iterates (x_best / y_best)
     T *c = nullptr, *h = nullptr, *bl = nullptr, *bu = nullptr;        // scaled data
     T *vals = nullptr, *tvals = nullptr;
     bool ready = false;
@@ -335,6 +336,7 @@ int alloc_iters(pdcs_problem *P) {
     I.tn1 = M.alloc<T>(n); I.tn2 = M.alloc<T>(n); I.tm1 = M.alloc<T>(m); I.tm2 = M.alloc<T>(m);
     I.h1 = M.alloc<T>(n); I.h2 = M.alloc<T>(m); I.xd = M.alloc<T>(n); I.yd = M.alloc<T>(m);
     I.rx = M.alloc<T>(n); I.ry = M.alloc<T>(m);
+    I.xb = M.alloc<T>(n); I.yb = M.alloc<T>(m);
     I.c = M.alloc<T>(n); I.h = M.alloc<T>(m); I.bl = M.alloc<T>(P->nb); I.bu = M.alloc<T>(P->nb);
     if (std::is_same<T, double>::value) {
         I.vals = (T *)P->d_vals;
@@ -346,7 +348,7 @@ int alloc_iters(pdcs_problem *P) {
         launch_convert(P->ctx->stream, (float *)I.vals, P->d_vals, P->nnz);
         launch_convert(P->ctx->stream, (float *)I.tvals, P->d_tvals, P->nnz);
     }
-    if (!I.it.x || !I.h2 || !I.bu || !I.yd || !I.rx || !I.ry || !I.it.g || !I.it.gd)
+    if (!I.it.x || !I.h2 || !I.bu || !I.yd || !I.rx || !I.ry || !I.it.g || !I.it.gd || !I.xb || !I.yb)
         return fail(P->ctx, PDCS_ERR_HIP, "iterate allocation failed");
     // scaled data in precision T: c/Dr, h/Dl, bl*Dr, bu*Dr
     std::vector<double> hDr(P->n), hDl(P->m);
@@ -689,6 +691,16 @@ void fill_result(const ConvInfo &c, pdcs_result *r) {
 }
 
 // ---------------------------------------------------------------- solver --
+// Deterministic stand-in for rand(n): uniform on (0,1) from the splitmix64
+// finaliser of the index (Steele, Lea, Flood, OOPSLA 2014).
+static double unit_uniform(uint64_t j) {
+    uint64_t z = j + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
 template <typename T>
 struct Solver {
     pdcs_problem *P;
@@ -708,7 +720,10 @@ struct Solver {
     int exit_code = 7, exit_seq = 0;
     double t_start = 0.0, time_limit = INFINITY, t_kkt = 0.0, t_restart = 0.0;
     int64_t check_freq = 2000;
-    bool pessimistic = false;
+    // best iterates by l2 relative residual (x_best / y_best, def_struct.jl:846-858)
+    double pres_best = 1e30, dres_best = 1e30;
+    int64_t pess_iter = -1;    // attempts when the pessimistic re-solve started (-1: not run)
+    double pess_norm = 0.0;    // its estimate of ||G||_2
     int rc = PDCS_OK;
 
     Solver(pdcs_problem *P_, const pdcs_params *p_) : P(P_), prm(p_), I(iters<T>(P_)), s(P_->ctx->stream) {
@@ -1118,8 +1133,11 @@ struct Solver {
         h.dec = h.dec_next = std::min(0.9 + 0.1 * std::log10(max_kkt * 1e5), 1.0);
         if ((rc = write_state())) return rc;
         Iter<T> &it = I.it;
+        launch_copy<T>(s, I.xb, it.xs, n);
+        launch_copy<T>(s, I.yb, it.ys, m);
         bool done = false;
         for (int64_t outer = 1; outer <= prm->max_outer_iter && !done; ++outer) {
+            update_best();
             max_kkt = max_l2(ci[0]);
             min_kkt = std::min(std::min(std::min(ci[0].l2_rel_p, ci[0].l2_rel_d), ci[0].rel_gap), min_kkt);
             const double extra = -0.1 * std::log10(max_kkt) + 0.2;
@@ -1134,11 +1152,18 @@ struct Solver {
             else restart_threshold(restart_dg_flag);
             if (rc) return rc;
             {
+                // triggers of the pessimistic re-solve, rpdhg_alg_gpu_gen_scaling.jl:811-836
                 const bool f1 = outer > 10 && max_kkt > 1e5;
                 const double ratio = ci[0].l2_rel_p / ci[0].l2_rel_d;
                 const bool f2 = outer > 80 && (ratio > 1e10 || ratio < 1e-10);
                 const bool f3 = outer > 10 && tau < 1e-6 && sigma < 1e-6;
-                if (f1 || f2 || f3) pessimistic = true;
+                const bool f4 = h.iter > 1000000 && ci[0].rel_gap > 0.98 && ci[0].l2_rel_p > 1.0 &&
+                                ci[0].l2_rel_d > 1.0;
+                if (f1 || f2 || f3 || f4 || outer == prm->pessimistic_at) {
+                    if ((rc = pessimistic_resolve(max_iter_total, stop_tol))) return rc;
+                    done = true;
+                    break;
+                }
             }
             // epoch start: xd/yd hold the epoch's starting point
             launch_copy<T>(s, I.xd, it.xs, n); launch_copy<T>(s, I.yd, it.ys, m);
@@ -1165,20 +1190,7 @@ struct Solver {
                     if (rc) return rc;
                     max_kkt = max_l2(ci[0]);
                     h.dec_next = std::min(0.9 + 0.1 * std::log10(max_kkt * 1e5), 1.0);
-                    if (stop_tol > 0) {
-                        // FP32 phase: hand over to the FP64 polish once the README KKT is
-                        // below stop_tol or improves by less than 10% over a check
-                        // interval (the FP32 rounding floor)
-                        const double kk = std::min(kkt_readme(ci[0]), kkt_readme(ci[1]));
-                        const bool stalled = n_checks >= 2 && kk > 0.9 * best_kkt;
-                        best_kkt = std::min(best_kkt, kk);
-                        n_checks++;
-                        if (kk < stop_tol || stalled) {
-                            exit_code = 10;   // phase switch
-                            done = true;
-                            break;
-                        }
-                    }
+                    if (phase_switch(stop_tol)) { done = true; break; }
                     if (h.iter >= max_iter_total) { exit_code = 1; done = true; break; }
                     if ((rc = write_state())) return rc;
                     launch_continue(s, slot, P->st);
@@ -1195,7 +1207,8 @@ struct Solver {
                         if (restart_check(h.inner, h.iter, false)) brk = true;
                         else if (outer == 1) brk = true;
                     }
-                    if (!brk && prm->use_duality_gap_restart && h.iter % prm->duality_gap_restart_freq == 0) {
+                    if (!brk && prm->use_duality_gap_restart && prm->duality_gap_restart_freq > 0 &&
+                        h.iter % prm->duality_gap_restart_freq == 0) {
                         if (restart_check(h.inner, h.iter, restart_dg_flag)) brk = true;
                         else if (outer == 1) brk = true;
                     }
@@ -1221,6 +1234,144 @@ struct Solver {
         return rc;
     }
     double eta_init = 1.0;
+
+    // FP32 phase of a mixed solve (stop_tol > 0): hand over to the FP64 polish
+    // once the README KKT is below stop_tol or improves by less than 10% over a
+    // check interval (the FP32 rounding floor)
+    bool phase_switch(double stop_tol) {
+        if (stop_tol <= 0) return false;
+        const double kk = std::min(kkt_readme(ci[0]), kkt_readme(ci[1]));
+        const bool stalled = n_checks >= 2 && kk > 0.9 * best_kkt;
+        best_kkt = std::min(best_kkt, kk);
+        n_checks++;
+        if (kk < stop_tol || stalled) {
+            exit_code = 10;
+            return true;
+        }
+        return false;
+    }
+
+    // update_best_sol!, rpdhg_alg_gpu_gen_scaling.jl:640-649: the current
+    // iterate (xs, ys) becomes the best primal / dual iterate when the last
+    // residual evaluation's l2 relative residual improved
+    void update_best() {
+        if (ci[0].l2_rel_p < pres_best) {
+            pres_best = ci[0].l2_rel_p;
+            launch_copy<T>(s, I.xb, I.it.xs, n);
+        }
+        if (ci[0].l2_rel_d < dres_best) {
+            dres_best = ci[0].l2_rel_d;
+            launch_copy<T>(s, I.yb, I.it.ys, m);
+        }
+    }
+
+    // power_method!, src/pdcs_gpu/def_rpdhg_gen.jl:891-917: sqrt of the largest
+    // eigenvalue of G'G (scaled G, precision T, FP64 dots), stopping when the
+    // estimate moves by less than 1e-3, at most 1000 rounds; flag = 0 when it
+    // converged.  The start vector is unit_uniform(j) normalised (the
+    // reference draws rand(n)).
+    int power_norm(double &norm, int &flag) {
+        std::vector<double> u(n);
+        double s2 = 0.0;
+        for (int64_t j = 0; j < n; ++j) { u[j] = unit_uniform((uint64_t)j); s2 += u[j] * u[j]; }
+        const double un = std::sqrt(s2);
+        std::vector<T> hb(n);
+        for (int64_t j = 0; j < n; ++j) hb[j] = (T)(u[j] / un);
+        HIPCHK(P->ctx, copy_sync(P->ctx, I.tn1, hb.data(), sizeof(T) * n, hipMemcpyHostToDevice));
+        double lam = 0.0, lam_old = 0.0;
+        flag = 1;
+        for (int r = 0; r < 1000; ++r) {
+            launch_spmv<T>(s, G, I.tn1, I.tm1);
+            launch_spmv<T>(s, Gt, I.tm1, I.tn2);
+            const double nb = nrm2(I.tn2, n);
+            if (!(nb > 0.0)) break;   // G b = 0: no estimate
+            launch_div<T>(s, I.tn1, I.tn2, nb, n);
+            launch_spmv<T>(s, G, I.tn1, I.tm1);
+            launch_spmv<T>(s, Gt, I.tm1, I.tn2);
+            const T *A[1] = {I.tn1};
+            const T *B[1] = {I.tn2};
+            double o[1];
+            if (int e = dots<1>(A, B, n, o)) return e;
+            lam = o[0];
+            if (std::fabs(lam - lam_old) < 1e-3) { flag = 0; break; }
+            lam_old = lam;
+        }
+        norm = lam > 0.0 ? std::sqrt(lam) : 1.0;
+        return sync();
+    }
+
+    static double clamp_weight(double w) { return std::min(std::max(w, 0.8), 1.2); }
+
+    // resolving_pessimistic_step!, rpdhg_alg_gpu_gen_scaling.jl:511-638: plain
+    // PDHG with the constant steps 0.9 / ||G||_2 (0.8 when the power method did
+    // not converge) and the primal weight clamped to [0.8, 1.2], uniform
+    // averages, the main loop's restart and exit checks (the exit check first),
+    // started from the best iterates.  The current iterate is (xs, ys), as in the restart checks.  It
+    // ends the solve: exit_code is set (or rc on an error); the final iterate is
+    // left in (x, y) and the attempt count in the device state.
+    int pessimistic_resolve(int64_t max_iter_total, double stop_tol) {
+        Iter<T> &it = I.it;
+        StepState &h = *P->h_st;
+        if ((rc = read_state())) return rc;
+        int64_t iter = h.iter;
+        pess_iter = iter;
+        int flag = 1;
+        double lam = 1.0;
+        if ((rc = power_norm(lam, flag))) return rc;
+        pess_norm = lam;
+        tau = sigma = (flag == 0 ? 0.9 : 0.8) / lam;
+        omega = clamp_weight(init_primal_weight());
+        if (prm->verbose)
+            fprintf(stderr, "[pdcs %s] pessimistic re-solve at iter %ld: ||G||_2 ~ %.6e (flag %d), omega %.4f\n",
+                    sizeof(T) == 4 ? "fp32" : "fp64", (long)iter, lam, flag, omega);
+        launch_copy<T>(s, it.xm, I.xb, n);
+        launch_copy<T>(s, it.ym, I.yb, m);
+        bool done = false;
+        exit_code = 1;
+        exit_seq = 0;
+        for (int64_t outer = 1; outer <= prm->max_outer_iter && !done; ++outer) {
+            launch_copy<T>(s, I.x0l, I.x0, n); launch_copy<T>(s, I.y0l, I.y0, m);
+            launch_copy<T>(s, I.x0, it.xm, n); launch_copy<T>(s, I.y0, it.ym, m);
+            launch_copy<T>(s, it.xs, it.xm, n); launch_copy<T>(s, it.ys, it.ym, m);
+            if (prm->use_kkt_restart) restart_threshold(false);
+            else restart_threshold(restart_dg_flag);
+            if (rc) return rc;
+            launch_copy<T>(s, I.xd, it.xs, n); launch_copy<T>(s, I.yd, it.ys, m);
+            for (int64_t inner = 1;; ++inner) {
+                ++iter;
+                launch_constant_step<T>(s, G, Gt, PV, P->K, P->dsc, P->dsc2, it.xs, it.ys, it.xbar, it.xm, it.ym,
+                                        tau / omega, sigma * omega, inner);
+                // the exit check comes before the restart checks, as in the main
+                // loop: the reference checks after them and skips the check
+                // whenever one restarts, so a re-solve that restarts at every
+                // interval would never exit
+                if (iter % check_freq == 0) {
+                    const bool ex = exit_check(it.xs, it.ys, iter);
+                    if (rc) return rc;
+                    if (ex || phase_switch(stop_tol)) { done = true; break; }
+                    if (iter >= max_iter_total) { exit_code = 1; exit_seq = 0; done = true; break; }
+                }
+                bool brk = false;
+                if (prm->use_kkt_restart && prm->kkt_restart_freq > 0 && iter % prm->kkt_restart_freq == 0)
+                    brk = restart_check(inner, iter, false) || outer == 1;
+                if (!brk && prm->use_duality_gap_restart && prm->duality_gap_restart_freq > 0 &&
+                    iter % prm->duality_gap_restart_freq == 0)
+                    brk = restart_check(inner, iter, restart_dg_flag) || outer == 1;
+                if (rc) return rc;
+                if (brk) break;
+            }
+            if (done) break;
+            launch_sub<T>(s, I.xd, I.xd, it.xs, n);
+            launch_sub<T>(s, I.yd, I.yd, it.ys, m);
+            omega = clamp_weight(primal_weight_update(omega));
+        }
+        launch_copy<T>(s, it.x, it.xs, n);
+        launch_copy<T>(s, it.y, it.ys, m);
+        if ((rc = read_state())) return rc;
+        h.iter = iter;
+        h.omega = omega;
+        return rc = write_state();
+    }
 
     // infeasibility exits report the certificate's ray objectives (termination.jl:208-242)
     void fill_ray(pdcs_result *r) const {
@@ -1370,6 +1521,7 @@ int pdcs_solve(pdcs_problem *P, const pdcs_params *prm_in, const double *x0, con
     pdcs_ctx *ctx = P->ctx;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     std::memset(res, 0, sizeof(*res));
+    res->pessimistic_iter = -1;
     const double t_setup = now_s();
     int rc;
     // attempt partial buffers (per-block sums of the attempt chain)
@@ -1434,6 +1586,8 @@ int pdcs_solve(pdcs_problem *P, const pdcs_params *prm_in, const double *x0, con
             res->restarts = S32.restart_used;
             res->omega = S32.omega;
             res->time_solve = now_s() - t0;
+            res->pessimistic_iter = S32.pess_iter;
+            res->pessimistic_norm = S32.pess_norm;
             fill_sched(P->sched[1], res);
             store_solution<float>(P, S32.exit_seq ? P->f32.it.xm : P->f32.it.x, S32.exit_seq ? P->f32.it.ym : P->f32.it.y);
             return PDCS_OK;
@@ -1454,6 +1608,8 @@ int pdcs_solve(pdcs_problem *P, const pdcs_params *prm_in, const double *x0, con
             launch_copy<double>(ctx->stream, P->f64.it.ym, P->f64.it.y, P->m);
         }
         polish_base = iters_total;
+        res->pessimistic_iter = S32.pess_iter;
+        res->pessimistic_norm = S32.pess_norm;
     } else {
         if ((rc = init_iterates<double>(P, &prm, x0, y0))) return rc;
         if ((rc = conv_info<double>(P, P->f64.it.xs, P->f64.it.ys, S64.ci[0]))) return rc;
@@ -1470,6 +1626,10 @@ int pdcs_solve(pdcs_problem *P, const pdcs_params *prm_in, const double *x0, con
     res->restarts = S64.restart_used;
     res->omega = S64.omega;
     res->time_solve = now_s() - t0;
+    if (S64.pess_iter >= 0) {
+        res->pessimistic_iter = S64.pess_iter;
+        res->pessimistic_norm = S64.pess_norm;
+    }
     fill_sched(P->sched[0], res);
     res->time_kkt += S64.t_kkt;
     res->time_restart += S64.t_restart;

@@ -40,8 +40,8 @@ def test_params_default_matches_reference_defaults():
 def test_struct_sizes_match_header_layout():
     # pdcs_params / pdcs_result / pdcs_cones as laid out by a C compiler
     assert C.sizeof(_lib.Cones) == 8 * 8
-    assert C.sizeof(_lib.Params) == 8 * 5 + 8 * 3 + 4 * 10 + 8 * 3 + 4 * 2 + 8 * 4
-    assert C.sizeof(_lib.Result) == 4 * 2 + 8 * 3 + 8 * 17 + 4 * 4
+    assert C.sizeof(_lib.Params) == 8 * 5 + 8 * 3 + 4 * 10 + 8 * 3 + 4 * 2 + 8 * 4 + 8
+    assert C.sizeof(_lib.Result) == 4 * 2 + 8 * 3 + 8 * 17 + 4 * 4 + 8 * 2
 
 
 def test_ctx_create_without_device_fails_cleanly():

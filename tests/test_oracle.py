@@ -210,6 +210,30 @@ def test_oracle_lp_vs_highs():
         assert abs(res["pobj"] - lp.fun) <= 2e-6 * (1 + abs(lp.fun)), (res["pobj"], lp.fun)
 
 
+@pytest.mark.parametrize("kkt_restart", [0, 1])
+def test_oracle_pessimistic_resolve_vs_highs(kkt_restart):
+    """The pessimistic re-solve, forced at the second outer iteration, reaches
+    the HiGHS optimum; its power-method norm is a Rayleigh quotient of the
+    scaled matrix, so it lies below the largest singular value (numpy SVD) and,
+    with the reference's 1e-3 stopping rule, within 10% of it."""
+    import scipy.sparse as sp
+    from scipy.optimize import linprog
+
+    for seed in (1, 2):   # seeds whose first epoch does not finish the solve
+        P = instances.lp_box(40, 8, 25, seed=seed)
+        A = P.dense()
+        lp = linprog(P.c, A_ub=-A[8:], b_ub=-P.h[8:], A_eq=A[:8], b_eq=P.h[:8],
+                     bounds=list(zip(P.bl, P.bu)), method="highs")
+        assert lp.status == 0
+        res, x, y = oracle.solve(P, time_limit=60.0, pessimistic_at=2, use_kkt_restart=kkt_restart)
+        assert res["pessimistic_flag"] == 1 and res["pessimistic_iter"] == 2000
+        assert res["exit_code"] == 0
+        assert abs(res["pobj"] - lp.fun) <= 2e-6 * (1 + abs(lp.fun)), (res["pobj"], lp.fun)
+        _, _, vals = oracle.scale(P)
+        sv = np.linalg.norm(sp.csr_matrix((vals, P.colidx, P.rowptr), shape=(P.m, P.n)).toarray(), 2)
+        assert 0.9 * sv <= res["pessimistic_norm"] <= sv * (1 + 1e-12), (res["pessimistic_norm"], sv)
+
+
 def test_oracle_workload0_weak_duality():
     P = instances.workload0()
     res, x, y = oracle.solve(P, time_limit=120.0)
