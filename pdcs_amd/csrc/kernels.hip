@@ -31,12 +31,24 @@
 
 namespace pdcs {
 
+// Row dot product over L lanes.  Entries are taken two per lane per step, so
+// the index, value and gather loads of both are in flight together (one
+// dependent-load chain per pair instead of one per entry); the products are
+// accumulated in entry order, as in a one-per-step loop.
 template <typename T, int L>
 __device__ __forceinline__ double row_dot(const Csr<T> &A, int64_t row, int lane, const T *x) {
     double acc = 0.0;
     if (row < A.rows) {
         const int32_t s = A.rowptr[row], e = A.rowptr[row + 1];
-        for (int32_t k = s + lane; k < e; k += L) acc += (double)A.val[k] * (double)x[A.col[k]];
+        int32_t k = s + lane;
+        for (; k + L < e; k += 2 * L) {
+            const int32_t c0 = A.col[k], c1 = A.col[k + L];
+            const T v0 = A.val[k], v1 = A.val[k + L];
+            const T x0 = x[c0], x1 = x[c1];
+            acc += (double)v0 * (double)x0;
+            acc += (double)v1 * (double)x1;
+        }
+        if (k < e) acc += (double)A.val[k] * (double)x[A.col[k]];
     }
     if (L > 1) acc = group_sum<L>(acc);
     return acc;
@@ -239,16 +251,22 @@ __global__ __launch_bounds__(kBlock) void k_dual(Csr<T> G, ProbVec<T> P, ConeLay
                                                  T *__restrict__ y, T *__restrict__ dy, StepState *st,
                                                  int slot, double *__restrict__ part) {
     __shared__ double lds[kBlock / 64];
-    if (halted(st, slot)) return;
-    const double ds = st->eta * st->omega * st->dec;  // dual_step_size, scaling.jl:698-700
+    // The halt flag and the step scalars are requested with the first row's
+    // loads and tested before the first store (reads of a halted attempt are
+    // harmless; the flag is uniform over the grid).
+    const double eta = st->eta, omega = st->omega, dec = st->dec;
+    const bool halt = halted(st, slot);
+    const double ds = eta * omega * dec;               // dual_step_size, scaling.jl:698-700
     const int lane = (int)(threadIdx.x % L);
     double d2 = 0.0;
     for (int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / L; row < P.m;
          row += (int64_t)gridDim.x * (kBlock / L)) {
+    // row operands first, so their loads overlap the dot product's chain
+    const double yl = (double)ys[row], hv = (double)P.h[row];
     const double acc = row_dot<T, L>(G, row, lane, xbar);
+    if (halt) return;
     if (lane == 0) {
-        const double yl = (double)ys[row];
-        double v = yl - ds * (acc - (double)P.h[row]);             // dual_update, utils.cu:28
+        double v = yl - ds * (acc - hv);                              // dual_update, utils.cu:28
         if (row < K.cone_begin) {
             if (row >= K.mzero) v = fmax(v, 0.0);                   // code 4 (code 1: free)
             const T yv = (T)v;
@@ -261,6 +279,7 @@ __global__ __launch_bounds__(kBlock) void k_dual(Csr<T> G, ProbVec<T> P, ConeLay
         }
     }
     }
+    if (halt) return;
     d2 = block_sum(d2, lds);
     if (threadIdx.x == 0) part[blockIdx.x] = d2;
 }
@@ -344,7 +363,9 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_bigsoc_moments(ConeLayout K, const double *__restrict__ d2,
                                                            const T *__restrict__ y, StepState *st, int slot) {
     __shared__ double lds[(kBlock / 64) * kBigSums];
-    if (st != nullptr && halted(st, slot)) return;
+    // the halt flag is tested before the first store, so its load overlaps the
+    // block's (harmless when halted, the flag is uniform over the grid)
+    const bool halt = st != nullptr && halted(st, slot);
     const int b = blockIdx.x;
     const int cone = K.big_blk_cone[b];
     const int64_t st0 = K.soc_big_start[cone];
@@ -356,6 +377,7 @@ __global__ __launch_bounds__(kBlock) void k_bigsoc_moments(ConeLayout K, const d
     big_init(v, ismax);
     const int e0 = K.big_blk_first[b], e1 = big_blk_end(K, b, k);
     big_block_loop(y, d2, st0, e0, e1, [&](double yv, double dv) { big_accumulate(v, yv, dv, s, t0, true); });
+    if (halt) return;
     block_reduce_multi<kBigSums>(v, ismax, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
@@ -399,23 +421,37 @@ __device__ __forceinline__ double big_point(int p, double tr, double t0, double 
 
 // projection of this workgroup's block of a big cone with multiplier lam
 // (case cs); the head when the block is the cone's first; returns sum dy^2
+// One thread's kBigUnroll tail elements of a block (y, d^2, ys), loaded together.
+struct BigChunk {
+    double y[kBigUnroll], d[kBigUnroll], s[kBigUnroll];
+};
+template <typename T>
+__device__ __forceinline__ void big_load_chunk(BigChunk &c, const T *__restrict__ y, const double *__restrict__ d2,
+                                               const T *__restrict__ ys, bool with_ys, int64_t st0, int eb, int e1) {
+#pragma unroll
+    for (int u = 0; u < kBigUnroll; ++u) {
+        const int e = eb + u * kBlock;
+        const int64_t i = st0 + 1 + e;
+        const bool in = e < e1;
+        c.y[u] = in ? (double)y[i] : 0.0;
+        c.d[u] = in ? d2[i] : 1.0;
+        c.s[u] = (in && with_ys) ? (double)ys[i] : 0.0;
+    }
+}
+
+// `first`, when given, holds the thread's first chunk (loaded by the caller
+// before the cone's decision)
 template <typename T>
 __device__ __forceinline__ double big_apply_block(const ConeLayout &K, const double *__restrict__ d2,
                                                   T *__restrict__ y, const T *__restrict__ ys,
                                                   T *__restrict__ dy, int64_t st0, int e0, int e1, bool head,
-                                                  int cs, double lam, double th) {
+                                                  int cs, double lam, double th, const BigChunk *first = nullptr) {
     double my2 = 0.0;
     for (int eb = e0 + (int)threadIdx.x; eb < e1; eb += kBlock * kBigUnroll) {
-        double yv[kBigUnroll], dv[kBigUnroll], sv[kBigUnroll];
-#pragma unroll
-        for (int u = 0; u < kBigUnroll; ++u) {
-            const int e = eb + u * kBlock;
-            const int64_t i = st0 + 1 + e;
-            const bool in = e < e1;
-            yv[u] = in ? (double)y[i] : 0.0;
-            dv[u] = in ? d2[i] : 1.0;
-            sv[u] = (in && dy != nullptr) ? (double)ys[i] : 0.0;
-        }
+        BigChunk c;
+        if (first != nullptr && eb == e0 + (int)threadIdx.x) c = *first;
+        else big_load_chunk(c, y, d2, ys, dy != nullptr, st0, eb, e1);
+        const double *yv = c.y, *dv = c.d, *sv = c.s;
 #pragma unroll
         for (int u = 0; u < kBigUnroll; ++u) {
             const int e = eb + u * kBlock;
@@ -529,13 +565,18 @@ __global__ __launch_bounds__(kBlock) void k_bigsoc_apply(ConeLayout K, const dou
     __shared__ double sh_t, sh_lam, sh_th, sh_tL, sh_tR;
     __shared__ int sh_cs, sh_done;
     __shared__ BigSocState L1;
-    if (st != nullptr && halted(st, slot)) return;
+    // halt flag tested before the first store (as in k_bigsoc_moments); the
+    // block's first chunk is loaded before the decision, which it overlaps
+    const bool halt = st != nullptr && halted(st, slot);
     const int b = blockIdx.x;
     const int cone = K.big_blk_cone[b];
     BigSocState *S = K.big_state + cone;
     const int64_t st0 = K.soc_big_start[cone];
     const int k = K.soc_big_dim[cone] - 1;
     const int b0 = K.big_cone_blk0[cone], np = K.big_cone_nblk[cone];
+    const int e0 = K.big_blk_first[b], e1 = big_blk_end(K, b, k);
+    BigChunk first;
+    big_load_chunk(first, y, d2, ys, dy != nullptr, st0, e0 + (int)threadIdx.x, e1);
     const double s = S->s_in, t0 = S->t0_in;
     double v[kBigSums];
     bool ismax[kBigSums];
@@ -547,6 +588,7 @@ __global__ __launch_bounds__(kBlock) void k_bigsoc_apply(ConeLayout K, const dou
             v[q] = ismax[q] ? fmax(v[q], pv) : v[q] + pv;
         }
     }
+    if (halt) return;
     block_reduce_multi<kBigSums>(v, ismax, lds);
     if (threadIdx.x == 0) {
         BigSocState L{};
@@ -578,10 +620,9 @@ __global__ __launch_bounds__(kBlock) void k_bigsoc_apply(ConeLayout K, const dou
         }
     }
     __syncthreads();
-    const int e0 = K.big_blk_first[b], e1 = big_blk_end(K, b, k);
     double my2 = 0.0;
     if (sh_done) {
-        my2 = big_apply_block(K, d2, y, ys, dy, st0, e0, e1, b == b0, sh_cs, sh_lam, sh_th);
+        my2 = big_apply_block(K, d2, y, ys, dy, st0, e0, e1, b == b0, sh_cs, sh_lam, sh_th, &first);
     } else if (K.big_stages == 0) {
         if (b == b0) {
             big_fallback(K, S, L1, d2, y, ys, dy, st0, k, s, sh_t, b0, part, lds);
@@ -809,17 +850,21 @@ __global__ __launch_bounds__(kBlock) void k_atdy(Csr<T> Gt, int64_t n, const T *
                                                  double *__restrict__ gd, StepState *st, int slot,
                                                  double *__restrict__ part) {
     __shared__ double lds[kBlock / 64];
-    if (halted(st, slot)) return;
+    // halt flag tested before the first store, as in k_dual
+    const bool halt = halted(st, slot);
     const int lane = (int)(threadIdx.x % L);
     double v = 0.0;
     for (int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / L; row < n;
          row += (int64_t)gridDim.x * (kBlock / L)) {
+        const double dx = (double)x[row] - (double)xs[row];   // loads overlap the dot product
         const double acc = row_dot<T, L>(Gt, row, lane, dy);
+        if (halt) return;
         if (lane == 0) {
             gd[row] = acc;
-            v += ((double)x[row] - (double)xs[row]) * acc;
+            v += dx * acc;
         }
     }
+    if (halt) return;
     v = block_sum(v, lds);
     if (threadIdx.x == 0) part[blockIdx.x] = v;
 }
@@ -827,78 +872,102 @@ __global__ __launch_bounds__(kBlock) void k_atdy(Csr<T> Gt, int64_t n, const T *
 // Post-decision logic of an attempt (the accept branch of
 // pdhg_one_iter_..._adaptive_step_size! and the inner loop of
 // pdhg_main_iter_..._aggressive!, rpdhg_alg_gpu_gen_scaling.jl:746-767,843-860).
-__device__ void step_continue(StepState *st, int slot) {
-    const bool accept = st->accepted != 0;
-    const bool force = st->force != 0;
+__device__ __forceinline__ void step_continue(StepState &S, int slot) {
+    const bool accept = S.accepted != 0;
+    const bool force = S.force != 0;
     if (accept || force) {
         // force: the series returns eta = eta_prime (scaling.jl:752-765)
-        const double eta_acc = fmax(force && !accept ? st->eta_prime : st->eta_try, 1e-30);
-        st->reflect_k = (double)(st->inner + 1);
-        st->reflect_eta = eta_acc;
-        st->reflect_cum = st->eta_cum;
-        st->reflect_slot = slot;
-        st->eta_cum += eta_acc;
-        st->inner += 1;
-        st->eta = fmax(st->eta_prime, 1e-6);   // eta = max(eta, 1e-6) at series start, :667
-        st->count = 0;
-        st->dec = st->dec_next;
-        const bool art = ((double)st->inner >= (double)st->iter * 0.223 && st->iter > 10000) ||
-                         (force && st->inner > 100);
-        const bool rc = (st->restart_freq > 0 && st->iter % st->restart_freq == 0) ||
-                        (st->kkt_restart_freq > 0 && st->iter % st->kkt_restart_freq == 0);
-        if (art) { st->reason = force ? 8 : 4; st->stop_at = slot + 1; }
-        else if (rc) { st->reason = 2; st->stop_at = slot + 1; }
+        const double eta_acc = fmax(force && !accept ? S.eta_prime : S.eta_try, 1e-30);
+        S.reflect_k = (double)(S.inner + 1);
+        S.reflect_eta = eta_acc;
+        S.reflect_cum = S.eta_cum;
+        S.reflect_slot = slot;
+        S.eta_cum += eta_acc;
+        S.inner += 1;
+        S.eta = fmax(S.eta_prime, 1e-6);   // eta = max(eta, 1e-6) at series start, :667
+        S.count = 0;
+        S.dec = S.dec_next;
+        const bool art = ((double)S.inner >= (double)S.iter * 0.223 && S.iter > 10000) ||
+                         (force && S.inner > 100);
+        const bool rc = (S.restart_freq > 0 && S.iter % S.restart_freq == 0) ||
+                        (S.kkt_restart_freq > 0 && S.iter % S.kkt_restart_freq == 0);
+        if (art) { S.reason = force ? 8 : 4; S.stop_at = slot + 1; }
+        else if (rc) { S.reason = 2; S.stop_at = slot + 1; }
     } else {
-        st->eta = st->eta_prime;
-        st->reflect_slot = -1;
+        S.eta = S.eta_prime;
+        S.reflect_slot = -1;
     }
 }
 
+__device__ __forceinline__ void step_decide(StepState &S, int slot, double a, double b, double c);
+
+// The step decision works on a register copy of the state: the copy, the halt
+// flag and the partial sums are requested together (one memory round trip),
+// and the state is written back whole at the end.
 __global__ __launch_bounds__(1024) void k_step(StepState *st, int slot, const double *__restrict__ part,
                                                int64_t stride, int n1, int n2, int n3) {
-    __shared__ double lds[1024 / 64];
-    if (halted(st, slot)) return;
+    __shared__ double lds[3][1024 / 64];
+    StepState S = *st;
+    const bool halt = halted(st, slot);
     double a = 0.0, b = 0.0, c = 0.0;
     for (int i = threadIdx.x; i < n1; i += 1024) a += part[i];
     for (int i = threadIdx.x; i < n2; i += 1024) b += part[stride + i];
     for (int i = threadIdx.x; i < n3; i += 1024) c += part[2 * stride + i];
-    a = block_sum(a, lds);
-    b = block_sum(b, lds);
-    c = block_sum(c, lds);
+    if (halt) return;
+    // three block sums in one LDS round (each in the order of block_sum)
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { lds[0][w] = a; lds[1][w] = b; lds[2][w] = c; }
+    __syncthreads();
     if (threadIdx.x != 0) return;
-    st->iter += 1;
-    st->iter_stepsize += 1;
-    const double omega = st->omega, eta = st->eta;
-    st->eta_try = eta;
-    st->tau = eta / omega;
-    st->sigma = eta * omega;
-    st->dx2 = a;
-    st->dy2 = b;
-    st->dot = c;
+    a = 0.0; b = 0.0; c = 0.0;
+    for (int i = 0; i < 1024 / 64; ++i) { a += lds[0][i]; b += lds[1][i]; c += lds[2][i]; }
+    step_decide(S, slot, a, b, c);
+    *st = S;
+}
+
+// The scalar part of the step decision (pdhg_one_iter_..._adaptive_step_size!,
+// scaling.jl:682-767) from ||dx||^2, ||dy||^2 and <dx, G'dy>.
+__device__ __forceinline__ void step_decide(StepState &S, int slot, double a, double b, double c) {
+    S.iter += 1;
+    S.iter_stepsize += 1;
+    const double omega = S.omega, eta = S.eta;
+    S.eta_try = eta;
+    S.tau = eta / omega;
+    S.sigma = eta * omega;
+    S.dx2 = a;
+    S.dy2 = b;
+    S.dot = c;
     // eta_bar and eta', scaling.jl:712-718
     const double num = omega * a + b / omega;
     const double den = 2.0 * fabs(c);
     const double eta_bar = num / (den + 1e-20);
-    const double k1 = (double)(st->iter_stepsize + 1);
+    const double k1 = (double)(S.iter_stepsize + 1);
     const double p3 = exp(-0.3 * log(k1));           // k1^-0.3; k1^-0.6 = p3^2
     const double eta_prime = fmin((1.0 - p3) * eta_bar, (1.0 + p3 * p3) * eta);
-    st->eta_bar = eta_bar;
-    st->eta_prime = eta_prime;
+    S.eta_bar = eta_bar;
+    S.eta_prime = eta_prime;
     const bool accept = eta <= eta_bar;
-    st->accepted = accept ? 1 : 0;
-    if (!accept) st->count += 1;
-    st->force = (!accept && st->count > 1000000) ? 1 : 0;
-    if (st->iter % st->check_freq == 0) {
+    S.accepted = accept ? 1 : 0;
+    if (!accept) S.count += 1;
+    S.force = (!accept && S.count > 1000000) ? 1 : 0;
+    if (S.iter % S.check_freq == 0) {
         // exit check is due before the acceptance test: the host decides
-        st->reason = 1;
-        st->stop_at = slot + 1;
-        st->reflect_slot = -1;
-        return;
+        S.reason = 1;
+        S.stop_at = slot + 1;
+        S.reflect_slot = -1;
+    } else {
+        step_continue(S, slot);
     }
-    step_continue(st, slot);
 }
 
-__global__ void k_continue(StepState *st, int slot) { step_continue(st, slot); }
+__global__ void k_continue(StepState *st, int slot) {
+    StepState S = *st;
+    step_continue(S, slot);
+    *st = S;
+}
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_reflect(Iter<T> it, int64_t n, int64_t m, StepState *st, int slot) {
@@ -933,10 +1002,12 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_reflect_primal(Iter<T> it, ProbVec<T> P, StepState *st, int slot,
                                                            int pg, double *__restrict__ part) {
     __shared__ double lds[kBlock / 64];
+    // every state scalar is loaded before the first branch (one round trip)
+    const double k = st->reflect_k, e = st->extra, eta = st->reflect_eta, cum = st->reflect_cum;
+    const double eta_n = st->eta, omega_n = st->omega, dec_n = st->dec;
     const bool refl = st->reflect_slot == slot;
     const bool prim = !halted(st, slot + 1);
     if (!refl && !prim) return;
-    const double k = st->reflect_k, e = st->extra, eta = st->reflect_eta, cum = st->reflect_cum;
     const double tmp = (k + 1.0) / (k + 2.0), tail = 1.0 / (k + 2.0);
     const double ga = tmp * (1.0 + e) + tail;   // ys' = ys + ga dy
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -947,7 +1018,7 @@ __global__ __launch_bounds__(kBlock) void k_reflect_primal(Iter<T> it, ProbVec<T
         it.ym[i] = (T)(((double)it.ym[i] * cum + (double)ny * eta) / (cum + eta));
     }
     if ((int)blockIdx.x >= pg) return;
-    const double ps = st->eta / st->omega * st->dec;   // primal_step_size of slot + 1, scaling.jl:687-689
+    const double ps = eta_n / omega_n * dec_n;   // primal_step_size of slot + 1, scaling.jl:687-689
     double d2 = 0.0;
     for (int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x; row < P.n; row += (int64_t)pg * kBlock) {
         double xl = (double)it.xs[row], gv = it.g[row];
