@@ -54,6 +54,26 @@ __device__ __forceinline__ double row_dot(const Csr<T> &A, int64_t row, int lane
     return acc;
 }
 
+// row_dot for an FP64 matrix and a vector of precision T (residual checks)
+template <typename T, int L>
+__device__ __forceinline__ double row_dot_mixed(const Csr<double> &A, int64_t row, int lane, const T *x) {
+    double acc = 0.0;
+    if (row < A.rows) {
+        const int32_t s = A.rowptr[row], e = A.rowptr[row + 1];
+        int32_t k = s + lane;
+        for (; k + L < e; k += 2 * L) {
+            const int32_t c0 = A.col[k], c1 = A.col[k + L];
+            const double v0 = A.val[k], v1 = A.val[k + L];
+            const T x0 = x[c0], x1 = x[c1];
+            acc += v0 * (double)x0;
+            acc += v1 * (double)x1;
+        }
+        if (k < e) acc += A.val[k] * (double)x[A.col[k]];
+    }
+    if (L > 1) acc = group_sum<L>(acc);
+    return acc;
+}
+
 __device__ __forceinline__ bool halted(const StepState *st, int slot) {
     return slot >= __builtin_nontemporal_load(&st->stop_at);
 }
@@ -1500,18 +1520,12 @@ static_assert(Q_N == kKktQuantities, "residual quantity count");
 template <typename T, int L>
 __global__ __launch_bounds__(kBlock) void k_kkt_n(KktData D, const T *__restrict__ x, const T *__restrict__ y,
                                                   double *__restrict__ part, int64_t bstride) {
-    __shared__ double lds[kBlock / 64];
+    __shared__ double lds[(kBlock / 64) * 9];
     const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t j = gid / L;
     const int lane = (int)(gid % L);
     double s_[5] = {0, 0, 0, 0, 0}, m_[4] = {0, 0, 0, 0};
-    // row dot with double matrix and T vector
-    double at = 0.0;
-    if (j < D.n) {
-        const int32_t s = D.Gt.rowptr[j], e = D.Gt.rowptr[j + 1];
-        for (int32_t k = s + lane; k < e; k += L) at += D.Gt.val[k] * (double)y[D.Gt.col[k]];
-    }
-    if (L > 1) at = group_sum<L>(at);
+    const double at = row_dot_mixed<T, L>(D.Gt, j, lane, y);
     if (j < D.n && lane == 0) {
         const double aty = at * D.Dr[j];              // G' y_raw = Dr .* (Gs' ys)
         const double xr = (double)x[j] / D.Dr[j];
@@ -1531,32 +1545,26 @@ __global__ __launch_bounds__(kBlock) void k_kkt_n(KktData D, const T *__restrict
         s_[4] = r * r;
         m_[2] = fabs(r);
     }
-    const int qs[5] = {Q_POBJ, Q_ATYN1, Q_DOBJ, Q_SN1, Q_D2};
-    const int qm[4] = {Q_ATYINF, Q_SINF, Q_DINF, Q_XINF};
-    for (int q = 0; q < 5; ++q) {
-        const double v = block_sum(s_[q], lds);
-        if (threadIdx.x == 0) part[qs[q] * D.qstride + bstride + blockIdx.x] = v;
-    }
-    for (int q = 0; q < 4; ++q) {
-        const double v = block_max(m_[q], lds);
-        if (threadIdx.x == 0) part[qm[q] * D.qstride + bstride + blockIdx.x] = v;
-    }
+    // one LDS round for all nine (each in block_sum / block_max order; the
+    // maxima are of non-negative values)
+    const int qi[9] = {Q_POBJ, Q_ATYN1, Q_DOBJ, Q_SN1, Q_D2, Q_ATYINF, Q_SINF, Q_DINF, Q_XINF};
+    double v[9] = {s_[0], s_[1], s_[2], s_[3], s_[4], m_[0], m_[1], m_[2], m_[3]};
+    const bool ismax[9] = {false, false, false, false, false, true, true, true, true};
+    block_reduce_multi<9>(v, ismax, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < 9; ++q) part[qi[q] * D.qstride + bstride + blockIdx.x] = v[q];
 }
 
 // m side: G x, h'y, residual of the elementwise cones; cone rows go to rtmp
 template <typename T, int L>
 __global__ __launch_bounds__(kBlock) void k_kkt_m(KktData D, const T *__restrict__ x, const T *__restrict__ y,
                                                   double *__restrict__ part, int64_t bstride, int64_t dobj_off) {
-    __shared__ double lds[kBlock / 64];
+    __shared__ double lds[(kBlock / 64) * 7];
     const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t i = gid / L;
     const int lane = (int)(gid % L);
-    double ax = 0.0;
-    if (i < D.m) {
-        const int32_t s = D.G.rowptr[i], e = D.G.rowptr[i + 1];
-        for (int32_t k = s + lane; k < e; k += L) ax += D.G.val[k] * (double)x[D.G.col[k]];
-    }
-    if (L > 1) ax = group_sum<L>(ax);
+    const double ax = row_dot_mixed<T, L>(D.G, i, lane, x);
     double s_[3] = {0, 0, 0}, m_[4] = {0, 0, 0, 0};
     if (i < D.m && lane == 0) {
         const double axr = ax * D.Dl[i];             // G x_raw = Dl .* (Gs xs)
@@ -1576,16 +1584,13 @@ __global__ __launch_bounds__(kBlock) void k_kkt_m(KktData D, const T *__restrict
             D.rtmp[i] = r;
         }
     }
-    const int qs[3] = {Q_DOBJ, Q_AXN1, Q_P2};
-    const int qm[4] = {Q_AXINF, Q_PINF, Q_PROJINF, Q_YINF};
-    for (int q = 0; q < 3; ++q) {
-        const double v = block_sum(s_[q], lds);
-        if (threadIdx.x == 0) part[qs[q] * D.qstride + (q == 0 ? dobj_off : bstride) + blockIdx.x] = v;
-    }
-    for (int q = 0; q < 4; ++q) {
-        const double v = block_max(m_[q], lds);
-        if (threadIdx.x == 0) part[qm[q] * D.qstride + bstride + blockIdx.x] = v;
-    }
+    const int qi[7] = {Q_DOBJ, Q_AXN1, Q_P2, Q_AXINF, Q_PINF, Q_PROJINF, Q_YINF};
+    double v[7] = {s_[0], s_[1], s_[2], m_[0], m_[1], m_[2], m_[3]};
+    const bool ismax[7] = {false, false, false, true, true, true, true};
+    block_reduce_multi<7>(v, ismax, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < 7; ++q) part[qi[q] * D.qstride + (q == 0 ? dobj_off : bstride) + blockIdx.x] = v[q];
 }
 
 // cone rows: plain SOC projection of r = Gx - h in place on a copy, residual sums
@@ -1717,33 +1722,47 @@ __global__ __launch_bounds__(kBlock) void k_kkt_big_res(ConeLayout K, const doub
     }
 }
 
+// One workgroup per residual quantity (blockIdx.x = q): each thread sums its
+// strided partials in index order (eight loads in flight), then the block sum
+// or maximum.
+constexpr int kKktFinalUnroll = 8;
 __global__ __launch_bounds__(kBlock) void k_kkt_final(const double *part, KktCounts counts, int64_t qs, KktOut *out) {
     __shared__ double lds[kBlock / 64];
-    const bool ismax[Q_N] = {false, false, false, false, false, true, true, true, false, false, true, true, true,
-                             true, true};
-    double res[Q_N];
-    for (int q = 0; q < Q_N; ++q) {
-        double v = 0.0;
-        for (int i = threadIdx.x; i < counts.c[q]; i += kBlock)
-            v = ismax[q] ? fmax(v, part[q * qs + i]) : v + part[q * qs + i];
-        res[q] = ismax[q] ? block_max(v, lds) : block_sum(v, lds);
+    const int q = (int)blockIdx.x;
+    const bool ismax = q == Q_ATYINF || q == Q_SINF || q == Q_DINF || q == Q_AXINF || q == Q_PINF ||
+                       q == Q_PROJINF || q == Q_XINF || q == Q_YINF;
+    const double *pq = part + (int64_t)q * qs;
+    const int cnt = counts.c[q];
+    double v = 0.0;
+    for (int i0 = (int)threadIdx.x; i0 < cnt; i0 += kBlock * kKktFinalUnroll) {
+        double t[kKktFinalUnroll];
+#pragma unroll
+        for (int u = 0; u < kKktFinalUnroll; ++u) {
+            const int i = i0 + u * kBlock;
+            t[u] = i < cnt ? pq[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kKktFinalUnroll; ++u)
+            if (i0 + u * kBlock < cnt) v = ismax ? fmax(v, t[u]) : v + t[u];
     }
-    if (threadIdx.x == 0) {
-        out->pobj = res[Q_POBJ];
-        out->dobj = res[Q_DOBJ];
-        out->atyInf = res[Q_ATYINF];
-        out->atyN1 = res[Q_ATYN1];
-        out->sInf = res[Q_SINF];
-        out->sN1 = res[Q_SN1];
-        out->d2 = res[Q_D2];
-        out->dInf = res[Q_DINF];
-        out->axInf = res[Q_AXINF];
-        out->axN1 = res[Q_AXN1];
-        out->p2 = res[Q_P2];
-        out->pInf = res[Q_PINF];
-        out->projInf = res[Q_PROJINF];
-        out->xInf = res[Q_XINF];
-        out->yInf = res[Q_YINF];
+    const double r = ismax ? block_max(v, lds) : block_sum(v, lds);
+    if (threadIdx.x != 0) return;
+    switch (q) {
+    case Q_POBJ: out->pobj = r; break;
+    case Q_DOBJ: out->dobj = r; break;
+    case Q_ATYINF: out->atyInf = r; break;
+    case Q_ATYN1: out->atyN1 = r; break;
+    case Q_SINF: out->sInf = r; break;
+    case Q_SN1: out->sN1 = r; break;
+    case Q_D2: out->d2 = r; break;
+    case Q_DINF: out->dInf = r; break;
+    case Q_AXINF: out->axInf = r; break;
+    case Q_AXN1: out->axN1 = r; break;
+    case Q_P2: out->p2 = r; break;
+    case Q_PINF: out->pInf = r; break;
+    case Q_PROJINF: out->projInf = r; break;
+    case Q_XINF: out->xInf = r; break;
+    default: out->yInf = r; break;
     }
 }
 
@@ -1787,7 +1806,7 @@ void launch_kkt(hipStream_t s, const KktData &D, const T *x, const T *y, double 
         k_kkt_big_res<<<bs2, kBlock, 0, s>>>(D.K, D.rtmp, D.scratch_big, part, bm + bs1, D.qstride);
     }
     if (be) k_kkt_exp<<<be, kBlock, 0, s>>>(D.K, D.rtmp, part, bm + bs1 + bs2, D.qstride);
-    k_kkt_final<<<1, kBlock, 0, s>>>(part, kc, D.qstride, out);
+    k_kkt_final<<<Q_N, kBlock, 0, s>>>(part, kc, D.qstride, out);
 }
 template void launch_kkt<double>(hipStream_t, const KktData &, const double *, const double *, double *, KktOut *);
 template void launch_kkt<float>(hipStream_t, const KktData &, const float *, const float *, double *, KktOut *);
