@@ -31,47 +31,48 @@
 
 namespace pdcs {
 
-// Row dot product over L lanes.  Entries are taken two per lane per step, so
-// the index, value and gather loads of both are in flight together (one
-// dependent-load chain per pair instead of one per entry); the products are
-// accumulated in entry order, as in a one-per-step loop.
-template <typename T, int L>
-__device__ __forceinline__ double row_dot(const Csr<T> &A, int64_t row, int lane, const T *x) {
+// Row dot product over L lanes.  Each lane takes up to kRowBatch of its
+// entries per step, with all their index, value and gather loads in flight
+// together (rows of up to kRowBatch * L entries cost one dependent-load chain;
+// lanes past the row end load x[0] and drop it).  The products are
+// accumulated in entry order, as in a one-entry-per-step loop.
+constexpr int kRowBatch = 4;
+template <typename TV, typename T, int L>
+__device__ __forceinline__ double row_dot_batched(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                                                  const TV *__restrict__ val, int64_t rows, int64_t row, int lane,
+                                                  const T *__restrict__ x) {
     double acc = 0.0;
-    if (row < A.rows) {
-        const int32_t s = A.rowptr[row], e = A.rowptr[row + 1];
-        int32_t k = s + lane;
-        for (; k + L < e; k += 2 * L) {
-            const int32_t c0 = A.col[k], c1 = A.col[k + L];
-            const T v0 = A.val[k], v1 = A.val[k + L];
-            const T x0 = x[c0], x1 = x[c1];
-            acc += (double)v0 * (double)x0;
-            acc += (double)v1 * (double)x1;
+    if (row < rows) {
+        const int32_t s = rowptr[row], e = rowptr[row + 1];
+        for (int32_t k = s + lane; k < e; k += kRowBatch * L) {
+            int32_t c[kRowBatch];
+            TV v[kRowBatch];
+#pragma unroll
+            for (int u = 0; u < kRowBatch; ++u) {
+                const int32_t kk = k + u * L;
+                const bool in = kk < e;
+                c[u] = in ? col[kk] : 0;
+                v[u] = in ? val[kk] : (TV)0;
+            }
+            T xv[kRowBatch];
+#pragma unroll
+            for (int u = 0; u < kRowBatch; ++u) xv[u] = x[c[u]];
+#pragma unroll
+            for (int u = 0; u < kRowBatch; ++u)
+                if (k + u * L < e) acc += (double)v[u] * (double)xv[u];
         }
-        if (k < e) acc += (double)A.val[k] * (double)x[A.col[k]];
     }
     if (L > 1) acc = group_sum<L>(acc);
     return acc;
 }
-
-// row_dot for an FP64 matrix and a vector of precision T (residual checks)
+template <typename T, int L>
+__device__ __forceinline__ double row_dot(const Csr<T> &A, int64_t row, int lane, const T *x) {
+    return row_dot_batched<T, T, L>(A.rowptr, A.col, A.val, A.rows, row, lane, x);
+}
+// an FP64 matrix with a vector of precision T (residual checks)
 template <typename T, int L>
 __device__ __forceinline__ double row_dot_mixed(const Csr<double> &A, int64_t row, int lane, const T *x) {
-    double acc = 0.0;
-    if (row < A.rows) {
-        const int32_t s = A.rowptr[row], e = A.rowptr[row + 1];
-        int32_t k = s + lane;
-        for (; k + L < e; k += 2 * L) {
-            const int32_t c0 = A.col[k], c1 = A.col[k + L];
-            const double v0 = A.val[k], v1 = A.val[k + L];
-            const T x0 = x[c0], x1 = x[c1];
-            acc += v0 * (double)x0;
-            acc += v1 * (double)x1;
-        }
-        if (k < e) acc += A.val[k] * (double)x[A.col[k]];
-    }
-    if (L > 1) acc = group_sum<L>(acc);
-    return acc;
+    return row_dot_batched<double, T, L>(A.rowptr, A.col, A.val, A.rows, row, lane, x);
 }
 
 __device__ __forceinline__ bool halted(const StepState *st, int slot) {

@@ -936,9 +936,9 @@ struct Solver {
             copy_sync(P->ctx, &b, P->K.big_state, sizeof(b), hipMemcpyDeviceToHost);
             fprintf(stderr,
                     "[pdcs trace] big cone 0: case %d done %d series %d newton iters %d t %.17g s %.17g | root finds %ld "
-                    "series first %ld refine %ld fallback %ld (newton steps %ld)\n",
+                    "series first %ld refine %ld fallback %ld (newton steps %ld) | chunks %ld robust %ld\n",
                     b.cs, b.done, b.series, b.iters, b.t, b.s, (long)b.n_root, (long)b.n_first, (long)b.n_refine,
-                    (long)b.n_fallback, (long)b.n_newton);
+                    (long)b.n_fallback, (long)b.n_newton, (long)chunks_run, (long)chunks_robust);
         }
         if (prm->verbose)
             fprintf(stderr, "[pdcs %s] iter %ld relp %.3e reld %.3e gap %.3e pobj %.10e dobj %.10e restarts %ld\n",
@@ -1089,8 +1089,11 @@ struct Solver {
     // ends with the primal step of the next one
     bool primal_pending = true;
 
+    int64_t chunks_run = 0, chunks_robust = 0;   // diagnostics (PDCS_TRACE)
     int run_chunk(int len) {
         const bool robust = robust_next && P->K.nsoc_big > 0;
+        chunks_run++;
+        chunks_robust += robust ? 1 : 0;
         if (primal_pending)
             launch_attempt_part<T>(0, s, 0, G, Gt, PV, P->K, P->dsc2, I.it, P->st, P->apart, P->apart_stride);
         P->h_st->stop_at = (int64_t)1 << 62;
