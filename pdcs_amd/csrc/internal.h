@@ -21,7 +21,7 @@ struct Csr {
     int32_t grid;    // workgroup cap of the grid-stride attempt kernels (0 = kAttemptMaxGrid)
 };
 
-constexpr int kBigMaxStages = 8;
+constexpr int kBigMaxStages = 16;
 // Per-cone scalar state of the multi-workgroup scaled-SOC root find.
 struct BigSocState {
     double s;          // head of the point being projected
@@ -37,8 +37,9 @@ struct BigSocState {
     int32_t iters;     // exact Newton steps taken by the fallback
     int32_t series;    // 1 if the series solve was accepted
     // counters since the solve started (diagnostics): root finds, series accepted
-    // in the first / refine pass, exact fallbacks and their Newton steps
-    int64_t n_root, n_first, n_refine, n_fallback, n_newton;
+    // in the first / refine pass, projections finished by Newton steps (stages or
+    // the single-workgroup fallback) and those steps, single-workgroup fallbacks
+    int64_t n_root, n_first, n_refine, n_fallback, n_newton, n_single;
 };
 constexpr int kBigBlockElems = 1024;               // tail elements per workgroup (at least)
 constexpr int kBigMaxBlocks = 256;                 // workgroups per big cone (at most)
@@ -48,7 +49,7 @@ constexpr int kBigMoments = 16;                    // series terms M_0..M_{K}, K
 // (proj_rel_tol, src/pdcs_gpu/PDCS_GPU.jl:29)
 constexpr double kBigSeriesTol = 1e-13;
 constexpr int kBigPoints = 3;                      // expansion points of the second moment pass
-constexpr int kBigStages = 8;                      // Newton stages of a robust chain
+constexpr int kBigStages = 16;                     // Newton stages of a robust chain
 static_assert(kBigStages <= kBigMaxStages, "BigSocState::fin holds kBigMaxStages + 1 flags");
 constexpr int kBigSums = 4 + kBigMoments + 1;      // case sums, moments, max e_i
 

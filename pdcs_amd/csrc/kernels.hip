@@ -551,6 +551,7 @@ __device__ void big_fallback(const ConeLayout &K, BigSocState *S, BigSocState &L
         S->t_prev = th;
         S->n_fallback += 1;
         S->n_newton += L.iters;
+        S->n_single += 1;
     }
 }
 

@@ -936,9 +936,11 @@ struct Solver {
             copy_sync(P->ctx, &b, P->K.big_state, sizeof(b), hipMemcpyDeviceToHost);
             fprintf(stderr,
                     "[pdcs trace] big cone 0: case %d done %d series %d newton iters %d t %.17g s %.17g | root finds %ld "
-                    "series first %ld refine %ld fallback %ld (newton steps %ld) | chunks %ld robust %ld\n",
+                    "series first %ld refine %ld newton-finished %ld (steps %ld, single-workgroup %ld) | chunks %ld "
+                    "robust %ld | chunk seconds fast %.4f robust %.4f\n",
                     b.cs, b.done, b.series, b.iters, b.t, b.s, (long)b.n_root, (long)b.n_first, (long)b.n_refine,
-                    (long)b.n_fallback, (long)b.n_newton, (long)chunks_run, (long)chunks_robust);
+                    (long)b.n_fallback, (long)b.n_newton, (long)b.n_single, (long)chunks_run, (long)chunks_robust,
+                    t_chunks[0], t_chunks[1]);
         }
         if (prm->verbose)
             fprintf(stderr, "[pdcs %s] iter %ld relp %.3e reld %.3e gap %.3e pobj %.10e dobj %.10e restarts %ld\n",
@@ -1090,10 +1092,17 @@ struct Solver {
     bool primal_pending = true;
 
     int64_t chunks_run = 0, chunks_robust = 0;   // diagnostics (PDCS_TRACE)
+    double t_chunks[2] = {0.0, 0.0};              // wall seconds of fast / robust chunks
     int run_chunk(int len) {
         const bool robust = robust_next && P->K.nsoc_big > 0;
         chunks_run++;
         chunks_robust += robust ? 1 : 0;
+        const double tc0 = now_s();
+        const int r = run_chunk_body(len, robust);
+        t_chunks[robust ? 1 : 0] += now_s() - tc0;
+        return r;
+    }
+    int run_chunk_body(int len, bool robust) {
         if (primal_pending)
             launch_attempt_part<T>(0, s, 0, G, Gt, PV, P->K, P->dsc2, I.it, P->st, P->apart, P->apart_stride);
         P->h_st->stop_at = (int64_t)1 << 62;
