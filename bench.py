@@ -9,7 +9,9 @@ mixed-precision (FP32 iterates/SpMV, FP64 reductions and tests, FP64 polish)
 time-to-tolerance and the FP64 path is timed beside it.  The JSON line also
 carries the roofline of the dominant kernel (HIP-event timing inside the
 library's stream) and the CPU baseline (the FP64 oracle on this host, one
-thread, on a bounded sample of attempts).
+thread, on a bounded sample of attempts).  Extra keys report the same
+regression at 8x the size (bandwidth-bound kernels), the batched portfolio
+workload and the differentiable-layer workload.
 """
 from __future__ import annotations
 
@@ -39,6 +41,7 @@ def parse():
     ap.add_argument("--reps", type=int, default=50)
     ap.add_argument("--no-graph", action="store_true", help="plain launches (for profilers that cannot follow graphs)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-portfolio and adjoint measurements")
+    ap.add_argument("--no-large", action="store_true", help="skip the 8x-size SOC regression measurement")
     ap.add_argument("--schedule64", type=str, default="", metavar="LG,LGT,CG,CGT",
                     help="pin the FP64 SpMV row schedule (lanes of G, of G', workgroup caps) instead of autotuning")
     ap.add_argument("--schedule32", type=str, default="", metavar="LG,LGT,CG,CGT",
@@ -97,6 +100,34 @@ def batch_section(a, np, cpu):
         out["cpu_baseline"] = {"value": round(per * len(probs), 2), "unit": "s", "cores": 1, "kind": "port",
                                "sample": f"FP64 oracle solves of {len(sample)} instances of the batch on one host "
                                          f"thread ({per * 1e3:.1f} ms each), scaled to {len(probs)} instances"}
+    return out
+
+
+def large_section(a, np):
+    """The same SOC regression at 8x the size (A 1600000 x 400000 at density
+    1.25e-5: 8M nonzeros, one SOC of dimension 1600001, 400000 box bounds), where
+    the attempt kernels are bandwidth-bound: one warm-up and one timed solve per
+    precision (FP64 checks at the default interval and every 200 attempts)."""
+    from pdcs_amd import instances
+    from pdcs_amd.solver import Problem
+
+    t0 = time.time()
+    P = instances.workload2(rows=1600000, cols=400000, density=1.25e-5)
+    prob = Problem.from_instance(P)
+    out = {"workload": "large SOC robust regression 1600000x400000 density 1.25e-05", "n": P.n, "m": P.m,
+           "nnz": P.nnz, "setup_s": round(time.time() - t0, 2)}
+    for prec, tag, kw in ((1, "mixed", {}), (0, "fp64", {}), (0, "fp64_check200", {"check_terminate_freq": 200})):
+        prob.solve(precision=prec, time_limit=600.0, **kw)               # warm-up
+        r = prob.solve(precision=prec, time_limit=600.0, **kw)
+        out[tag] = {"time_to_tol_s": round(r["time_solve"], 4), "iters": r["iter"], "iter_fp32": r["iter_fp32"],
+                    "exit_code": r["exit_code"], "pobj": r["pobj"],
+                    "readme_residuals": [r["rel_primal_res"], r["rel_dual_res"], r["rel_gap"]]}
+        print(f"[bench] 8x {tag}: exit {r['exit_code']} iters {r['iter']} time {r['time_solve']:.4f}s "
+              f"pobj {r['pobj']:.10e}", file=sys.stderr, flush=True)
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["time_to_tol_s"] / out["mixed"]["time_to_tol_s"], 3)
+    out["speedup_mixed_vs_fp64_check200"] = round(out["fp64_check200"]["time_to_tol_s"] /
+                                                  out["mixed"]["time_to_tol_s"], 3)
+    prob.close()
     return out
 
 
@@ -222,6 +253,9 @@ def main():
                "sample": f"FP64 oracle, first {done} PDHG attempts of this workload timed on one host thread "
                          f"({t_cpu:.1f}s), scaled to the {s64['iters']:.0f} attempts the FP64 GPU solve needed"}
     extra = {}
+    if not a.no_large and a.rows < 1600000:
+        prob.close()
+        extra["large_soc_8x"] = large_section(a, np)
     if not a.no_batch:
         extra["batched_portfolios"] = batch_section(a, np, not a.no_cpu)
         extra["adjoint_layers"] = adjoint_section(a, np)
