@@ -696,9 +696,22 @@ __global__ __launch_bounds__(kBlock) void k_bigsoc_rare(ConeLayout K, const doub
     const int k = K.soc_big_dim[cone] - 1;
     const int b0 = K.big_cone_blk0[cone], np = K.big_cone_nblk[cone];
     const double s = S->s, tr = S->t, tL = S->tL, tR = S->tR, t0 = S->t0_in;
-    if (threadIdx.x == 0) sh_acc = -1;
+    // Without an accepted series, M_0 and M_1 of each point are S(tp) and
+    // S1(tp) exactly: one exact Newton step from every point tightens the
+    // bracket (tbL, tbR) and gives the stages a start one step ahead, taken
+    // from the point closest to the root from the left (Newton from the left
+    // of the concave phi stays left of the root).  A point whose step has
+    // converged finishes the projection as an accepted series would.
+    __shared__ double sh_ts, sh_tbL, sh_tbR;
+    if (threadIdx.x == 0) {
+        sh_acc = -1;
+        sh_tbL = tL;
+        sh_tbR = tR;
+        sh_ts = NAN;
+    }
     double v[kBigSums];
     bool ismax[kBigSums];
+    double best_left = -INFINITY, ts_left = NAN, lowest_right = INFINITY, ts_right = NAN;
 #pragma unroll 1
     for (int p = 0; p < kBigPoints; ++p) {
         const double tp = big_point(p, tr, t0, tL, tR);
@@ -722,11 +735,33 @@ __global__ __launch_bounds__(kBlock) void k_bigsoc_rare(ConeLayout K, const doub
             if (Lp.done) {
                 sh_acc = p;
                 sh_th = Lp.t;
+            } else if (v[4] > 0.0 && tp > sh_tbL && tp < sh_tbR) {
+                double bl = sh_tbL, br = sh_tbR;
+                bool conv = false;
+                const double tn = soc_newton_step(tp, v[4], v[5], bl, br, conv);
+                const bool left = 1.0 / sqrt(v[4]) - 1.0 < 0.0;
+                sh_tbL = bl;
+                sh_tbR = br;
+                if (conv && tn >= bl && tn <= br) {
+                    sh_acc = p;
+                    sh_th = tn;
+                } else if (left && tp > best_left) {
+                    best_left = tp;
+                    ts_left = tn;
+                } else if (!left && tp < lowest_right) {
+                    lowest_right = tp;
+                    ts_right = tn;
+                }
             }
         }
         __syncthreads();
         if (sh_acc >= 0) break;
     }
+    if (threadIdx.x == 0 && sh_acc < 0) {
+        const double cand = ts_left == ts_left ? ts_left : ts_right;
+        sh_ts = (cand >= sh_tbL && cand <= sh_tbR) ? cand : NAN;
+    }
+    __syncthreads();
     if (sh_acc >= 0) {
         const double th = sh_th;
         const double my2 = big_apply_block(K, d2, y, ys, dy, st0, K.big_blk_first[b], big_blk_end(K, b, k), b == b0, 3,
@@ -742,19 +777,27 @@ __global__ __launch_bounds__(kBlock) void k_bigsoc_rare(ConeLayout K, const doub
         }
         return;
     }
+    // start of the exact Newton steps: the best point's Newton iterate inside
+    // the tightened bracket, else tr
+    const double tstart = sh_ts == sh_ts ? sh_ts : fmin(fmax(tr, sh_tbL), sh_tbR);
     if (K.big_stages > 0) {
-        // robust chain: Newton sums of this block at tr for the first stage
-        big_stage_sums(K, d2, y, st0, K.big_blk_first[b], big_blk_end(K, b, k), s, tr, b, 0, lds);
+        // robust chain: Newton sums of this block at tstart for the first stage
+        big_stage_sums(K, d2, y, st0, K.big_blk_first[b], big_blk_end(K, b, k), s, tstart, b, 0, lds);
         if (b == b0 && threadIdx.x == 0) {
-            S->tn[0] = tr;
-            S->tnL[0] = tL;
-            S->tnR[0] = tR;
+            S->tn[0] = tstart;
+            S->tnL[0] = sh_tbL;
+            S->tnR[0] = sh_tbR;
             S->fin[0] = 0;
         }
         return;
     }
     if (b != b0) return;
-    big_fallback(K, S, L, d2, y, ys, dy, st0, k, s, tr, b0, part, lds);
+    if (threadIdx.x == 0) {
+        S->tL = sh_tbL;
+        S->tR = sh_tbR;
+    }
+    __syncthreads();
+    big_fallback(K, S, L, d2, y, ys, dy, st0, k, s, tstart, b0, part, lds);
 }
 
 // Newton stage j of a robust chain: every workgroup reduces the stage sums at
