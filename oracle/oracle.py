@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU FP64 oracle (liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product (pdcs_amd) never
+cpu_baseline leg of bench.py, to check results against.  The product (pdcs_amd) never
 imports this module.
 """
 from __future__ import annotations

@@ -1,8 +1,8 @@
 /*
  * pdcs_oracle.h -- CPU FP64 restatement of the PDCS PDHG hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This oracle is the checker for the HIP library
- * (libpdcs_hip.so).  Only tests/, __graft_entry__.smoke() and bench.py's
+ * TEST INFRASTRUCTURE ONLY.  The HIP library (libpdcs_hip.so) is tested against
+ * this oracle.  Only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg may load it.  It is never linked into, called by, or used
  * as a fallback for the product path.
  *

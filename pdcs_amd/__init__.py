@@ -1,7 +1,7 @@
 """pdcs_amd: MI355X-native (gfx950) PDHG conic solver, a drop-in for the PDCS GPU path.
 
-The compute path is libpdcs_hip.so (hand-written HIP kernels + a C++ host
-driver behind the C ABI of include/pdcs.h).  This package is the Python host
+The compute path is libpdcs_hip.so (hand-written HIP kernels + C++ host
+control code behind the C ABI of include/pdcs.h).  This package is the Python host
 side: ctypes bindings (``_lib``), the reference-shaped solve entry point
 (``solver.rpdhg_gpu_solve``) and the synthetic workload generators
 (``instances``).

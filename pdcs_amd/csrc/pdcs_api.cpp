@@ -1,6 +1,6 @@
-// pdcs_api.cpp -- C ABI (include/pdcs.h) and host driver of libpdcs_hip.so.
+// pdcs_api.cpp -- C ABI (include/pdcs.h) and host control of libpdcs_hip.so.
 //
-// The driver restates the control flow of the reference GPU solver
+// The host code restates the control flow of the reference GPU solver
 // (pdhg_main_iter_average_diagonal_rescaling_restarts_adaptive_weight_resolving_aggressive!,
 // src/pdcs_gpu/rpdhg_alg_gpu_gen_scaling.jl:770-896) around the device-resident
 // attempt chain of kernels.hip.  The host only steps in at halts: every

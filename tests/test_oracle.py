@@ -1,4 +1,4 @@
-"""Pins of the CPU FP64 oracle (the checker of the HIP path).
+"""Pins of the CPU FP64 oracle (the reference the HIP path is tested against).
 
 Each check is fixed by mathematics, not by retyping the oracle: closed forms,
 the optimality conditions that characterise a Euclidean projection onto a
