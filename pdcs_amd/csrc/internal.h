@@ -49,8 +49,14 @@ constexpr int kBigMoments = 16;                    // series terms M_0..M_{K}, K
 // (proj_rel_tol, src/pdcs_gpu/PDCS_GPU.jl:29)
 constexpr double kBigSeriesTol = 1e-13;
 constexpr int kBigPoints = 3;                      // expansion points of the second moment pass
-constexpr int kBigStages = 16;                     // Newton stages of a robust chain
-static_assert(kBigStages <= kBigMaxStages, "BigSocState::fin holds kBigMaxStages + 1 flags");
+// Newton stages of a robust chain: a cone still unprojected after the last one
+// is finished by one workgroup, whose cost grows with the cone (about 8 ms at
+// 1.6M rows), while each stage launch costs a few microseconds even when the
+// cone is done; so cones above kBigStagesLargeDim rows get more stages.
+constexpr int kBigStages = 8;
+constexpr int kBigStagesLarge = 16;
+constexpr int64_t kBigStagesLargeDim = 1 << 20;
+static_assert(kBigStagesLarge <= kBigMaxStages, "BigSocState::fin holds kBigMaxStages + 1 flags");
 constexpr int kBigSums = 4 + kBigMoments + 1;      // case sums, moments, max e_i
 
 // Cone layout of the dual rows on the device.
@@ -73,6 +79,7 @@ struct ConeLayout {
     double *big_part2;           // second moment pass: [(point * kBigSums + q) * nbig_blocks + block]
     double *big_part3;           // Newton stage sums: [(parity * 2 + q) * nbig_blocks + block]
     int32_t big_stages;          // Newton stage kernels in this attempt chain (0: fast chain)
+    int32_t robust_stages;       // stage count of a robust chain for these cones
     int32_t nexp;                // exponential cones (code 12: y in Kexp*)
     int32_t ndexp;               // dual exponential cones (code 15: y in Kexp)
     int64_t exp_begin;           // first row of the exponential section

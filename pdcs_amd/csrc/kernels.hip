@@ -1358,7 +1358,7 @@ void launch_project_dual(hipStream_t s, const ConeLayout &K, const double *d, co
     // standalone projections start from arbitrary points (no useful warm start):
     // robust chain, multi-workgroup Newton stages for the large cones
     ConeLayout Kr = K;
-    Kr.big_stages = kBigStages;
+    Kr.big_stages = K.robust_stages;
     launch_cones<T>(s, Kr, d2, y, nullptr, nullptr, nullptr, 0, nullptr);
 }
 template void launch_project_dual<double>(hipStream_t, const ConeLayout &, const double *, const double *, double *);

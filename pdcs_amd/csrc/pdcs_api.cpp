@@ -269,6 +269,9 @@ int build_cone_layout(pdcs_ctx *ctx, DevBuf &M, int64_t m, int64_t mzero, int64_
     K.soc_big_start = up32(sb_start);
     K.soc_big_dim = up32(sb_dim);
     K.nbig_blocks = (int32_t)blk_cone.size();
+    K.robust_stages = kBigStages;
+    for (int32_t d : sb_dim)
+        if ((int64_t)d > kBigStagesLargeDim) K.robust_stages = kBigStagesLarge;
     K.big_blk_cone = up32(blk_cone);
     K.big_blk_first = up32(blk_first);
     K.big_cone_blk0 = up32(cone_blk0);
@@ -768,14 +771,19 @@ struct Solver {
     // Mnorm, rpdhg_alg_gpu_gen.jl:133-138
     double Mnorm2(const T *x, const T *y, double tau_, double sigma_) {
         launch_spmv<T>(s, G, x, I.tm2);
+        // ||x||^2 into dout[0], <y, Gx> and ||y||^2 into dout[1..2]: one read-back
         const T *A1[1] = {x};
-        double o1[1];
-        dots<1>(A1, A1, n, o1);
+        launch_dots<T>(s, 1, A1, A1, n, P->dpart, P->dout);
         const T *A2[2] = {y, y};
         const T *B2[2] = {I.tm2, y};
-        double o2[2];
-        dots<2>(A2, B2, m, o2);
-        return o1[0] / tau_ - 2 * o2[0] + o2[1] / sigma_;
+        launch_dots<T>(s, 2, A2, B2, m, P->dpart, P->dout + 1);
+        if (hipMemcpyAsync(P->h_dout, P->dout, sizeof(double) * 3, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            sync() != PDCS_OK) {
+            rc = fail(P->ctx, PDCS_ERR_HIP, "residual norm read-back failed");
+            return NAN;
+        }
+        const double *o = P->h_dout;
+        return o[0] / tau_ - 2 * o[1] + o[2] / sigma_;
     }
     // approximate_cal_diagonal! + diffs, rpdhg_alg_gpu_gen_scaling.jl:1-13,41-45
     double bs_eval(double t, const T *primal, const T *dual, double tau_, double sigma_) {
@@ -1055,7 +1063,7 @@ struct Solver {
     int clean_chunks = 0;
     ConeLayout chain_layout(bool robust) const {
         ConeLayout Kc = P->K;
-        Kc.big_stages = (robust && P->K.nsoc_big > 0) ? kBigStages : 0;
+        Kc.big_stages = (robust && P->K.nsoc_big > 0) ? P->K.robust_stages : 0;
         return Kc;
     }
     int update_chain_choice() {
