@@ -50,7 +50,8 @@ def _random_cones(rng, dims):
     return y, d
 
 
-@pytest.mark.parametrize("dims", [[10] * 200, [3, 7, 64, 65, 200, 1024], [1025, 2049, 5000, 10], [200001]])
+@pytest.mark.parametrize("dims", [[10] * 200, [3, 7, 64, 65, 200, 1024], [1025, 2049, 5000, 10], [200001],
+                                  [1100001]])   # above 2^20 rows: the longer Newton-stage chain
 def test_project_dual_scaled_soc_matches_oracle(ctx, dims):
     rng = np.random.default_rng(len(dims) + dims[0])
     y, d = _random_cones(rng, dims)
