@@ -145,6 +145,18 @@ int pdcs_problem_create_device(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, 
                                const double *h, const double *bl, const double *bu,
                                const pdcs_cones *cones, pdcs_problem **out);
 int pdcs_problem_destroy(pdcs_problem *p);
+/* New data on the same sparsity pattern: c (n), the values of G in the order
+ * of the creation call (nnz), h (m), bl and bu (nb); NULL keeps an array.  The
+ * scaling is recomputed; the transpose, cone layout, SpMV schedules and
+ * captured attempt graphs are kept, and the last solution is dropped.
+ * Replaces the re-solve with a fixed pattern of the reference's Python bridge
+ * (update_G! / update_solver!, src/pdcs_gpu/cvxpy_wrapper/data_updating.jl:4-75).
+ * Problems with rotated-SOC blocks are rejected (PDCS_ERR_UNSUPPORTED). */
+int pdcs_problem_update_host(pdcs_problem *p, const double *c, const double *vals, const double *h,
+                             const double *bl, const double *bu);
+/* Same from device pointers. */
+int pdcs_problem_update_device(pdcs_problem *p, const double *c, const double *vals, const double *h,
+                               const double *bl, const double *bu);
 
 /* Whole solve (main loop pdhg_main_iter_..._aggressive!,
  * src/pdcs_gpu/rpdhg_alg_gpu_gen_scaling.jl:770-896).  x0 (n) / y0 (m) are

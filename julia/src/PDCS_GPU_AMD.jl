@@ -11,9 +11,10 @@ MOI wrapper reads (src/pdcs_gpu/MOI_wrapper/MOI_wrapper.jl:648-659, 700-710):
 
 The MOI wrapper's `SparseMatrixCSC` (1-based CSC) is converted to the 0-based
 CSR the C ABI takes.  Beyond the reference's API it adds `batch_solve` (many
-independent programs in one persistent launch) and `conic_layer` (a solve whose
+independent programs in one persistent launch), `conic_layer` (a solve whose
 reverse-mode derivative, `rrule` in ext/PDCS_GPU_AMDChainRulesExt.jl, comes
-from `pdcs_adjoint`).  This package is not exercised by the repository's tests
+from `pdcs_adjoint`) and `update_solver!` / `resolve!` (new data on the
+pattern of a resident layer problem).  This package is not exercised by the repository's tests
 (there is no Julia toolchain in the build image).
 """
 module PDCS_GPU_AMD
@@ -352,5 +353,56 @@ end
 
 """x = conic_layer(c, G, h; kw...): the solution of the layer program (see `conic_layer_forward`)."""
 conic_layer(c, G, h; kw...) = first(conic_layer_forward(c, G, h; kw...))
+
+"""
+    update_solver!(R; c = nothing, G = nothing, h = nothing, bl = nothing, bu = nothing)
+
+New data for the GPU-resident problem of `R` on the same sparsity pattern
+(`pdcs_problem_update_host`): arrays left as `nothing` are kept, and `G` must
+have the pattern of `R.G`. The problem is rescaled on the GPU; `resolve!`
+returns the new solution. This replaces the reference's `update_G!` /
+`update_solver!` (src/pdcs_gpu/cvxpy_wrapper/data_updating.jl:4-75).
+"""
+function update_solver!(R::ResidentProblem; c = nothing, G = nothing, h = nothing, bl = nothing, bu = nothing)
+    vals = nothing
+    if G !== nothing
+        Gs = SparseMatrixCSC{Float64, Int}(G)
+        (size(Gs) == size(R.G) && Gs.colptr == R.G.colptr && Gs.rowval == R.G.rowval) ||
+            error("PDCS_GPU_AMD: update_solver! needs G with the pattern of the resident problem")
+        vals = csr0(Gs)[3]
+        R.G = Gs
+    end
+    arg(v) = v === nothing ? Ptr{Float64}(C_NULL) : Vector{Float64}(v)
+    cv, vv, hv, blv, buv = arg(c), arg(vals), arg(h), arg(bl), arg(bu)
+    GC.@preserve cv vv hv blv buv begin
+        check(ccall((:pdcs_problem_update_host, libpdcs[]), Cint,
+            (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+            R.prob, cv, vv, hv, blv, buv), R.ctx)
+    end
+    return R
+end
+
+"""
+    resolve!(R; precision = :fp64, rel_tol = 1e-8, abs_tol = 1e-10) -> x
+
+Solves the resident problem of `R` again (after `update_solver!`) and returns
+x; the solution stays on the GPU for `conic_layer_adjoint`.
+"""
+function resolve!(R::ResidentProblem; precision::Symbol = :fp64, rel_tol::Float64 = 1e-8, abs_tol::Float64 = 1e-10)
+    m, n = size(R.G)
+    prm = PdcsParams()
+    ccall((:pdcs_params_default, libpdcs[]), Cvoid, (Ref{PdcsParams},), prm)
+    prm.rel_tol = rel_tol; prm.abs_tol = abs_tol
+    prm.precision = precision == :mixed ? 1 : 0
+    res = PdcsResult()
+    check(ccall((:pdcs_solve, libpdcs[]), Cint,
+        (Ptr{Cvoid}, Ref{PdcsParams}, Ptr{Float64}, Ptr{Float64}, Ref{PdcsResult}),
+        R.prob, prm, C_NULL, C_NULL, res), R.ctx)
+    res.exit_code == 0 || error("PDCS_GPU_AMD: re-solve ended with exit code $(res.exit_code)")
+    x = zeros(n); y = zeros(m); s = zeros(n)
+    check(ccall((:pdcs_get_solution, libpdcs[]), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+        R.prob, x, y, s), R.ctx)
+    return x
+end
 
 end # module

@@ -77,6 +77,7 @@ class BatchResult(C.Structure):
 # every symbol declared in include/pdcs.h
 EXPORTS = ["pdcs_params_default", "pdcs_ctx_create", "pdcs_ctx_destroy", "pdcs_last_error",
            "pdcs_problem_create_host", "pdcs_problem_create_device", "pdcs_problem_destroy",
+           "pdcs_problem_update_host", "pdcs_problem_update_device",
            "pdcs_solve", "pdcs_get_solution", "pdcs_get_scaling", "pdcs_evaluate", "pdcs_spmv",
            "pdcs_project_dual", "pdcs_project_primal_cone", "pdcs_time_kernels",
            "pdcs_batch_create_host", "pdcs_batch_destroy", "pdcs_batch_solve", "pdcs_batch_get_solution",
@@ -109,6 +110,8 @@ def load():
     L.pdcs_problem_create_device.argtypes = [vp, c_i64, c_i64, c_i64, vp, vp, vp, vp, vp, vp, vp,
                                              C.POINTER(Cones), C.POINTER(vp)]
     L.pdcs_problem_destroy.argtypes = [vp]
+    L.pdcs_problem_update_host.argtypes = [vp, c_dp, c_dp, c_dp, c_dp, c_dp]
+    L.pdcs_problem_update_device.argtypes = [vp, vp, vp, vp, vp, vp]
     L.pdcs_solve.argtypes = [vp, C.POINTER(Params), c_dp, c_dp, C.POINTER(Result)]
     L.pdcs_get_solution.argtypes = [vp, c_dp, c_dp, c_dp]
     L.pdcs_get_scaling.argtypes = [vp, c_dp, c_dp]

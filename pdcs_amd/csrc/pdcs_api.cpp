@@ -49,7 +49,8 @@ struct IterSet {
     T *xb = nullptr, *yb = nullptr;                                   // best iterates (x_best / y_best)
     T *c = nullptr, *h = nullptr, *bl = nullptr, *bu = nullptr;        // scaled data
     T *vals = nullptr, *tvals = nullptr;
-    bool ready = false;
+    bool ready = false;   // allocated
+    bool stale = true;    // scaled data not yet (or no longer) filled from the problem data
 };
 
 struct ConvInfo {
@@ -83,6 +84,7 @@ struct pdcs_problem {
     double *c_raw = nullptr, *h_raw = nullptr, *blf = nullptr, *buf = nullptr, *laml = nullptr, *lamu = nullptr;
     double *Dr = nullptr, *Dl = nullptr, *dsc = nullptr, *dsc2 = nullptr;
     double *kkt_big = nullptr;   // large-cone norm partials of the residual check
+    double *scale_rowv = nullptr, *scale_colv = nullptr;   // row / column factors of one scaling pass
     double *rtmp = nullptr, *kpart = nullptr, *dpart = nullptr, *dout = nullptr, *apart = nullptr;
     int64_t kqstride = 0;   // residual partial slots per quantity
     int64_t apart_stride = 0;
@@ -298,7 +300,9 @@ int build_cone_layout(pdcs_ctx *ctx, DevBuf &M, int64_t m, int64_t mzero, int64_
 int scale_device(pdcs_problem *P) {
     pdcs_ctx *ctx = P->ctx;
     hipStream_t s = ctx->stream;
-    double *rowv = P->mem.alloc<double>(P->m), *colv = P->mem.alloc<double>(P->n);
+    if (!P->scale_rowv) P->scale_rowv = P->mem.alloc<double>(P->m);
+    if (!P->scale_colv) P->scale_colv = P->mem.alloc<double>(P->n);
+    double *rowv = P->scale_rowv, *colv = P->scale_colv;
     if (!rowv || !colv) return fail(ctx, PDCS_ERR_HIP, "alloc failed");
     Csr<double> G{P->d_rowptr, P->d_col, P->d_vals, P->m, P->lanesG};
     Csr<double> Gt{P->d_trowptr, P->d_tcol, P->d_tvals, P->n, P->lanesGt};
@@ -326,12 +330,15 @@ IterSet<double> &iters<double>(pdcs_problem *P) { return P->f64; }
 template <>
 IterSet<float> &iters<float>(pdcs_problem *P) { return P->f32; }
 
+// Iterate buffers of precision T (allocated once per problem) and the scaled
+// data in precision T (refilled after pdcs_problem_update).
 template <typename T>
 int alloc_iters(pdcs_problem *P) {
     IterSet<T> &I = iters<T>(P);
-    if (I.ready) return PDCS_OK;
+    if (I.ready && !I.stale) return PDCS_OK;
     const int64_t n = P->n, m = P->m;
     auto &M = P->mem;
+    if (!I.ready) {
     I.it.x = M.alloc<T>(n); I.it.xs = M.alloc<T>(n); I.it.xm = M.alloc<T>(n); I.it.xbar = M.alloc<T>(n);
     I.it.y = M.alloc<T>(m); I.it.ys = M.alloc<T>(m); I.it.ym = M.alloc<T>(m); I.it.dy = M.alloc<T>(m);
     I.it.g = M.alloc<double>(n); I.it.gd = M.alloc<double>(n);
@@ -348,11 +355,15 @@ int alloc_iters(pdcs_problem *P) {
         I.vals = M.alloc<T>(P->nnz);
         I.tvals = M.alloc<T>(P->nnz);
         if (!I.vals || !I.tvals) return fail(P->ctx, PDCS_ERR_HIP, "alloc failed");
-        launch_convert(P->ctx->stream, (float *)I.vals, P->d_vals, P->nnz);
-        launch_convert(P->ctx->stream, (float *)I.tvals, P->d_tvals, P->nnz);
     }
     if (!I.it.x || !I.h2 || !I.bu || !I.yd || !I.rx || !I.ry || !I.it.g || !I.it.gd || !I.xb || !I.yb)
         return fail(P->ctx, PDCS_ERR_HIP, "iterate allocation failed");
+    I.ready = true;
+    }
+    if (!std::is_same<T, double>::value) {
+        launch_convert(P->ctx->stream, (float *)I.vals, P->d_vals, P->nnz);
+        launch_convert(P->ctx->stream, (float *)I.tvals, P->d_tvals, P->nnz);
+    }
     // scaled data in precision T: c/Dr, h/Dl, bl*Dr, bu*Dr
     std::vector<double> hDr(P->n), hDl(P->m);
     HIPCHK(P->ctx, copy_sync(P->ctx, hDr.data(), P->Dr, sizeof(double) * P->n, hipMemcpyDeviceToHost));
@@ -367,7 +378,7 @@ int alloc_iters(pdcs_problem *P) {
         HIPCHK(P->ctx, copy_sync(P->ctx, I.bl, bl.data(), sizeof(T) * P->nb, hipMemcpyHostToDevice));
         HIPCHK(P->ctx, copy_sync(P->ctx, I.bu, bu.data(), sizeof(T) * P->nb, hipMemcpyHostToDevice));
     }
-    I.ready = true;
+    I.stale = false;
     return PDCS_OK;
 }
 
@@ -376,6 +387,8 @@ KktData kkt_data(pdcs_problem *P);
 int build_problem_rotated(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double *c, const int32_t *rowptr,
                           const int32_t *colidx, const double *vals, const double *h, const double *bl,
                           const double *bu, const pdcs_cones *cones, pdcs_problem **out);
+
+int upload_problem_data(pdcs_problem *P);
 
 int build_problem(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double *c, const int32_t *rowptr,
                   const int32_t *colidx, const double *vals, const double *h, const double *bl, const double *bu,
@@ -422,8 +435,6 @@ int build_problem(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double 
                 P->tmap[sl] = k;
             }
     }
-    std::vector<double> tvals(nnz);
-    for (int64_t k = 0; k < nnz; ++k) tvals[k] = vals[P->tmap[k]];
     auto &M = P->mem;
     P->d_rowptr = M.alloc<int32_t>(m + 1); P->d_col = M.alloc<int32_t>(nnz);
     P->d_trowptr = M.alloc<int32_t>(n + 1); P->d_tcol = M.alloc<int32_t>(nnz); P->d_tmap = M.alloc<int32_t>(nnz);
@@ -479,10 +490,26 @@ int build_problem(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double 
     upload(ctx, P->d_trowptr, P->trowptr.data(), n + 1);
     upload(ctx, P->d_tcol, P->tcol.data(), nnz);
     upload(ctx, P->d_tmap, P->tmap.data(), nnz);
-    upload(ctx, P->d_vals, vals, nnz);
+    if ((rc = upload_problem_data(P))) {
+        delete P;
+        return rc;
+    }
+    *out = P;
+    return PDCS_OK;
+}
+
+// Unscaled data from the host copies (P->vals, c, h, bl, bu) to the device, unit
+// scaling vectors and the data norms: the state before scale_device.
+int upload_problem_data(pdcs_problem *P) {
+    pdcs_ctx *ctx = P->ctx;
+    const int64_t n = P->n, m = P->m, nb = P->nb, nnz = P->nnz;
+    std::vector<double> tvals(nnz);
+    for (int64_t k = 0; k < nnz; ++k) tvals[k] = P->vals[P->tmap[k]];
+    upload(ctx, P->d_vals, P->vals.data(), nnz);
     upload(ctx, P->d_tvals, tvals.data(), nnz);
-    upload(ctx, P->c_raw, c, n);
-    upload(ctx, P->h_raw, h, m);
+    upload(ctx, P->c_raw, P->c.data(), n);
+    upload(ctx, P->h_raw, P->h.data(), m);
+    const double *bl = P->bl.data(), *bu = P->bu.data();
     std::vector<double> blf(nb), buf(nb), laml(nb), lamu(nb), ones_n(n, 1.0), ones_m(m, 1.0);
     for (int64_t j = 0; j < nb; ++j) {
         blf[j] = std::isinf(bl[j]) ? 0.0 : bl[j];            // replace_inf_with_zero, utils.cu:192
@@ -498,9 +525,9 @@ int build_problem(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double 
     upload(ctx, P->Dl, ones_m.data(), m);
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     // norms of the original data (cal_constant, def_struct.jl:936)
-    for (int64_t i = 0; i < m; ++i) { P->hN1 += std::fabs(h[i]); P->hNinf = std::max(P->hNinf, std::fabs(h[i])); }
-    for (int64_t j = 0; j < n; ++j) { P->cN1 += std::fabs(c[j]); P->cNinf = std::max(P->cNinf, std::fabs(c[j])); }
-    *out = P;
+    P->hN1 = P->hNinf = P->cN1 = P->cNinf = 0.0;
+    for (int64_t i = 0; i < m; ++i) { P->hN1 += std::fabs(P->h[i]); P->hNinf = std::max(P->hNinf, std::fabs(P->h[i])); }
+    for (int64_t j = 0; j < n; ++j) { P->cN1 += std::fabs(P->c[j]); P->cNinf = std::max(P->cNinf, std::fabs(P->c[j])); }
     return PDCS_OK;
 }
 
@@ -1522,6 +1549,57 @@ int pdcs_problem_create_device(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, 
     }
     return pdcs_problem_create_host(ctx, n, m, nb, hcv.data(), hr.data(), hc.data(), hv.data(), hh.data(), hbl.data(),
                                     hbu.data(), cones, out);
+}
+
+int pdcs_problem_update_host(pdcs_problem *P, const double *c, const double *vals, const double *h,
+                             const double *bl, const double *bu) {
+    if (!P) return fail(nullptr, PDCS_ERR_ARG, "problem is NULL");
+    pdcs_ctx *ctx = P->ctx;
+    if (!P->rsoc_rows.empty())
+        return fail(ctx, PDCS_ERR_UNSUPPORTED, "data update of a problem with rotated-SOC blocks");
+    const int64_t nb = P->nb;
+    // validate before changing anything
+    const double *nbl = bl ? bl : P->bl.data(), *nbu = bu ? bu : P->bu.data();
+    for (int64_t j = 0; j < nb; ++j)
+        if (!(nbl[j] <= nbu[j]))
+            return fail(ctx, PDCS_ERR_ARG, "Not all elements of bl are less than or equal to the corresponding elements in bu");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (c) P->c.assign(c, c + P->n);
+    if (vals) P->vals.assign(vals, vals + P->nnz);
+    if (h) P->h.assign(h, h + P->m);
+    if (bl) P->bl.assign(bl, bl + nb);
+    if (bu) P->bu.assign(bu, bu + nb);
+    int rc = upload_problem_data(P);
+    if (!rc) rc = scale_device(P);
+    if (!rc) rc = finish_scaling(P);
+    if (rc) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    P->f64.stale = P->f32.stale = true;
+    P->solved = false;
+    return PDCS_OK;
+}
+
+int pdcs_problem_update_device(pdcs_problem *P, const double *c, const double *vals, const double *h,
+                               const double *bl, const double *bu) {
+    if (!P) return fail(nullptr, PDCS_ERR_ARG, "problem is NULL");
+    pdcs_ctx *ctx = P->ctx;
+    std::vector<double> hc, hv, hh, hbl, hbu;
+    auto stage = [&](std::vector<double> &dst, const double *src, int64_t len) -> int {
+        if (!src) return PDCS_OK;
+        dst.resize(len);
+        if (len > 0 && copy_sync(ctx, dst.data(), src, sizeof(double) * len, hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(ctx, PDCS_ERR_ARG, "update input is not a device pointer");
+        return PDCS_OK;
+    };
+    int rc = stage(hc, c, P->n);
+    if (!rc) rc = stage(hv, vals, P->nnz);
+    if (!rc) rc = stage(hh, h, P->m);
+    if (!rc) rc = stage(hbl, bl, P->nb);
+    if (!rc) rc = stage(hbu, bu, P->nb);
+    if (rc) return rc;
+    return pdcs_problem_update_host(P, c ? hc.data() : nullptr, vals ? hv.data() : nullptr, h ? hh.data() : nullptr,
+                                    bl ? hbl.data() : nullptr, bu ? hbu.data() : nullptr);
 }
 
 int pdcs_problem_destroy(pdcs_problem *p) {

@@ -86,7 +86,7 @@ def make_cones(mzero=0, mnonneg=0, soc=(), rsoc=(), nexp=0, ndexp=0):
 
 
 class Problem:
-    """A conic program resident on the GPU (scaled once at creation)."""
+    """A conic program resident on the GPU (scaled at creation and at each data update)."""
 
     def __init__(self, ctx: Context, n, m, nb, c, rowptr, colidx, vals, h, bl, bu, cones):
         self.ctx = ctx
@@ -111,6 +111,25 @@ class Problem:
         ctx = ctx or default_context()
         cones = make_cones(P.mzero, P.mnonneg, P.soc, getattr(P, "rsoc", ()), P.nexp, P.ndexp)
         return cls(ctx, P.n, P.m, P.nb, P.c, P.rowptr, P.colidx, P.vals, P.h, P.bl, P.bu, cones)
+
+    def update(self, c=None, vals=None, h=None, bl=None, bu=None):
+        """New data on the same sparsity pattern (pdcs_problem_update_host): c,
+        the CSR values of G in the creation order, h, bl, bu; None keeps one.
+        The problem is rescaled and the last solution dropped."""
+        new = {0: c, 3: vals, 4: h, 5: bl, 6: bu}
+        sizes = {0: self.n, 3: len(self._arrs[3]), 4: self.m, 5: self.nb, 6: self.nb}
+        staged = {}
+        for i, v in new.items():
+            if v is not None:
+                a = np.ascontiguousarray(v, dtype=np.float64)
+                if a.shape != (sizes[i],):
+                    raise ValueError(f"update array {i} has shape {a.shape}, expected ({sizes[i]},)")
+                staged[i] = a
+        ptr = {i: _p(a) for i, a in staged.items()}
+        _lib.check(self.ctx.L.pdcs_problem_update_host(self.h, ptr.get(0), ptr.get(3), ptr.get(4), ptr.get(5),
+                                                       ptr.get(6)), self.ctx.h)
+        for i, a in staged.items():
+            self._arrs[i] = a
 
     def solve(self, params: _lib.Params | None = None, x0=None, y0=None, **kw):
         prm = params or _lib.default_params()
