@@ -245,6 +245,15 @@ int pdcs_batch_create_host(pdcs_ctx *ctx, int64_t count, const pdcs_batch_shape 
                            const int32_t *rowptr, const int32_t *colidx, const double *vals, const double *h,
                            const double *bl, const double *bu, const int64_t *soc, pdcs_batch **out);
 int pdcs_batch_destroy(pdcs_batch *b);
+/* New data on every instance's fixed sparsity pattern, flat arrays laid out as
+ * at creation: c (sum n_i), vals (sum nnz_i), h (sum m_i), bl/bu (sum nb_i);
+ * a NULL array keeps its current data, bl and bu are given together.  The
+ * scaling is rebuilt when vals change; the previous solution is dropped.
+ * Batched form of the re-solve with a fixed pattern of the reference's
+ * Python bridge (update_G! / update_solver!,
+ * src/pdcs_gpu/cvxpy_wrapper/data_updating.jl:4-75). */
+int pdcs_batch_update_host(pdcs_batch *b, const double *c, const double *vals, const double *h, const double *bl,
+                           const double *bu);
 /* Solves every instance (params as pdcs_solve; PDCS_MIXED runs an FP32 launch
  * then an FP64 polish launch over the instances it handed over).  res: count
  * entries; *time_total: seconds from the first launch to the end of the last. */

@@ -95,6 +95,7 @@ struct BatchData {
 
 // launchers (batch.hip)
 void launch_batch_scale(hipStream_t s, const BatchData &B);
+void launch_batch_tvals(hipStream_t s, const BatchData &B);   // tval64 from val64 (transpose map)
 template <typename T>
 void launch_batch_prepare(hipStream_t s, const BatchData &B, BatchVecs<T> V);
 template <typename T>

@@ -861,8 +861,16 @@ __global__ __launch_bounds__(kBlock) void k_batch_recover(BatchData B, BatchVecs
     }
 }
 
+// G' values of every instance from its G values through the transpose map
+// (instance-local slots), one workgroup per instance
+__global__ __launch_bounds__(kBlock) void k_batch_tvals(BatchData B) {
+    const BatchInst I = B.inst[blockIdx.x];
+    for (int32_t k = threadIdx.x; k < I.nnz; k += kBlock) B.tval64[I.ko + k] = B.val64[I.ko + B.tmap[I.ko + k]];
+}
+
 // ------------------------------------------------------------ launchers --
 void launch_batch_scale(hipStream_t s, const BatchData &B) { k_batch_scale<<<B.count, kBlock, 0, s>>>(B); }
+void launch_batch_tvals(hipStream_t s, const BatchData &B) { k_batch_tvals<<<B.count, kBlock, 0, s>>>(B); }
 
 template <typename T>
 void launch_batch_prepare(hipStream_t s, const BatchData &B, BatchVecs<T> V) {

@@ -275,6 +275,87 @@ int pdcs_batch_create_host(pdcs_ctx *ctx, int64_t count, const pdcs_batch_shape 
     return PDCS_OK;
 }
 
+int pdcs_batch_update_host(pdcs_batch *b, const double *c, const double *vals, const double *h, const double *bl,
+                           const double *bu) {
+    if (!b) return fail(nullptr, PDCS_ERR_ARG, "batch is NULL");
+    pdcs_ctx *ctx = b->ctx;
+    if ((bl == nullptr) != (bu == nullptr)) return fail(ctx, PDCS_ERR_ARG, "bl and bu are updated together");
+    // validate before changing anything (bounds: sum of nb over the instances, creation order)
+    if (bl) {
+        int64_t bo = 0;
+        for (int64_t q = 0; q < b->count; ++q) {
+            for (int64_t j = 0; j < b->inst[q].nb; ++j)
+                if (!(bl[bo + j] <= bu[bo + j]))
+                    return fail(ctx, PDCS_ERR_ARG, "instance %ld: Not all elements of bl are less than or equal to the corresponding elements in bu", (long)q);
+            bo += b->inst[q].nb;
+        }
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    BatchData &B = b->B;
+    auto up = [&](const double *dst, const double *src, int64_t len) -> int {
+        HIPCHK(ctx, copy_sync(ctx, (void *)dst, src, sizeof(double) * (size_t)len, hipMemcpyHostToDevice));
+        return PDCS_OK;
+    };
+    int rc;
+    if (c) {
+        if ((rc = up(B.c_raw, c, b->sum_n))) return rc;
+        for (auto &I : b->inst) {
+            I.cN1 = I.cNinf = 0.0;
+            for (int64_t j = 0; j < I.n; ++j) { I.cN1 += std::fabs(c[I.xo + j]); I.cNinf = std::max(I.cNinf, std::fabs(c[I.xo + j])); }
+        }
+    }
+    if (h) {
+        if ((rc = up(B.h_raw, h, b->sum_m))) return rc;
+        for (auto &I : b->inst) {
+            I.hN1 = I.hNinf = 0.0;
+            for (int64_t i = 0; i < I.m; ++i) { I.hN1 += std::fabs(h[I.yo + i]); I.hNinf = std::max(I.hNinf, std::fabs(h[I.yo + i])); }
+        }
+    }
+    if (bl) {
+        // the n-long bound vectors as at creation (free past nb)
+        std::vector<double> blr, bur, blf, buf, laml, lamu;
+        int64_t bo = 0;
+        for (const auto &I : b->inst) {
+            for (int64_t j = 0; j < I.n; ++j) {
+                const bool box = j < I.nb;
+                const double l = box ? bl[bo + j] : -INFINITY, u = box ? bu[bo + j] : INFINITY;
+                blr.push_back(l);
+                bur.push_back(u);
+                blf.push_back(std::isinf(l) ? 0.0 : l);
+                buf.push_back(std::isinf(u) ? 0.0 : u);
+                laml.push_back(u == INFINITY ? 0.0 : -INFINITY);
+                lamu.push_back(l == -INFINITY ? 0.0 : INFINITY);
+            }
+            bo += I.nb;
+        }
+        if ((rc = up(B.bl_raw, blr.data(), b->sum_n)) || (rc = up(B.bu_raw, bur.data(), b->sum_n)) ||
+            (rc = up(B.blf, blf.data(), b->sum_n)) || (rc = up(B.buf, buf.data(), b->sum_n)) ||
+            (rc = up(B.laml, laml.data(), b->sum_n)) || (rc = up(B.lamu, lamu.data(), b->sum_n)))
+            return rc;
+    }
+    if (vals) {
+        // raw values, their transpose, then the scaling (it depends on G only)
+        if ((rc = up(B.val64, vals, b->sum_nnz))) return rc;
+        launch_batch_tvals(s, B);
+        launch_batch_scale(s, B);
+        if (b->val32) {
+            launch_batch_f32(s, B.val64, b->val32, b->sum_nnz);
+            launch_batch_f32(s, B.tval64, b->tval32, b->sum_nnz);
+        }
+    }
+    if (c || h) {
+        HIPCHK(ctx, copy_sync(ctx, (void *)B.inst, b->inst.data(), sizeof(BatchInst) * b->count, hipMemcpyHostToDevice));
+        if (b->d64.views && (rc = build_views<double>(b, b->d64))) return rc;
+        if (b->d32.views && (rc = build_views<float>(b, b->d32))) return rc;
+    }
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    b->solved = false;
+    return PDCS_OK;
+}
+
 int pdcs_batch_destroy(pdcs_batch *b) {
     if (!b) return PDCS_OK;
     hipSetDevice(b->ctx->device);

@@ -234,6 +234,33 @@ class Batch:
         self._yo = np.cumsum([0] + [P.m for P in self.problems])
         _live_problems.add(self)
 
+    def update(self, c=None, vals=None, h=None, bl=None, bu=None):
+        """New data on every instance's sparsity pattern (pdcs_batch_update_host):
+        each argument is a list with one array per instance (c, the CSR values
+        of G in the creation order, h, bl[:nb], bu[:nb]) or None to keep it.
+        The scaling is rebuilt and the last solutions dropped."""
+        P = self.problems
+        new = {0: (c, [p.n for p in P]), 3: (vals, [p.nnz for p in P]), 4: (h, [p.m for p in P]),
+               5: (bl, [p.nb for p in P]), 6: (bu, [p.nb for p in P])}
+        staged = {}
+        for i, (v, sizes) in new.items():
+            if v is None:
+                continue
+            parts = list(v)
+            if len(parts) != len(P):
+                raise ValueError(f"update array {i}: {len(parts)} parts for {len(P)} instances")
+            parts = [np.asarray(a, dtype=np.float64).reshape(-1)[:sz] if i in (5, 6) else np.asarray(a, dtype=np.float64)
+                     for a, sz in zip(parts, sizes)]
+            for q, (a, sz) in enumerate(zip(parts, sizes)):
+                if a.shape != (sz,):
+                    raise ValueError(f"update array {i}, instance {q}: shape {a.shape}, expected ({sz},)")
+            staged[i] = np.ascontiguousarray(np.concatenate(parts) if parts else np.zeros(0), dtype=np.float64)
+        ptr = {i: _p(a) for i, a in staged.items()}
+        _lib.check(self.ctx.L.pdcs_batch_update_host(self.h, ptr.get(0), ptr.get(3), ptr.get(4), ptr.get(5),
+                                                     ptr.get(6)), self.ctx.h)
+        for i, a in staged.items():
+            self._arrs[i] = a
+
     def solve(self, params: _lib.Params | None = None, **kw):
         """Returns (list of per-instance result dicts, total seconds of the launches)."""
         prm = params or _lib.default_params()

@@ -1,0 +1,106 @@
+"""GPU: new data on every instance's fixed sparsity pattern
+(pdcs_batch_update_host), the batched form of the re-solve path of the
+reference's Python bridge (update_G! / update_solver!,
+src/pdcs_gpu/cvxpy_wrapper/data_updating.jl:4-75).
+
+A batch created with data D1 and updated to D2 must solve exactly as one
+created with D2: one workgroup runs each instance, so its result depends only
+on that instance's data and the parameters.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from pdcs_amd import _lib, instances
+from pdcs_amd.instances import _soc_interior
+from pdcs_amd.solver import Batch
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _same_pattern(P, seed):
+    """New feasible data on the pattern of P: G values rescaled entrywise,
+    h = G x0 - s0, c = G' y0 + noise, a new box on the first nb columns."""
+    rng = np.random.default_rng(seed)
+    Q = dataclasses.replace(P, vals=P.vals * rng.uniform(0.5, 1.5, P.vals.shape))
+    x0 = rng.uniform(-4.0, 4.0, P.n)
+    Q.h = Q.matvec(x0) - _soc_interior(rng, P.soc)
+    Q.c = Q.rmatvec(_soc_interior(rng, P.soc)) + rng.normal(0.0, 0.1, P.n)
+    Q.bl = np.where(np.arange(P.n) < P.nb, -8.0, -np.inf)
+    Q.bu = np.where(np.arange(P.n) < P.nb, 8.0, np.inf)
+    return Q
+
+
+def _new_rhs(P, seed):
+    """A new feasible h on the same G: h = G x0 - s0, x0 inside the box."""
+    rng = np.random.default_rng(seed)
+    x0 = rng.uniform(-4.0, 4.0, P.n)
+    return P.matvec(x0) - _soc_interior(rng, P.soc)
+
+
+def _fields(r):
+    return (r["exit_code"], r["iter"], r["pobj"], r["dobj"], r["rel_primal_res"], r["rel_dual_res"], r["rel_gap"])
+
+
+def _programs(seed):
+    return [instances.soc_program(40 + 20 * i, 90 + 30 * i, 0.08, 5, seed=seed + i) for i in range(6)]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_update_matches_fresh_batch(ctx, precision):
+    P1 = _programs(300)
+    P2 = [_same_pattern(P, 400 + i) for i, P in enumerate(P1)]
+    fresh = Batch(P2, ctx)
+    r_fresh, _ = fresh.solve(precision=precision, time_limit=30.0)
+    s_fresh = fresh.solutions()
+    B = Batch(P1, ctx)
+    r1, _ = B.solve(precision=precision, time_limit=30.0)      # fills the scaled copies of D1
+    assert all(r["exit_code"] == 0 for r in r1)
+    B.update(c=[P.c for P in P2], vals=[P.vals for P in P2], h=[P.h for P in P2],
+             bl=[P.bl[:P.nb] for P in P2], bu=[P.bu[:P.nb] for P in P2])
+    r2, _ = B.solve(precision=precision, time_limit=30.0)
+    s2 = B.solutions()
+    for q, P in enumerate(P2):
+        print(f"prec {precision} instance {q}: fresh {_fields(r_fresh[q])} updated {_fields(r2[q])}", flush=True)
+        assert _fields(r2[q]) == _fields(r_fresh[q])
+        assert np.array_equal(s2[q][0], s_fresh[q][0]) and np.array_equal(s2[q][1], s_fresh[q][1])
+        assert r2[q]["exit_code"] == 0
+        # the new data solved by the oracle to 1e-9; README tolerance 1e-6 on the GPU side
+        ref, _, _ = oracle.solve(P, rel_tol=1e-9, abs_tol=1e-12)
+        for k in ("pobj", "dobj"):
+            assert abs(r2[q][k] - ref[k]) <= 2e-6 * (1 + abs(ref[k])), (q, k, r2[q][k], ref[k])
+        chk = oracle.kkt(P, s2[q][0], s2[q][1])
+        assert max(chk["rel_primal_res"], chk["rel_dual_res"], chk["rel_gap"]) < 1e-6, (q, chk)
+
+
+def test_batch_partial_update_and_rejections(ctx):
+    P1 = _programs(500)
+    P2 = [_same_pattern(P, 600 + i) for i, P in enumerate(P1)]
+    Ph = [dataclasses.replace(P, h=_new_rhs(P, 700 + i)) for i, P in enumerate(P1)]     # only h changes
+    fresh = Batch(Ph, ctx)
+    r_fresh, _ = fresh.solve(time_limit=30.0)
+    B = Batch(P1, ctx)
+    r1, _ = B.solve(time_limit=30.0)
+    B.update(h=[P.h for P in Ph])
+    r2, _ = B.solve(time_limit=30.0)
+    assert [_fields(r) for r in r2] == [_fields(r) for r in r_fresh]
+    # bl > bu in one instance is rejected before anything changes
+    bad_l = [P.bl[:P.nb].copy() for P in P1]
+    bad_u = [P.bu[:P.nb].copy() for P in P1]
+    bad_l[3][:] = 1.0
+    bad_u[3][:] = -1.0
+    with pytest.raises(_lib.PdcsError):
+        B.update(bl=bad_l, bu=bad_u)
+    r3, _ = B.solve(time_limit=30.0)
+    assert [_fields(r) for r in r3] == [_fields(r) for r in r2]
+    # bounds alone need both sides
+    with pytest.raises(_lib.PdcsError):
+        B.update(bl=[P.bl[:P.nb] for P in P1])
+    # back to D1: the first solve again, bit for bit
+    B.update(h=[P.h for P in P1])
+    r4, _ = B.solve(time_limit=30.0)
+    assert [_fields(r) for r in r4] == [_fields(r) for r in r1]
