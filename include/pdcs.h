@@ -258,6 +258,12 @@ int pdcs_batch_update_host(pdcs_batch *b, const double *c, const double *vals, c
  * then an FP64 polish launch over the instances it handed over).  res: count
  * entries; *time_total: seconds from the first launch to the end of the last. */
 int pdcs_batch_solve(pdcs_batch *b, const pdcs_params *params, pdcs_batch_result *res, double *time_total);
+/* The same from unscaled warm starts x0 (sum n_i) / y0 (sum m_i), flat host
+ * arrays, used when params->warm_start (batched form of pdcs_solve's warm
+ * start; the warm-start branch of rpdhg_gpu_solve,
+ * src/pdcs_gpu/rpdhg_alg_gpu_gen.jl:1350-1372, applied to every instance). */
+int pdcs_batch_solve_warm(pdcs_batch *b, const pdcs_params *params, const double *x0, const double *y0,
+                          pdcs_batch_result *res, double *time_total);
 /* unscaled solutions of the last solve, flat: x sum n_i, y sum m_i */
 int pdcs_batch_get_solution(pdcs_batch *b, double *x, double *y);
 

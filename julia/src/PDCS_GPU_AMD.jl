@@ -215,16 +215,19 @@ struct PdcsBatchResult
 end
 
 """
-    batch_solve(problems; precision = :fp64, rel_tol = 1e-6, abs_tol = 1e-6, time_limit = Inf, device = 0)
+    batch_solve(problems; precision = :fp64, rel_tol = 1e-6, abs_tol = 1e-6, time_limit = Inf, device = 0,
+                x0 = nothing, y0 = nothing)
 
 Solves independent programs in one persistent launch.  Each element of
 `problems` has fields `c, G, h, bl, bu, mzero, mnonneg, soc` (and optionally
 `nexp`, `ndualexp`; rows of G in that cone order, the first `length(bl)`
 variables boxed).  Returns `(results, xs, ys, time_total)` with one
-`PdcsBatchResult` and one unscaled solution per program.
+`PdcsBatchResult` and one unscaled solution per program.  `x0` / `y0`: optional
+unscaled warm starts, one vector per program.
 """
 function batch_solve(problems; precision::Symbol = :fp64, rel_tol::Float64 = 1e-6, abs_tol::Float64 = 1e-6,
-                     time_limit::Float64 = Inf, device::Integer = 0)
+                     time_limit::Float64 = Inf, device::Integer = 0, x0 = nothing, y0 = nothing)
+    (x0 === nothing) == (y0 === nothing) || throw(ArgumentError("x0 and y0 are given together"))
     count = length(problems)
     shapes = PdcsBatchShape[]
     c = Float64[]; h = Float64[]; bl = Float64[]; bu = Float64[]; vals = Float64[]
@@ -253,8 +256,19 @@ function batch_solve(problems; precision::Symbol = :fp64, rel_tol::Float64 = 1e-
         ccall((:pdcs_params_default, libpdcs[]), Cvoid, (Ref{PdcsParams},), prm)
         prm.rel_tol = rel_tol; prm.abs_tol = abs_tol; prm.time_limit = time_limit
         prm.precision = precision == :mixed ? 1 : 0
-        check(ccall((:pdcs_batch_solve, libpdcs[]), Cint,
-            (Ptr{Cvoid}, Ref{PdcsParams}, Ptr{PdcsBatchResult}, Ref{Float64}), bat[], prm, res, total), ctx[])
+        if x0 === nothing
+            check(ccall((:pdcs_batch_solve, libpdcs[]), Cint,
+                (Ptr{Cvoid}, Ref{PdcsParams}, Ptr{PdcsBatchResult}, Ref{Float64}), bat[], prm, res, total), ctx[])
+        else
+            # unscaled warm starts, one vector per instance (pdcs_batch_solve_warm)
+            all(length(x0[q]) == shapes[q].n && length(y0[q]) == shapes[q].m for q in 1:count) ||
+                throw(ArgumentError("Warmstart doesn't match the problem size"))
+            prm.warm_start = 1
+            xw = Float64.(reduce(vcat, x0)); yw = Float64.(reduce(vcat, y0))
+            check(ccall((:pdcs_batch_solve_warm, libpdcs[]), Cint,
+                (Ptr{Cvoid}, Ref{PdcsParams}, Ptr{Float64}, Ptr{Float64}, Ptr{PdcsBatchResult}, Ref{Float64}),
+                bat[], prm, xw, yw, res, total), ctx[])
+        end
         x = zeros(sum(s.n for s in shapes)); y = zeros(sum(s.m for s in shapes))
         check(ccall((:pdcs_batch_get_solution, libpdcs[]), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
             bat[], x, y), ctx[])

@@ -97,7 +97,8 @@ struct BatchData {
 void launch_batch_scale(hipStream_t s, const BatchData &B);
 void launch_batch_tvals(hipStream_t s, const BatchData &B);   // tval64 from val64 (transpose map)
 template <typename T>
-void launch_batch_prepare(hipStream_t s, const BatchData &B, BatchVecs<T> V);
+void launch_batch_prepare(hipStream_t s, const BatchData &B, BatchVecs<T> V, const double *wx,
+                          const double *wy);   // wx/wy: unscaled warm start (device, flat) or NULL
 template <typename T>
 void launch_batch_solve(hipStream_t s, const BatchData &B, const BView<T> *views, const BatchParams *p, int grid);
 void launch_batch_promote(hipStream_t s, const BatchData &B, BatchVecs<float> A, BatchVecs<double> D);

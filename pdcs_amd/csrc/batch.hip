@@ -80,7 +80,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_scale(BatchData B) {
 
 // scaled data in precision T, SOC scales d^2 = (Dl[start] / Dl[i])^2, zero iterates
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_batch_prepare(BatchData B, BatchVecs<T> V) {
+__global__ __launch_bounds__(kBlock) void k_batch_prepare(BatchData B, BatchVecs<T> V, const double *wx,
+                                                          const double *wy) {
     const int q = blockIdx.x;
     const BatchInst I = B.inst[q];
     const double *Dr = B.Dr + I.xo, *Dl = B.Dl + I.yo;
@@ -91,12 +92,16 @@ __global__ __launch_bounds__(kBlock) void k_batch_prepare(BatchData B, BatchVecs
             V.bl[g] = (T)(B.bl_raw[g] * Dr[j]);
             V.bu[g] = (T)(B.bu_raw[g] * Dr[j]);
         }
-        V.x[g] = V.xs[g] = V.xm[g] = V.x0[g] = V.x0l[g] = V.xbar[g] = (T)0;
+        // start point: zero, or the unscaled warm start times Dr
+        const T x = wx ? (T)(wx[g] * Dr[j]) : (T)0;
+        V.x[g] = V.xs[g] = V.xm[g] = V.x0[g] = V.x0l[g] = V.xbar[g] = x;
     }
     BFOR(i, I.m) {
         const int64_t g = I.yo + i;
         V.h[g] = (T)(B.h_raw[g] / Dl[i]);
-        V.y[g] = V.ys[g] = V.ym[g] = V.y0[g] = V.y0l[g] = V.dy[g] = (T)0;
+        const T y = wy ? (T)(wy[g] * Dl[i]) : (T)0;
+        V.y[g] = V.ys[g] = V.ym[g] = V.y0[g] = V.y0l[g] = y;
+        V.dy[g] = (T)0;
         B.dsc2[g] = 1.0;
     }
     __syncthreads();
@@ -873,11 +878,13 @@ void launch_batch_scale(hipStream_t s, const BatchData &B) { k_batch_scale<<<B.c
 void launch_batch_tvals(hipStream_t s, const BatchData &B) { k_batch_tvals<<<B.count, kBlock, 0, s>>>(B); }
 
 template <typename T>
-void launch_batch_prepare(hipStream_t s, const BatchData &B, BatchVecs<T> V) {
-    k_batch_prepare<T><<<B.count, kBlock, 0, s>>>(B, V);
+void launch_batch_prepare(hipStream_t s, const BatchData &B, BatchVecs<T> V, const double *wx, const double *wy) {
+    k_batch_prepare<T><<<B.count, kBlock, 0, s>>>(B, V, wx, wy);
 }
-template void launch_batch_prepare<double>(hipStream_t, const BatchData &, BatchVecs<double>);
-template void launch_batch_prepare<float>(hipStream_t, const BatchData &, BatchVecs<float>);
+template void launch_batch_prepare<double>(hipStream_t, const BatchData &, BatchVecs<double>, const double *,
+                                           const double *);
+template void launch_batch_prepare<float>(hipStream_t, const BatchData &, BatchVecs<float>, const double *,
+                                          const double *);
 
 void launch_batch_f32(hipStream_t s, const double *a, float *b, int64_t n) {
     int64_t g = (n + kBlock - 1) / kBlock;

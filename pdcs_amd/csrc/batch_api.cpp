@@ -43,6 +43,7 @@ struct pdcs_batch {
     BatchVecStore<double> d64;
     BatchVecStore<float> d32;
     float *val32 = nullptr, *tval32 = nullptr;
+    double *wx = nullptr, *wy = nullptr;   // staged warm start (sum n, sum m)
     double *xo = nullptr, *yo = nullptr;   // unscaled solution (device)
     BatchParams *params = nullptr;         // device copy of the solve controls
     bool solved = false;
@@ -364,6 +365,11 @@ int pdcs_batch_destroy(pdcs_batch *b) {
 }
 
 int pdcs_batch_solve(pdcs_batch *b, const pdcs_params *prm_in, pdcs_batch_result *res, double *time_total) {
+    return pdcs_batch_solve_warm(b, prm_in, nullptr, nullptr, res, time_total);
+}
+
+int pdcs_batch_solve_warm(pdcs_batch *b, const pdcs_params *prm_in, const double *x0, const double *y0,
+                          pdcs_batch_result *res, double *time_total) {
     if (!b || !res) return fail(b ? b->ctx : nullptr, PDCS_ERR_ARG, "NULL argument");
     pdcs_params prm;
     if (prm_in) prm = *prm_in; else pdcs_params_default(&prm);
@@ -413,9 +419,18 @@ int pdcs_batch_solve(pdcs_batch *b, const pdcs_params *prm_in, pdcs_batch_result
     hipEvent_t e0, e1;
     HIPCHK(ctx, hipEventCreate(&e0));
     HIPCHK(ctx, hipEventCreate(&e1));
+    const double *wx = nullptr, *wy = nullptr;
+    if (prm.warm_start && x0 && y0) {
+        if (!b->wx && (!(b->wx = b->mem.alloc<double>(b->sum_n)) || !(b->wy = b->mem.alloc<double>(b->sum_m))))
+            return fail(ctx, PDCS_ERR_HIP, "batch allocation failed");
+        HIPCHK(ctx, copy_sync(ctx, b->wx, x0, sizeof(double) * b->sum_n, hipMemcpyHostToDevice));
+        HIPCHK(ctx, copy_sync(ctx, b->wy, y0, sizeof(double) * b->sum_m, hipMemcpyHostToDevice));
+        wx = b->wx;
+        wy = b->wy;
+    }
     HIPCHK(ctx, hipMemsetAsync(b->B.out, 0, sizeof(BatchOut) * b->count, s));
-    launch_batch_prepare<double>(s, b->B, b->d64.v);
-    if (mixed) launch_batch_prepare<float>(s, b->B, b->d32.v);
+    launch_batch_prepare<double>(s, b->B, b->d64.v, wx, wy);
+    if (mixed) launch_batch_prepare<float>(s, b->B, b->d32.v, wx, wy);
     HIPCHK(ctx, hipEventRecord(e0, s));
     if (mixed) {
         P.check_freq = prm.mixed_check_freq > 0 ? prm.mixed_check_freq : prm.check_terminate_freq;

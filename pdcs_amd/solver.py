@@ -261,14 +261,30 @@ class Batch:
         for i, a in staged.items():
             self._arrs[i] = a
 
-    def solve(self, params: _lib.Params | None = None, **kw):
-        """Returns (list of per-instance result dicts, total seconds of the launches)."""
+    def solve(self, params: _lib.Params | None = None, x0=None, y0=None, **kw):
+        """Returns (list of per-instance result dicts, total seconds of the launches).
+        x0 / y0: optional unscaled warm starts, one array per instance (both or neither)."""
         prm = params or _lib.default_params()
         for k, v in kw.items():
             setattr(prm, k, v)
         res = (_lib.BatchResult * len(self.problems))()
         tt = C.c_double(0.0)
-        _lib.check(self.ctx.L.pdcs_batch_solve(self.h, C.byref(prm), res, C.byref(tt)), self.ctx.h)
+        if (x0 is None) != (y0 is None):
+            raise ValueError("x0 and y0 are given together")
+        if x0 is None:
+            _lib.check(self.ctx.L.pdcs_batch_solve(self.h, C.byref(prm), res, C.byref(tt)), self.ctx.h)
+        else:
+            xs, ys = list(x0), list(y0)
+            if len(xs) != len(self.problems) or len(ys) != len(self.problems):
+                raise ValueError("one warm start per instance")
+            for q, P in enumerate(self.problems):
+                if np.shape(xs[q]) != (P.n,) or np.shape(ys[q]) != (P.m,):
+                    raise ValueError(f"instance {q}: warm start shapes {np.shape(xs[q])}, {np.shape(ys[q])}")
+            xw = np.ascontiguousarray(np.concatenate(xs), dtype=np.float64)
+            yw = np.ascontiguousarray(np.concatenate(ys), dtype=np.float64)
+            prm.warm_start = 1
+            _lib.check(self.ctx.L.pdcs_batch_solve_warm(self.h, C.byref(prm), _p(xw), _p(yw), res, C.byref(tt)),
+                       self.ctx.h)
         return [r.as_dict() for r in res], tt.value
 
     def solutions(self):

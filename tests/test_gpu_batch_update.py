@@ -104,3 +104,33 @@ def test_batch_partial_update_and_rejections(ctx):
     B.update(h=[P.h for P in P1])
     r4, _ = B.solve(time_limit=30.0)
     assert [_fields(r) for r in r4] == [_fields(r) for r in r1]
+
+
+def test_batch_warm_start(ctx):
+    """pdcs_batch_solve_warm follows the oracle's warm start (x0 Dr, y0 Dl as
+    the first iterate, rpdhg_alg_gpu_gen.jl:1350-1372): the same objectives,
+    the same iteration counts on most instances; a zero warm start is the cold
+    solve bit for bit."""
+    P1 = _programs(800)
+    rng = np.random.default_rng(801)
+    x0 = [rng.uniform(-1.0, 1.0, P.n) for P in P1]
+    y0 = [_soc_interior(rng, P.soc) * 0.1 for P in P1]
+    B = Batch(P1, ctx)
+    r_cold, _ = B.solve(time_limit=30.0)
+    r_zero, _ = B.solve(x0=[np.zeros(P.n) for P in P1], y0=[np.zeros(P.m) for P in P1], time_limit=30.0)
+    assert [_fields(r) for r in r_zero] == [_fields(r) for r in r_cold]
+    r_w, _ = B.solve(x0=x0, y0=y0, time_limit=30.0)
+    s_w = B.solutions()
+    same = 0
+    for q, P in enumerate(P1):
+        ref, _, _ = oracle.solve(P, x0=x0[q], y0=y0[q])
+        print(f"instance {q}: oracle iter {ref['iter']} batch iter {r_w[q]['iter']}", flush=True)
+        assert r_w[q]["exit_code"] == 0
+        same += int(ref["iter"] == r_w[q]["iter"])
+        for k in ("pobj", "dobj"):
+            assert abs(r_w[q][k] - ref[k]) <= 2e-6 * (1 + abs(ref[k])), (q, k, r_w[q][k], ref[k])
+        chk = oracle.kkt(P, s_w[q][0], s_w[q][1])
+        assert max(chk["rel_primal_res"], chk["rel_dual_res"], chk["rel_gap"]) < 1e-6, (q, chk)
+    assert same >= len(P1) - 2
+    with pytest.raises(ValueError):
+        B.solve(x0=x0)

@@ -91,3 +91,28 @@ def test_partial_update_and_rejections(ctx):
     pr = Problem.from_instance(R, ctx)
     with pytest.raises(_lib.PdcsError):
         pr.update(c=R.c)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_warm_start_matches_oracle(ctx, precision):
+    """pdcs_solve with params.warm_start: the unscaled (x0, y0) times (Dr, Dl)
+    is the first iterate (rpdhg_alg_gpu_gen.jl:1350-1372), as in the oracle."""
+    P = _same_pattern(instances.soc_program(300, 600, 0.02, 10, seed=31), 32)
+    rng = np.random.default_rng(52)
+    x0 = rng.uniform(-1.0, 1.0, P.n)
+    y0 = _soc_interior(rng, P.soc) * 0.1
+    ref, _, _ = oracle.solve(P, x0=x0, y0=y0)
+    opt, _, _ = oracle.solve(P, rel_tol=1e-8, abs_tol=1e-12)
+    prob = Problem.from_instance(P, ctx)
+    r = prob.solve(precision=precision, x0=x0, y0=y0, time_limit=30.0)
+    x, y, _ = prob.solution()
+    print(f"prec {precision}: oracle iter {ref['iter']} gpu iter {r['iter']}", flush=True)
+    assert r["exit_code"] == 0
+    for k in ("pobj", "dobj"):
+        assert abs(r[k] - opt[k]) <= 2e-6 * (1 + abs(opt[k])), (k, r[k], opt[k])
+    chk = oracle.kkt(P, x, y)
+    assert max(chk["rel_primal_res"], chk["rel_dual_res"], chk["rel_gap"]) < 1e-6, chk
+    # a zero warm start is the cold solve
+    cold = prob.solve(precision=precision, autotune=0, time_limit=30.0)
+    zero = prob.solve(precision=precision, autotune=0, x0=np.zeros(P.n), y0=np.zeros(P.m), time_limit=30.0)
+    assert _fields(zero) == _fields(cold)
