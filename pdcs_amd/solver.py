@@ -76,6 +76,28 @@ def default_context() -> Context:
     return _default_ctx
 
 
+def _device_ptr(t, name, length, kind):
+    """Device address of a contiguous GPU tensor (torch or any object with
+    data_ptr / is_cuda / dtype / numel) of `length` elements of `kind`."""
+    want = {"f64": "float64", "i32": "int32"}[kind]
+    if not getattr(t, "is_cuda", False):
+        raise ValueError(f"{name}: expected a GPU tensor")
+    if str(t.dtype).split(".")[-1] != want:
+        raise ValueError(f"{name}: dtype {t.dtype}, expected {want}")
+    if not t.is_contiguous() or t.numel() != length:
+        raise ValueError(f"{name}: expected {length} contiguous elements, got {t.numel()}")
+    return C.c_void_p(t.data_ptr())
+
+
+def _sync_producers(tensors):
+    """Wait for the work that wrote the tensors (the library reads them on its
+    own stream)."""
+    import torch
+    for t in tensors:
+        if t is not None:
+            torch.cuda.current_stream(t.device).synchronize()
+
+
 def make_cones(mzero=0, mnonneg=0, soc=(), rsoc=(), nexp=0, ndexp=0):
     soc_a = np.ascontiguousarray(np.asarray(list(soc), dtype=np.int64).reshape(-1))
     rsoc_a = np.ascontiguousarray(np.asarray(list(rsoc), dtype=np.int64).reshape(-1))
@@ -99,12 +121,43 @@ class Problem:
                       np.ascontiguousarray(bl, dtype=np.float64),
                       np.ascontiguousarray(bu, dtype=np.float64)]
         a = self._arrs
+        self.nnz = len(a[3])
         self.cones = cones
         self.h = C.c_void_p()
         _lib.check(ctx.L.pdcs_problem_create_host(ctx.h, self.n, self.m, self.nb, _p(a[0]), _p(a[1], C.c_int32),
                                                   _p(a[2], C.c_int32), _p(a[3]), _p(a[4]), _p(a[5]), _p(a[6]),
                                                   C.byref(cones), C.byref(self.h)), ctx.h)
         _live_problems.add(self)
+
+    @classmethod
+    def from_tensors(cls, n, m, nb, c, rowptr, colidx, vals, h, bl, bu, cones, ctx: Context | None = None):
+        """A problem from GPU tensors (pdcs_problem_create_device): c (n), CSR
+        rowptr (m+1) / colidx (nnz) int32, vals (nnz), h (m), bl / bu (nb),
+        float64, contiguous; cones from make_cones."""
+        ctx = ctx or default_context()
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.n, self.m, self.nb = int(n), int(m), int(nb)
+        self.nnz = int(vals.numel())
+        self.cones = cones
+        self.h = C.c_void_p()
+        ptrs = [_device_ptr(c, "c", self.n, "f64"), _device_ptr(rowptr, "rowptr", self.m + 1, "i32"),
+                _device_ptr(colidx, "colidx", self.nnz, "i32"), _device_ptr(vals, "vals", self.nnz, "f64"),
+                _device_ptr(h, "h", self.m, "f64"), _device_ptr(bl, "bl", self.nb, "f64"),
+                _device_ptr(bu, "bu", self.nb, "f64")]
+        _sync_producers([c, rowptr, colidx, vals, h, bl, bu])
+        self._arrs = [None] * 7
+        _lib.check(ctx.L.pdcs_problem_create_device(ctx.h, self.n, self.m, self.nb, *ptrs, C.byref(cones),
+                                                    C.byref(self.h)), ctx.h)
+        _live_problems.add(self)
+        return self
+
+    def update_tensors(self, c=None, vals=None, h=None, bl=None, bu=None):
+        """Problem.update from GPU tensors (pdcs_problem_update_device)."""
+        spec = [(c, "c", self.n), (vals, "vals", self.nnz), (h, "h", self.m), (bl, "bl", self.nb), (bu, "bu", self.nb)]
+        ptrs = [_device_ptr(t, nm, ln, "f64") if t is not None else None for t, nm, ln in spec]
+        _sync_producers([c, vals, h, bl, bu])
+        _lib.check(self.ctx.L.pdcs_problem_update_device(self.h, *ptrs), self.ctx.h)
 
     @classmethod
     def from_instance(cls, P, ctx: Context | None = None):
@@ -117,7 +170,7 @@ class Problem:
         the CSR values of G in the creation order, h, bl, bu; None keeps one.
         The problem is rescaled and the last solution dropped."""
         new = {0: c, 3: vals, 4: h, 5: bl, 6: bu}
-        sizes = {0: self.n, 3: len(self._arrs[3]), 4: self.m, 5: self.nb, 6: self.nb}
+        sizes = {0: self.n, 3: self.nnz, 4: self.m, 5: self.nb, 6: self.nb}
         staged = {}
         for i, v in new.items():
             if v is not None:
@@ -155,7 +208,7 @@ class Problem:
         last solution, by implicit differentiation of the KKT fixed point."""
         gx = np.ascontiguousarray(dL_dx, dtype=np.float64)
         gy = np.ascontiguousarray(dL_dy, dtype=np.float64) if dL_dy is not None else None
-        dc, dh, dG = np.zeros(self.n), np.zeros(self.m), np.zeros(len(self._arrs[3]))
+        dc, dh, dG = np.zeros(self.n), np.zeros(self.m), np.zeros(self.nnz)
         _lib.check(self.ctx.L.pdcs_adjoint(self.h, _p(gx), _p(gy) if gy is not None else None, _p(dc), _p(dh), _p(dG)),
                    self.ctx.h)
         return dc, dh, dG

@@ -7,6 +7,14 @@ backward pass returns dL/dc, dL/dh and dL/dG (values on the CSR pattern) from
 the library's adjoint (pdcs_adjoint / pdcs_batch_adjoint).  The solve and the
 derivative both run on the GPU through libpdcs_hip.so; tensors are moved to
 host memory at the boundary (the C ABI takes host arrays).
+
+``ConicLayer(problem)`` keeps one problem resident on the GPU and re-solves it
+with each call's (c, h, vals) on the fixed sparsity pattern (the cvxpy-style
+re-solve of the reference's Python bridge, update_G! / update_solver!,
+src/pdcs_gpu/cvxpy_wrapper/data_updating.jl:4-75): GPU tensors go in through
+pdcs_problem_update_device, and a backward pass whose forward data has since
+been replaced re-solves that data first (a re-solve after an update equals
+the first solve bit for bit).
 """
 from __future__ import annotations
 
@@ -75,3 +83,57 @@ class BatchConicSolve(torch.autograd.Function):
                 raise RuntimeError("singular KKT Jacobian at a solution")
             out += [torch.as_tensor(a, dtype=torch.float64, device=ctx.dev) for a in g]
         return tuple(out)
+
+
+class _ResidentSolve(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, c, h, vals, layer):
+        layer._load(c, h, vals)
+        x = layer._solve()
+        ctx.layer = layer
+        ctx.gen = layer._gen
+        ctx.data = (c.detach(), h.detach(), vals.detach())
+        ctx.dev = c.device
+        return torch.as_tensor(x, dtype=torch.float64, device=c.device)
+
+    @staticmethod
+    def backward(ctx, gx):
+        layer = ctx.layer
+        if layer._gen != ctx.gen:            # a later call replaced the data: solve this call's again
+            layer._load(*ctx.data)
+            layer._solve()
+        dc, dh, dG = layer.prob.adjoint(_np(gx))
+        t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=ctx.dev)
+        return t(dc), t(dh), t(dG), None
+
+
+class ConicLayer(torch.nn.Module):
+    """x*(c, h, G) on the pattern, cones and bounds of `problem`, with the
+    problem kept resident between calls.  solve_kw: pdcs_params fields."""
+
+    def __init__(self, problem, ctx=None, **solve_kw):
+        super().__init__()
+        self.prob = Problem.from_instance(problem, ctx)
+        self.solve_kw = solve_kw
+        self._gen = 0
+
+    def _load(self, c, h, vals):
+        if c.is_cuda and h.is_cuda and vals.is_cuda:
+            d = lambda t: t.detach().to(torch.float64).contiguous()
+            self.prob.update_tensors(c=d(c), vals=d(vals), h=d(h))
+        else:
+            self.prob.update(c=_np(c), vals=_np(vals), h=_np(h))
+        self._gen += 1
+
+    def _solve(self):
+        res = self.prob.solve(**self.solve_kw)
+        if res["exit_code"] != 0:
+            raise RuntimeError(f"solve did not converge: exit code {res['exit_code']}")
+        x, _, _ = self.prob.solution()
+        return x
+
+    def forward(self, c, h, vals):
+        return _ResidentSolve.apply(c, h, vals, self)
+
+    def close(self):
+        self.prob.close()
