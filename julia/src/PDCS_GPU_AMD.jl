@@ -14,7 +14,8 @@ CSR the C ABI takes.  Beyond the reference's API it adds `batch_solve` (many
 independent programs in one persistent launch), `conic_layer` (a solve whose
 reverse-mode derivative, `rrule` in ext/PDCS_GPU_AMDChainRulesExt.jl, comes
 from `pdcs_adjoint`) and `update_solver!` / `resolve!` (new data on the
-pattern of a resident layer problem).  This package is not exercised by the repository's tests
+pattern of a resident layer problem).  `PDCS_GPU_AMD.Optimizer` (MOI_wrapper.jl)
+is the MathOptInterface / JuMP front-end.  This package is not exercised by the repository's tests
 (there is no Julia toolchain in the build image).
 """
 module PDCS_GPU_AMD
@@ -418,5 +419,7 @@ function resolve!(R::ResidentProblem; precision::Symbol = :fp64, rel_tol::Float6
         R.prob, x, y, s), R.ctx)
     return x
 end
+
+include("MOI_wrapper.jl")
 
 end # module
