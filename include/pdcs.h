@@ -138,6 +138,31 @@ int pdcs_problem_create_host(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, co
                              const int32_t *rowptr, const int32_t *colidx, const double *vals,
                              const double *h, const double *bl, const double *bu,
                              const pdcs_cones *cones, pdcs_problem **out);
+/* Primal-side cones: x = (box part of nb variables, SOC blocks soc[0..nsoc),
+ * rotated SOC blocks, nexp exponential and ndualexp dual exponential blocks of
+ * 3, remaining variables free), the reference's soc_x / rsoc_x / exp_x /
+ * dual_exp_x (rpdhg_alg_gpu_gen.jl:1234-1237, block layout :1393-1400). */
+typedef struct {
+    int64_t nsoc;
+    const int64_t *soc;
+    int64_t nrsoc;
+    const int64_t *rsoc;
+    int64_t nexp;
+    int64_t ndualexp;
+} pdcs_primal_cones;
+
+/* pdcs_problem_create_host with primal-side cones (NULL or all zero: the same
+ * as pdcs_problem_create_host).  The reference declares them but their
+ * projections are undefined (def_rpdhg_gen.jl:153,161,299-313).  Here each
+ * block x_B in K_B is solved as the extra rows I x_B - 0 in K_B after the
+ * caller's rows of the same cone type; solutions, scalings, warm starts and
+ * pdcs_evaluate use the caller's m rows, and the dual slack s = c - G'y of
+ * pdcs_get_solution holds the blocks' cone multipliers.  Data updates and the
+ * adjoint reject such problems (PDCS_ERR_UNSUPPORTED). */
+int pdcs_problem_create_host_x(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double *c,
+                               const int32_t *rowptr, const int32_t *colidx, const double *vals,
+                               const double *h, const double *bl, const double *bu,
+                               const pdcs_cones *cones, const pdcs_primal_cones *xcones, pdcs_problem **out);
 /* Same from device pointers (copied on the device; replaces
  * rpdhg_gpu_solve_input_gpu_data, rpdhg_alg_gpu_gen.jl:387). */
 int pdcs_problem_create_device(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double *c,

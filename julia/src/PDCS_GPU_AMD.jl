@@ -35,6 +35,15 @@ struct PdcsCones
     ndualexp::Int64
 end
 
+struct PdcsPrimalCones
+    nsoc::Int64
+    soc::Ptr{Int64}
+    nrsoc::Int64
+    rsoc::Ptr{Int64}
+    nexp::Int64
+    ndualexp::Int64
+end
+
 mutable struct PdcsParams
     max_outer_iter::Int64
     max_inner_iter::Int64
@@ -131,8 +140,6 @@ function rpdhg_gpu_solve(;
     kkt_restart_freq::Integer = 2000, use_duality_gap_restart::Bool = true,
     duality_gap_restart_freq::Integer = 2000, precision::Symbol = :fp64, device::Integer = 0)
 
-    (isempty(soc_x) && isempty(rsoc_x) && exp_x == 0 && dual_exp_x == 0) ||
-        throw(ArgumentError("primal-side cones are not supported"))
     (rescaling_method == :ruiz_pock_chambolle && use_preconditioner) ||
         throw(ArgumentError("only :ruiz_pock_chambolle with the preconditioner is supported"))
     (use_adaptive_restart && use_adaptive_step_size_weight && use_resolving && use_aggressive &&
@@ -142,19 +149,23 @@ function rpdhg_gpu_solve(;
     rowptr, colidx, vals = csr0(G)
     soc = Vector{Int64}(socG)
     rsoc = Vector{Int64}(rsocG)
+    xsoc = Vector{Int64}(soc_x)          # primal blocks after the nb box variables
+    xrsoc = Vector{Int64}(rsoc_x)
     ctx = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:pdcs_ctx_create, libpdcs[]), Cint, (Cint, Ptr{Ptr{Cvoid}}), device, ctx), C_NULL)
     prob = Ref{Ptr{Cvoid}}(C_NULL)
     res = PdcsResult()
     x = zeros(n); y = zeros(m); s = zeros(n)
     try
-        GC.@preserve soc rsoc begin
+        GC.@preserve soc rsoc xsoc xrsoc begin
             cones = Ref(PdcsCones(mGzero, mGnonnegative, length(soc), pointer(soc), length(rsoc), pointer(rsoc),
                                   expG, dual_expG))
-            check(ccall((:pdcs_problem_create_host, libpdcs[]), Cint,
+            xcones = Ref(PdcsPrimalCones(length(xsoc), pointer(xsoc), length(xrsoc), pointer(xrsoc),
+                                         exp_x, dual_exp_x))
+            check(ccall((:pdcs_problem_create_host_x, libpdcs[]), Cint,
                 (Ptr{Cvoid}, Int64, Int64, Int64, Ptr{Float64}, Ptr{Int32}, Ptr{Int32}, Ptr{Float64},
-                 Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{PdcsCones}, Ptr{Ptr{Cvoid}}),
-                ctx[], n, m, nb, c, rowptr, colidx, vals, h, bl, bu, cones, prob), ctx[])
+                 Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{PdcsCones}, Ptr{PdcsPrimalCones}, Ptr{Ptr{Cvoid}}),
+                ctx[], n, m, nb, c, rowptr, colidx, vals, h, bl, bu, cones, xcones, prob), ctx[])
         end
         prm = PdcsParams()
         ccall((:pdcs_params_default, libpdcs[]), Cvoid, (Ref{PdcsParams},), prm)

@@ -78,6 +78,15 @@ struct pdcs_problem {
     // rotation is orthogonal and its own inverse, so residuals, objectives and
     // G'y are unchanged; y goes back through the same rotation on output.
     std::vector<int64_t> rsoc_rows;
+    // Primal-side cone blocks (pdcs_problem_create_host_x): the variables x_B
+    // of each block are carried as extra rows I x_B - 0 in K_B after the
+    // caller's rows of the same cone type, so the primal projection stays the
+    // box.  user_rows[i] is the row of the caller's row i (empty without such
+    // blocks); a caller's y is lifted to the extra rows with the reduced cost
+    // c - G'y of their variables.
+    int64_t user_m = -1;
+    std::vector<int64_t> user_rows;
+    std::vector<int64_t> xrow_var;   // per library row: the variable of an extra row, -1 for the caller's rows
     // device structure
     int32_t *d_rowptr = nullptr, *d_col = nullptr, *d_trowptr = nullptr, *d_tcol = nullptr, *d_tmap = nullptr;
     double *d_vals = nullptr, *d_tvals = nullptr;     // scaled FP64
@@ -612,6 +621,38 @@ void rsoc_rotate(const pdcs_problem *P, double *y) {
         y[r0] = r * (a + b);
         y[r0 + 1] = r * (a - b);
     }
+}
+
+// caller's y (user_m rows) -> library rows: the caller's rows in place, each
+// extra primal-cone row set to the reduced cost c - G'y of its variable
+std::vector<double> lift_y(const pdcs_problem *P, const double *y) {
+    std::vector<double> ya(P->m, 0.0);
+    if (P->user_m < 0) {
+        std::copy(y, y + P->m, ya.begin());
+        return ya;
+    }
+    for (int64_t i = 0; i < P->user_m; ++i) ya[P->user_rows[i]] = y[i];
+    // G'y of the caller's rows, taken in the library's (rotated) rows: a
+    // rotation of a block keeps the block's contribution
+    std::vector<double> yl(ya);
+    rsoc_rotate(P, yl.data());
+    std::vector<double> s(P->c);
+    for (int64_t i = 0; i < P->user_m; ++i) {
+        const int64_t r = P->user_rows[i];
+        for (int32_t k = P->rowptr[r]; k < P->rowptr[r + 1]; ++k) s[P->col[k]] -= P->vals[k] * yl[r];
+    }
+    for (int64_t r = 0; r < P->m; ++r)
+        if (P->xrow_var[r] >= 0) ya[r] = s[P->xrow_var[r]];
+    return ya;
+}
+
+// library rows -> the caller's rows
+void user_y(const pdcs_problem *P, const double *ya, double *y) {
+    if (P->user_m < 0) {
+        std::copy(ya, ya + P->m, y);
+        return;
+    }
+    for (int64_t i = 0; i < P->user_m; ++i) y[i] = ya[P->user_rows[i]];
 }
 
 // Rotated-SOC blocks: rotate rows (r0, r0 + 1) of G and h of every block, then
@@ -1444,7 +1485,7 @@ int init_iterates(pdcs_problem *P, const pdcs_params *prm, const double *x0, con
         std::vector<double> Dr(n), Dl(m);
         HIPCHK(P->ctx, copy_sync(P->ctx, Dr.data(), P->Dr, sizeof(double) * n, hipMemcpyDeviceToHost));
         HIPCHK(P->ctx, copy_sync(P->ctx, Dl.data(), P->Dl, sizeof(double) * m, hipMemcpyDeviceToHost));
-        std::vector<double> y0r(y0, y0 + m);
+        std::vector<double> y0r = lift_y(P, y0);
         rsoc_rotate(P, y0r.data());
         for (int64_t j = 0; j < n; ++j) hx[j] = (T)(x0[j] * Dr[j]);
         for (int64_t i = 0; i < m; ++i) hy[i] = (T)(y0r[i] * Dl[i]);
@@ -1528,6 +1569,95 @@ int pdcs_problem_create_host(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, co
     return PDCS_OK;
 }
 
+int pdcs_problem_create_host_x(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double *c, const int32_t *rowptr,
+                               const int32_t *colidx, const double *vals, const double *h, const double *bl,
+                               const double *bu, const pdcs_cones *cones, const pdcs_primal_cones *xcones,
+                               pdcs_problem **out) {
+    if (!ctx) return fail(nullptr, PDCS_ERR_ARG, "ctx is NULL");
+    if (!xcones || (xcones->nsoc == 0 && xcones->nrsoc == 0 && xcones->nexp == 0 && xcones->ndualexp == 0))
+        return pdcs_problem_create_host(ctx, n, m, nb, c, rowptr, colidx, vals, h, bl, bu, cones, out);
+    if (!cones || !c || !rowptr || !colidx || !vals || !h || !out) return fail(ctx, PDCS_ERR_ARG, "NULL argument");
+    if (xcones->nsoc < 0 || xcones->nrsoc < 0 || xcones->nexp < 0 || xcones->ndualexp < 0 ||
+        (xcones->nsoc > 0 && !xcones->soc) || (xcones->nrsoc > 0 && !xcones->rsoc))
+        return fail(ctx, PDCS_ERR_ARG, "bad primal cone description");
+    if (cones->mzero < 0 || cones->mnonneg < 0 || cones->nsoc < 0 || cones->nrsoc < 0 || cones->nexp < 0 ||
+        cones->ndualexp < 0 || (cones->nsoc > 0 && !cones->soc) || (cones->nrsoc > 0 && !cones->rsoc) || nb < 0 ||
+        m < 0 || n <= 0)
+        return fail(ctx, PDCS_ERR_ARG, "bad sizes");
+    // primal blocks follow the nb box variables: SOC, rotated SOC, exp, dual exp
+    int64_t nx = 0;
+    for (int64_t b = 0; b < xcones->nsoc; ++b) {
+        if (xcones->soc[b] < 2) return fail(ctx, PDCS_ERR_ARG, "The size of the SOC must be greater than 1");
+        nx += xcones->soc[b];
+    }
+    for (int64_t b = 0; b < xcones->nrsoc; ++b) {
+        if (xcones->rsoc[b] < 2) return fail(ctx, PDCS_ERR_ARG, "The size of the rotated SOC must be greater than 1");
+        nx += xcones->rsoc[b];
+    }
+    nx += 3 * (xcones->nexp + xcones->ndualexp);
+    if (nb + nx > n) return fail(ctx, PDCS_ERR_ARG, "box and primal cone blocks exceed n");
+    // the caller's row groups
+    int64_t gs = 0, gr = 0;
+    for (int64_t b = 0; b < cones->nsoc; ++b) gs += cones->soc[b];
+    for (int64_t b = 0; b < cones->nrsoc; ++b) gr += cones->rsoc[b];
+    const int64_t g0 = cones->mzero + cones->mnonneg, g1 = g0 + gs, g2 = g1 + gr, g3 = g2 + 3 * cones->nexp;
+    if (g3 + 3 * cones->ndualexp != m) return fail(ctx, PDCS_ERR_ARG, "cone sizes do not add up to m");
+    if (rowptr[0] != 0) return fail(ctx, PDCS_ERR_ARG, "rowptr must start at 0");
+    for (int64_t i = 0; i < m; ++i)
+        if (rowptr[i + 1] < rowptr[i]) return fail(ctx, PDCS_ERR_ARG, "rowptr not monotone");
+    if ((int64_t)rowptr[m] + nx >= ((int64_t)1 << 31)) return fail(ctx, PDCS_ERR_ARG, "too many nonzeros");
+    std::vector<int32_t> rp{0}, ci;
+    std::vector<double> va, hh;
+    std::vector<int64_t> user_rows(m), xrow_var;
+    ci.reserve(rowptr[m] + nx);
+    va.reserve(rowptr[m] + nx);
+    auto user = [&](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) {
+            user_rows[i] = (int64_t)rp.size() - 1;
+            for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) { ci.push_back(colidx[k]); va.push_back(vals[k]); }
+            rp.push_back((int32_t)ci.size());
+            hh.push_back(h[i]);
+            xrow_var.push_back(-1);
+        }
+    };
+    auto xrows = [&](int64_t j0, int64_t len) {
+        for (int64_t j = j0; j < j0 + len; ++j) {
+            ci.push_back((int32_t)j);
+            va.push_back(1.0);
+            rp.push_back((int32_t)ci.size());
+            hh.push_back(0.0);
+            xrow_var.push_back(j);
+        }
+    };
+    // first variable of each primal group
+    int64_t xr = nb, xe;
+    for (int64_t b = 0; b < xcones->nsoc; ++b) xr += xcones->soc[b];
+    xe = xr;
+    for (int64_t b = 0; b < xcones->nrsoc; ++b) xe += xcones->rsoc[b];
+    const int64_t xs = nb, xd = xe + 3 * xcones->nexp;
+    user(0, g1);
+    xrows(xs, xr - xs);
+    user(g1, g2);
+    xrows(xr, xe - xr);
+    user(g2, g3);
+    xrows(xe, xd - xe);
+    user(g3, m);
+    xrows(xd, 3 * xcones->ndualexp);
+    std::vector<int64_t> soc(cones->soc, cones->soc + cones->nsoc), rsoc(cones->rsoc, cones->rsoc + cones->nrsoc);
+    soc.insert(soc.end(), xcones->soc, xcones->soc + xcones->nsoc);
+    rsoc.insert(rsoc.end(), xcones->rsoc, xcones->rsoc + xcones->nrsoc);
+    pdcs_cones k2{cones->mzero, cones->mnonneg, (int64_t)soc.size(), soc.data(), (int64_t)rsoc.size(), rsoc.data(),
+                  cones->nexp + xcones->nexp, cones->ndualexp + xcones->ndualexp};
+    const int64_t ma = (int64_t)rp.size() - 1;
+    const int rc = pdcs_problem_create_host(ctx, n, ma, nb, c, rp.data(), ci.data(), va.data(), hh.data(), bl, bu,
+                                            &k2, out);
+    if (rc) return rc;
+    (*out)->user_m = m;
+    (*out)->user_rows = std::move(user_rows);
+    (*out)->xrow_var = std::move(xrow_var);
+    return PDCS_OK;
+}
+
 int pdcs_problem_create_device(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double *c, const int32_t *rowptr,
                                const int32_t *colidx, const double *vals, const double *h, const double *bl,
                                const double *bu, const pdcs_cones *cones, pdcs_problem **out) {
@@ -1557,6 +1687,7 @@ int pdcs_problem_update_host(pdcs_problem *P, const double *c, const double *val
     pdcs_ctx *ctx = P->ctx;
     if (!P->rsoc_rows.empty())
         return fail(ctx, PDCS_ERR_UNSUPPORTED, "data update of a problem with rotated-SOC blocks");
+    if (P->user_m >= 0) return fail(ctx, PDCS_ERR_UNSUPPORTED, "data update of a problem with primal cone blocks");
     const int64_t nb = P->nb;
     // validate before changing anything
     const double *nbl = bl ? bl : P->bl.data(), *nbu = bu ? bu : P->bu.data();
@@ -1841,14 +1972,18 @@ int pdcs_get_solution(pdcs_problem *P, double *x, double *y, double *s) {
     if (!P->solved) return fail(P->ctx, PDCS_ERR_STATE, "no solution: call pdcs_solve first");
     if (x) std::memcpy(x, P->sol_x.data(), sizeof(double) * P->n);
     if (y) {
-        std::memcpy(y, P->sol_y.data(), sizeof(double) * P->m);
-        rsoc_rotate(P, y);
+        std::vector<double> ya(P->sol_y);
+        rsoc_rotate(P, ya.data());
+        user_y(P, ya.data(), y);
     }
     if (s) {
-        // s = c - G'y on the original data
+        // s = c - G'y over the caller's rows (a rotation keeps each block's G'y)
         for (int64_t j = 0; j < P->n; ++j) s[j] = P->c[j];
-        for (int64_t i = 0; i < P->m; ++i)
+        const int64_t mu = P->user_m < 0 ? P->m : P->user_m;
+        for (int64_t q = 0; q < mu; ++q) {
+            const int64_t i = P->user_m < 0 ? q : P->user_rows[q];
             for (int32_t k = P->rowptr[i]; k < P->rowptr[i + 1]; ++k) s[P->col[k]] -= P->vals[k] * P->sol_y[i];
+        }
     }
     return PDCS_OK;
 }
@@ -1858,7 +1993,7 @@ int pdcs_adjoint(pdcs_problem *P, const double *dL_dx, const double *dL_dy, doub
     if (!P || !dL_dx || !dL_dc || !dL_dh || !dL_dG) return fail(P ? P->ctx : nullptr, PDCS_ERR_ARG, "NULL argument");
     pdcs_ctx *ctx = P->ctx;
     if (!P->solved) return fail(ctx, PDCS_ERR_STATE, "no solution: call pdcs_solve first");
-    if (P->nexp + P->ndexp > 0 || !P->rsoc_rows.empty())
+    if (P->nexp + P->ndexp > 0 || !P->rsoc_rows.empty() || P->user_m >= 0)
         return fail(ctx, PDCS_ERR_UNSUPPORTED, "the adjoint supports zero, nonnegative and SOC rows");
     const int64_t n = P->n, m = P->m, N = n + m;
     if (N > 16384) return fail(ctx, PDCS_ERR_UNSUPPORTED, "the dense adjoint solve is limited to n + m <= 16384");
@@ -1919,7 +2054,11 @@ int pdcs_adjoint(pdcs_problem *P, const double *dL_dx, const double *dL_dy, doub
 int pdcs_get_scaling(pdcs_problem *P, double *Dr, double *Dl) {
     if (!P) return fail(nullptr, PDCS_ERR_ARG, "problem is NULL");
     if (Dr) HIPCHK(P->ctx, copy_sync(P->ctx, Dr, P->Dr, sizeof(double) * P->n, hipMemcpyDeviceToHost));
-    if (Dl) HIPCHK(P->ctx, copy_sync(P->ctx, Dl, P->Dl, sizeof(double) * P->m, hipMemcpyDeviceToHost));
+    if (Dl) {
+        std::vector<double> d(P->m);
+        HIPCHK(P->ctx, copy_sync(P->ctx, d.data(), P->Dl, sizeof(double) * P->m, hipMemcpyDeviceToHost));
+        user_y(P, d.data(), Dl);
+    }
     return PDCS_OK;
 }
 
@@ -1930,7 +2069,7 @@ int pdcs_evaluate(pdcs_problem *P, const double *x, const double *y, pdcs_result
     std::vector<double> Dr(P->n), Dl(P->m), xs(P->n), ys(P->m);
     HIPCHK(P->ctx, copy_sync(P->ctx, Dr.data(), P->Dr, sizeof(double) * P->n, hipMemcpyDeviceToHost));
     HIPCHK(P->ctx, copy_sync(P->ctx, Dl.data(), P->Dl, sizeof(double) * P->m, hipMemcpyDeviceToHost));
-    std::vector<double> yr(y, y + P->m);
+    std::vector<double> yr = lift_y(P, y);
     rsoc_rotate(P, yr.data());
     for (int64_t j = 0; j < P->n; ++j) xs[j] = x[j] * Dr[j];
     for (int64_t i = 0; i < P->m; ++i) ys[i] = yr[i] * Dl[i];

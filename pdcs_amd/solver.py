@@ -107,10 +107,20 @@ def make_cones(mzero=0, mnonneg=0, soc=(), rsoc=(), nexp=0, ndexp=0):
     return k
 
 
+def make_primal_cones(soc=(), rsoc=(), nexp=0, ndexp=0):
+    """Primal-side cone blocks after the box variables (pdcs_problem_create_host_x)."""
+    soc_a = np.ascontiguousarray(np.asarray(list(soc), dtype=np.int64).reshape(-1))
+    rsoc_a = np.ascontiguousarray(np.asarray(list(rsoc), dtype=np.int64).reshape(-1))
+    k = _lib.PrimalCones(len(soc_a), _p(soc_a, C.c_int64) if len(soc_a) else None,
+                         len(rsoc_a), _p(rsoc_a, C.c_int64) if len(rsoc_a) else None, nexp, ndexp)
+    k._keep = (soc_a, rsoc_a)
+    return k
+
+
 class Problem:
     """A conic program resident on the GPU (scaled at creation and at each data update)."""
 
-    def __init__(self, ctx: Context, n, m, nb, c, rowptr, colidx, vals, h, bl, bu, cones):
+    def __init__(self, ctx: Context, n, m, nb, c, rowptr, colidx, vals, h, bl, bu, cones, xcones=None):
         self.ctx = ctx
         self.n, self.m, self.nb = int(n), int(m), int(nb)
         self._arrs = [np.ascontiguousarray(c, dtype=np.float64),
@@ -123,10 +133,17 @@ class Problem:
         a = self._arrs
         self.nnz = len(a[3])
         self.cones = cones
+        self.xcones = xcones
         self.h = C.c_void_p()
-        _lib.check(ctx.L.pdcs_problem_create_host(ctx.h, self.n, self.m, self.nb, _p(a[0]), _p(a[1], C.c_int32),
-                                                  _p(a[2], C.c_int32), _p(a[3]), _p(a[4]), _p(a[5]), _p(a[6]),
-                                                  C.byref(cones), C.byref(self.h)), ctx.h)
+        if xcones is None:
+            _lib.check(ctx.L.pdcs_problem_create_host(ctx.h, self.n, self.m, self.nb, _p(a[0]), _p(a[1], C.c_int32),
+                                                      _p(a[2], C.c_int32), _p(a[3]), _p(a[4]), _p(a[5]), _p(a[6]),
+                                                      C.byref(cones), C.byref(self.h)), ctx.h)
+        else:
+            _lib.check(ctx.L.pdcs_problem_create_host_x(ctx.h, self.n, self.m, self.nb, _p(a[0]),
+                                                        _p(a[1], C.c_int32), _p(a[2], C.c_int32), _p(a[3]), _p(a[4]),
+                                                        _p(a[5]), _p(a[6]), C.byref(cones), C.byref(xcones),
+                                                        C.byref(self.h)), ctx.h)
         _live_problems.add(self)
 
     @classmethod
@@ -404,12 +421,12 @@ def rpdhg_gpu_solve(*, n, m, nb, c, G, h, mGzero, mGnonnegative, socG, rsocG=(),
     """Mirror of ``PDCS_GPU.rpdhg_gpu_solve`` (rpdhg_alg_gpu_gen.jl:1219).
 
     ``G`` is a ``(rowptr, colidx, vals)`` 0-based CSR triple or a scipy sparse
-    matrix.  Unsupported reference options raise ``ValueError`` instead of
-    silently changing the algorithm (primal-side cones, rotated SOC,
-    non-default rescaling methods, the non-aggressive main loops).
+    matrix.  Primal-side cones (soc_x, rsoc_x: block sizes; exp_x,
+    dual_exp_x: block counts; placed after the nb box variables) go through
+    pdcs_problem_create_host_x.  Unsupported reference options raise
+    ``ValueError`` instead of silently changing the algorithm (non-default
+    rescaling methods, the non-aggressive main loops).
     """
-    if len(soc_x) or len(rsoc_x) or exp_x or dual_exp_x:
-        raise ValueError("primal-side cones are not supported (the reference's are undefined, def_rpdhg_gen.jl:153)")
     if rescaling_method != "ruiz_pock_chambolle" or not use_preconditioner:
         raise ValueError("only rescaling_method=:ruiz_pock_chambolle with the preconditioner is supported")
     if not (use_adaptive_restart and use_adaptive_step_size_weight and use_resolving and use_aggressive
@@ -423,7 +440,9 @@ def rpdhg_gpu_solve(*, n, m, nb, c, G, h, mGzero, mGnonnegative, socG, rsocG=(),
         rowptr, colidx, vals = G
     ctx = ctx or default_context()
     cones = make_cones(mGzero, mGnonnegative, socG, rsocG, expG, dual_expG)
-    prob = Problem(ctx, n, m, nb, c, rowptr, colidx, vals, h, bl, bu, cones)
+    has_x = len(soc_x) or len(rsoc_x) or exp_x or dual_exp_x
+    xcones = make_primal_cones(soc_x, rsoc_x, exp_x, dual_exp_x) if has_x else None
+    prob = Problem(ctx, n, m, nb, c, rowptr, colidx, vals, h, bl, bu, cones, xcones)
     prm = _lib.default_params()
     prm.max_outer_iter = max_outer_iter
     prm.max_inner_iter = max_inner_iter
