@@ -27,14 +27,14 @@
 
 namespace pdcs {
 
-#define BFOR(i, N) for (int i = (int)threadIdx.x; i < (N); i += kBlock)
+#define BFOR(i, N) for (int i = (int)threadIdx.x; i < (N); i += kBatchBlock)
 
 // ------------------------------------------------------------- scaling --
 // Ruiz l_inf (10 passes) then Pock-Chambolle alpha = 1 on one instance
 // (rescale_problem!, src/pdcs_gpu/preprocess.jl:347-575), exponential blocks
 // with one row scale each.  G and G' values are divided by the same product,
 // so they stay bitwise transposes of each other.
-__global__ __launch_bounds__(kBlock) void k_batch_scale(BatchData B) {
+__global__ __launch_bounds__(kBatchBlock) void k_batch_scale(BatchData B) {
     const int q = blockIdx.x;
     const BatchInst I = B.inst[q];
     const int32_t *rp = B.rowptr + I.ro, *cl = B.col + I.ko, *trp = B.trowptr + I.tro, *tcl = B.tcol + I.ko;
@@ -80,7 +80,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_scale(BatchData B) {
 
 // scaled data in precision T, SOC scales d^2 = (Dl[start] / Dl[i])^2, zero iterates
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_batch_prepare(BatchData B, BatchVecs<T> V, const double *wx,
+__global__ __launch_bounds__(kBatchBlock) void k_batch_prepare(BatchData B, BatchVecs<T> V, const double *wx,
                                                           const double *wy) {
     const int q = blockIdx.x;
     const BatchInst I = B.inst[q];
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_prepare(BatchData B, BatchVecs
     for (int c = 0; c < I.nsoc; ++c) {
         const int st = B.soc_start[I.so + c], dim = B.soc_dim[I.so + c];
         const double d0 = Dl[st];
-        for (int e = threadIdx.x; e < dim; e += kBlock) {
+        for (int e = threadIdx.x; e < dim; e += kBatchBlock) {
             const double d = d0 / Dl[st + e];
             B.dsc2[I.yo + st + e] = d * d;
         }
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_prepare(BatchData B, BatchVecs
 }
 
 __global__ void k_batch_f32(const double *a, float *b, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    for (int64_t i = (int64_t)blockIdx.x * kBatchBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBatchBlock)
         b[i] = (float)a[i];
 }
 
@@ -127,7 +127,7 @@ struct BConv {   // converge_info_calculation_diagonal! ratios
 };
 
 struct BShared {
-    double red[(kBlock / 64) * 16];
+    double red[(kBatchBlock / 64) * 16];
     BConv ci[2];
 };
 
@@ -168,7 +168,7 @@ struct BSolve {
 
     // ---- reductions (broadcast to every thread)
     template <int N>
-    __device__ void reduce(double (&v)[N], const bool (&mx)[N]) { block_reduce_multi<N>(v, mx, sh->red); }
+    __device__ void reduce(double (&v)[N], const bool (&mx)[N]) { block_reduce_multi<N, kBatchBlock>(v, mx, sh->red); }
     __device__ double dot(const T *a, const T *b, int len) {
         BV;
         double s = 0.0;
@@ -183,7 +183,7 @@ struct BSolve {
     template <typename V, typename X>
     __device__ void seg_products(const int32_t *segk, int ns, const int32_t *col, const V *vv, const X *xv) {
         BV;
-        for (int s = threadIdx.x; s < ns; s += kBlock) {
+        for (int s = threadIdx.x; s < ns; s += kBatchBlock) {
             double acc = 0.0;
             for (int32_t k = segk[s]; k < segk[s + 1]; ++k) acc += (double)vv[k] * (double)xv[col[k]];
             part[s] = acc;
@@ -219,7 +219,7 @@ struct BSolve {
             if (i >= mzero) yv[i] = (T)fmax((double)yv[i], 0.0);
         }
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        for (int q = wave; q < nsoc; q += kBlock / 64) {
+        for (int q = wave; q < nsoc; q += kBatchBlock / 64) {
             const int st0 = ss[q], k = sd[q] - 1;
             T *yt = yv + st0 + 1;
             const double *dt = dsc2 + st0 + 1;
@@ -276,7 +276,7 @@ struct BSolve {
         }
         bar();
         if (dyv != nullptr) {
-            for (int i = cb + threadIdx.x; i < m; i += kBlock) {
+            for (int i = cb + threadIdx.x; i < m; i += kBatchBlock) {
                 const double d = (double)yv[i] - (double)ysv[i];
                 dyv[i] = (T)d;
                 my2 += d * d;
@@ -332,7 +332,7 @@ struct BSolve {
         bar();
         // plain SOC (code 5) and Kexp / Kexp* (codes 13 / 16) of the residual rows
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        for (int q = wave; q < nsoc; q += kBlock / 64) {
+        for (int q = wave; q < nsoc; q += kBatchBlock / 64) {
             const double *w = rtmp + ss[q];
             const int k = sd[q] - 1;
             double nx = 0.0;
@@ -727,7 +727,7 @@ struct BSolve {
 // empty (every workgroup reaches the exit).  polish_phase: only instances the
 // FP32 phase handed over (exit code 10) are run, continuing its state.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_batch_solve(BatchData B, const BView<T> *__restrict__ views,
+__global__ __launch_bounds__(kBatchBlock) void k_batch_solve(BatchData B, const BView<T> *__restrict__ views,
                                                         const BatchParams *__restrict__ Pp) {
     __shared__ BShared sh;
     __shared__ int qslot;
@@ -769,10 +769,10 @@ __global__ __launch_bounds__(kBlock) void k_batch_solve(BatchData B, const BView
         } else {
             double g = 0.0;
             const int nnz = B.inst[q].nnz;
-            for (int k = threadIdx.x; k < nnz; k += kBlock) g = fmax(g, fabs(view.val64[k]));
+            for (int k = threadIdx.x; k < nnz; k += kBatchBlock) g = fmax(g, fabs(view.val64[k]));
             double v[1] = {g};
             const bool mx[1] = {true};
-            block_reduce_multi<1>(v, mx, sh.red);
+            block_reduce_multi<1, kBatchBlock>(v, mx, sh.red);
             S.eta = fmax(1.0 / v[0], 1e-6);
             S.tau = S.sigma = 0.9;
             S.bs_t0 = 1.0;
@@ -824,7 +824,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_solve(BatchData B, const BView
 
 // FP32 -> FP64 iterates of the instances handed over to the polish; the polish
 // epoch starts from the sequence that was best at the hand-over
-__global__ __launch_bounds__(kBlock) void k_batch_promote(BatchData B, BatchVecs<float> A, BatchVecs<double> D) {
+__global__ __launch_bounds__(kBatchBlock) void k_batch_promote(BatchData B, BatchVecs<float> A, BatchVecs<double> D) {
     const int q = blockIdx.x;
     if (B.out[q].exit_code != 10) return;
     const BatchInst I = B.inst[q];
@@ -849,7 +849,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_promote(BatchData B, BatchVecs
 
 // unscaled solution of the returned sequence (recover_solution!, postprocess.jl:10)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_batch_recover(BatchData B, BatchVecs<T> V, double *xo, double *yo,
+__global__ __launch_bounds__(kBatchBlock) void k_batch_recover(BatchData B, BatchVecs<T> V, double *xo, double *yo,
                                                           int polished_only) {
     const int q = blockIdx.x;
     const BatchOut &O = B.out[q];
@@ -868,18 +868,18 @@ __global__ __launch_bounds__(kBlock) void k_batch_recover(BatchData B, BatchVecs
 
 // G' values of every instance from its G values through the transpose map
 // (instance-local slots), one workgroup per instance
-__global__ __launch_bounds__(kBlock) void k_batch_tvals(BatchData B) {
+__global__ __launch_bounds__(kBatchBlock) void k_batch_tvals(BatchData B) {
     const BatchInst I = B.inst[blockIdx.x];
-    for (int32_t k = threadIdx.x; k < I.nnz; k += kBlock) B.tval64[I.ko + k] = B.val64[I.ko + B.tmap[I.ko + k]];
+    for (int32_t k = threadIdx.x; k < I.nnz; k += kBatchBlock) B.tval64[I.ko + k] = B.val64[I.ko + B.tmap[I.ko + k]];
 }
 
 // ------------------------------------------------------------ launchers --
-void launch_batch_scale(hipStream_t s, const BatchData &B) { k_batch_scale<<<B.count, kBlock, 0, s>>>(B); }
-void launch_batch_tvals(hipStream_t s, const BatchData &B) { k_batch_tvals<<<B.count, kBlock, 0, s>>>(B); }
+void launch_batch_scale(hipStream_t s, const BatchData &B) { k_batch_scale<<<B.count, kBatchBlock, 0, s>>>(B); }
+void launch_batch_tvals(hipStream_t s, const BatchData &B) { k_batch_tvals<<<B.count, kBatchBlock, 0, s>>>(B); }
 
 template <typename T>
 void launch_batch_prepare(hipStream_t s, const BatchData &B, BatchVecs<T> V, const double *wx, const double *wy) {
-    k_batch_prepare<T><<<B.count, kBlock, 0, s>>>(B, V, wx, wy);
+    k_batch_prepare<T><<<B.count, kBatchBlock, 0, s>>>(B, V, wx, wy);
 }
 template void launch_batch_prepare<double>(hipStream_t, const BatchData &, BatchVecs<double>, const double *,
                                            const double *);
@@ -887,26 +887,26 @@ template void launch_batch_prepare<float>(hipStream_t, const BatchData &, BatchV
                                           const double *);
 
 void launch_batch_f32(hipStream_t s, const double *a, float *b, int64_t n) {
-    int64_t g = (n + kBlock - 1) / kBlock;
+    int64_t g = (n + kBatchBlock - 1) / kBatchBlock;
     if (g > 8192) g = 8192;
-    if (g > 0) k_batch_f32<<<(unsigned)g, kBlock, 0, s>>>(a, b, n);
+    if (g > 0) k_batch_f32<<<(unsigned)g, kBatchBlock, 0, s>>>(a, b, n);
 }
 
 template <typename T>
 void launch_batch_solve(hipStream_t s, const BatchData &B, const BView<T> *views, const BatchParams *p, int grid) {
-    k_batch_solve<T><<<grid, kBlock, 0, s>>>(B, views, p);
+    k_batch_solve<T><<<grid, kBatchBlock, 0, s>>>(B, views, p);
 }
 template void launch_batch_solve<double>(hipStream_t, const BatchData &, const BView<double> *, const BatchParams *, int);
 template void launch_batch_solve<float>(hipStream_t, const BatchData &, const BView<float> *, const BatchParams *, int);
 
 void launch_batch_promote(hipStream_t s, const BatchData &B, BatchVecs<float> A, BatchVecs<double> D) {
-    k_batch_promote<<<B.count, kBlock, 0, s>>>(B, A, D);
+    k_batch_promote<<<B.count, kBatchBlock, 0, s>>>(B, A, D);
 }
 
 template <typename T>
 void launch_batch_recover(hipStream_t s, const BatchData &B, BatchVecs<T> V, double *x_out, double *y_out,
                           int polished_only) {
-    k_batch_recover<T><<<B.count, kBlock, 0, s>>>(B, V, x_out, y_out, polished_only);
+    k_batch_recover<T><<<B.count, kBatchBlock, 0, s>>>(B, V, x_out, y_out, polished_only);
 }
 template void launch_batch_recover<double>(hipStream_t, const BatchData &, BatchVecs<double>, double *, double *, int);
 template void launch_batch_recover<float>(hipStream_t, const BatchData &, BatchVecs<float>, double *, double *, int);

@@ -413,7 +413,7 @@ int pdcs_batch_solve_warm(pdcs_batch *b, const pdcs_params *prm_in, const double
     HIPCHK(ctx, hipGetDevice(&dev));
     HIPCHK(ctx, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
     P.time_limit_ticks = std::isfinite(prm.time_limit) ? (int64_t)(prm.time_limit * khz * 1e3) : 0;
-    const int cap = batch_max_grid(kBlock);
+    const int cap = batch_max_grid(kBatchBlock);
     if (cap <= 0) return fail(ctx, PDCS_ERR_HIP, "occupancy query failed");
     const int grid = (int)std::min<int64_t>(b->count, cap);
     hipEvent_t e0, e1;
