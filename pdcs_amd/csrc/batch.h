@@ -15,12 +15,11 @@ namespace pdcs {
 // lengths (the portfolio rows range from 1 to n nonzeros).
 constexpr int kSegNnz = 32;
 // Threads per instance workgroup.  The solve kernel keeps the whole solver
-// state of its instance in registers and takes all of a lane's registers, so a
-// CU runs one wave per SIMD: one-wave workgroups would put 4 instances in
-// flight per CU, 256-thread workgroups put 1.  On the 4096-portfolio batch the
-// 256-thread form is faster (FP64 4.1 s against 5.1 s with one wave): an
-// instance ran 3.4 times longer on one wave, and the largest instances bound
-// the launch.
+// state of its instance in registers, so a SIMD holds one or two of its waves:
+// one-wave workgroups would put 4 to 8 instances in flight per CU, 256-thread
+// workgroups put 1 to 2.  On the 4096-portfolio batch the 256-thread form is
+// faster (FP64 4.1 s against 5.1 s with one wave, one wave per SIMD each): an
+// instance ran 3.4 times longer on one wave.
 constexpr int kBatchBlock = 256;
 
 struct BatchInst {
