@@ -85,6 +85,67 @@ class BatchConicSolve(torch.autograd.Function):
         return tuple(out)
 
 
+class _ResidentBatchSolve(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, layer, *flat):
+        layer._load(flat)
+        xs = layer._solve()
+        ctx.layer = layer
+        ctx.gen = layer._gen
+        ctx.data = tuple(t.detach() for t in flat)
+        ctx.dev = flat[0].device
+        return tuple(torch.as_tensor(x, dtype=torch.float64, device=ctx.dev) for x in xs)
+
+    @staticmethod
+    def backward(ctx, *gxs):
+        layer = ctx.layer
+        if layer._gen != ctx.gen:            # a later call replaced the data: solve this call's again
+            layer._load(ctx.data)
+            layer._solve()
+        grads = layer.batch.adjoint([_np(g) for g in gxs])
+        out = [None]
+        for g in grads:
+            if g is None:
+                raise RuntimeError("singular KKT Jacobian at a solution")
+            out += [torch.as_tensor(a, dtype=torch.float64, device=ctx.dev) for a in g]
+        return tuple(out)
+
+
+class BatchConicLayer(torch.nn.Module):
+    """Batched ConicLayer: one resident batch (pdcs_batch_*) re-solved per call
+    with that call's per-instance (c, h, G values) through
+    pdcs_batch_update_host; the forward takes c_0, h_0, vals_0, c_1, ... and
+    returns one solution per instance."""
+
+    def __init__(self, problems, ctx=None, **solve_kw):
+        super().__init__()
+        self.batch = Batch(problems, ctx)
+        self.count = len(problems)
+        self.solve_kw = solve_kw
+        self._gen = 0
+
+    def _load(self, flat):
+        k = self.count
+        self.batch.update(c=[_np(flat[3 * i]) for i in range(k)], h=[_np(flat[3 * i + 1]) for i in range(k)],
+                          vals=[_np(flat[3 * i + 2]) for i in range(k)])
+        self._gen += 1
+
+    def _solve(self):
+        res, _ = self.batch.solve(**self.solve_kw)
+        bad = [i for i, r in enumerate(res) if r["exit_code"] != 0]
+        if bad:
+            raise RuntimeError(f"instances {bad[:8]} did not converge")
+        return [x for x, _ in self.batch.solutions()]
+
+    def forward(self, *flat):
+        if len(flat) != 3 * self.count:
+            raise ValueError(f"expected {3 * self.count} tensors (c, h, vals per instance), got {len(flat)}")
+        return _ResidentBatchSolve.apply(self, *flat)
+
+    def close(self):
+        self.batch.close()
+
+
 class _ResidentSolve(torch.autograd.Function):
     @staticmethod
     def forward(ctx, c, h, vals, layer):

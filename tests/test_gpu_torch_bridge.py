@@ -85,3 +85,46 @@ def test_conic_layer_matches_conic_solve(ctx):
         for g, r in zip((t.grad for t in lv), (gc, gh, gv)):
             assert np.array_equal(g.cpu().numpy(), r)
     layer.close()
+
+
+def test_batch_conic_layer_matches_batch_conic_solve(ctx):
+    """The resident batched layer (Batch.update per call) against the per-call
+    BatchConicSolve: the same solutions and gradients bit for bit, also when
+    two calls precede one backward."""
+    from pdcs_amd.torch_layer import BatchConicLayer, BatchConicSolve
+    probs = [instances.workload4_layer(k) for k in range(5)]
+    rng = np.random.default_rng(21)
+    xts = [_cuda(rng.standard_normal(P.n)) for P in probs]
+    kw = dict(rel_tol=1e-8, abs_tol=1e-14, time_limit=30.0)
+
+    def data(shift):
+        return [(P.c + shift * rng.normal(0.0, 0.05, P.n), P.h, P.vals * (1 + shift * rng.uniform(-0.05, 0.05, P.nnz)))
+                for P in probs]
+
+    datas = [data(0.0), data(1.0)]
+
+    def leaves(d):
+        return [_cuda(a).requires_grad_(True) for trip in d for a in trip]
+
+    def loss(xs):
+        return sum(((x - t) ** 2).sum() for x, t in zip(xs, xts))
+
+    ref = []
+    for d in datas:
+        lv = leaves(d)
+        xs = BatchConicSolve.apply(probs, kw, *lv)
+        loss(xs).backward()
+        ref.append(([x.detach().cpu().numpy() for x in xs], [t.grad.cpu().numpy() for t in lv]))
+    layer = BatchConicLayer(probs, ctx, **kw)
+    for d, (xr, gr) in zip(datas, ref):
+        lv = leaves(d)
+        xs = layer(*lv)
+        loss(xs).backward()
+        assert all(np.array_equal(x.detach().cpu().numpy(), r) for x, r in zip(xs, xr))
+        assert all(np.array_equal(t.grad.cpu().numpy(), r) for t, r in zip(lv, gr))
+    lvs = [leaves(d) for d in datas]
+    outs = [layer(*lv) for lv in lvs]
+    (loss(outs[0]) + loss(outs[1])).backward()
+    for lv, (_, gr) in zip(lvs, ref):
+        assert all(np.array_equal(t.grad.cpu().numpy(), r) for t, r in zip(lv, gr))
+    layer.close()
