@@ -169,6 +169,11 @@ int pdcs_problem_create_device(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, 
                                const int32_t *rowptr, const int32_t *colidx, const double *vals,
                                const double *h, const double *bl, const double *bu,
                                const pdcs_cones *cones, pdcs_problem **out);
+/* pdcs_problem_create_device with primal-side cones (as pdcs_problem_create_host_x). */
+int pdcs_problem_create_device_x(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double *c,
+                                 const int32_t *rowptr, const int32_t *colidx, const double *vals,
+                                 const double *h, const double *bl, const double *bu,
+                                 const pdcs_cones *cones, const pdcs_primal_cones *xcones, pdcs_problem **out);
 int pdcs_problem_destroy(pdcs_problem *p);
 /* New data on the same sparsity pattern: c (n), the values of G in the order
  * of the creation call (nnz), h (m), bl and bu (nb); NULL keeps an array.  The

@@ -81,7 +81,8 @@ class BatchResult(C.Structure):
 
 # every symbol declared in include/pdcs.h
 EXPORTS = ["pdcs_params_default", "pdcs_ctx_create", "pdcs_ctx_destroy", "pdcs_last_error",
-           "pdcs_problem_create_host", "pdcs_problem_create_host_x", "pdcs_problem_create_device", "pdcs_problem_destroy",
+           "pdcs_problem_create_host", "pdcs_problem_create_host_x", "pdcs_problem_create_device",
+           "pdcs_problem_create_device_x", "pdcs_problem_destroy",
            "pdcs_problem_update_host", "pdcs_problem_update_device",
            "pdcs_solve", "pdcs_get_solution", "pdcs_get_scaling", "pdcs_evaluate", "pdcs_spmv",
            "pdcs_project_dual", "pdcs_project_primal_cone", "pdcs_time_kernels",
@@ -115,6 +116,8 @@ def load():
     L.pdcs_problem_create_host_x.argtypes = [vp, c_i64, c_i64, c_i64, c_dp, C.POINTER(c_i32), C.POINTER(c_i32),
                                              c_dp, c_dp, c_dp, c_dp, C.POINTER(Cones), C.POINTER(PrimalCones),
                                              C.POINTER(vp)]
+    L.pdcs_problem_create_device_x.argtypes = [vp, c_i64, c_i64, c_i64, vp, vp, vp, vp, vp, vp, vp,
+                                               C.POINTER(Cones), C.POINTER(PrimalCones), C.POINTER(vp)]
     L.pdcs_problem_create_device.argtypes = [vp, c_i64, c_i64, c_i64, vp, vp, vp, vp, vp, vp, vp,
                                              C.POINTER(Cones), C.POINTER(vp)]
     L.pdcs_problem_destroy.argtypes = [vp]

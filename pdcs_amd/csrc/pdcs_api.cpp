@@ -1661,7 +1661,15 @@ int pdcs_problem_create_host_x(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, 
 int pdcs_problem_create_device(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double *c, const int32_t *rowptr,
                                const int32_t *colidx, const double *vals, const double *h, const double *bl,
                                const double *bu, const pdcs_cones *cones, pdcs_problem **out) {
+    return pdcs_problem_create_device_x(ctx, n, m, nb, c, rowptr, colidx, vals, h, bl, bu, cones, nullptr, out);
+}
+
+int pdcs_problem_create_device_x(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, const double *c,
+                                 const int32_t *rowptr, const int32_t *colidx, const double *vals, const double *h,
+                                 const double *bl, const double *bu, const pdcs_cones *cones,
+                                 const pdcs_primal_cones *xcones, pdcs_problem **out) {
     if (!ctx || !rowptr) return fail(ctx, PDCS_ERR_ARG, "NULL argument");
+    if (n <= 0 || m < 0 || nb < 0 || nb > n) return fail(ctx, PDCS_ERR_ARG, "bad sizes");
     // stage to the host: the transpose and the cone layout are built there
     std::vector<int32_t> hr(m + 1);
     if (copy_sync(ctx, hr.data(), rowptr, sizeof(int32_t) * (m + 1), hipMemcpyDeviceToHost) != hipSuccess)
@@ -1677,8 +1685,8 @@ int pdcs_problem_create_device(pdcs_ctx *ctx, int64_t n, int64_t m, int64_t nb, 
         HIPCHK(ctx, copy_sync(ctx, hbl.data(), bl, sizeof(double) * nb, hipMemcpyDeviceToHost));
         HIPCHK(ctx, copy_sync(ctx, hbu.data(), bu, sizeof(double) * nb, hipMemcpyDeviceToHost));
     }
-    return pdcs_problem_create_host(ctx, n, m, nb, hcv.data(), hr.data(), hc.data(), hv.data(), hh.data(), hbl.data(),
-                                    hbu.data(), cones, out);
+    return pdcs_problem_create_host_x(ctx, n, m, nb, hcv.data(), hr.data(), hc.data(), hv.data(), hh.data(),
+                                      hbl.data(), hbu.data(), cones, xcones, out);
 }
 
 int pdcs_problem_update_host(pdcs_problem *P, const double *c, const double *vals, const double *h,

@@ -147,16 +147,19 @@ class Problem:
         _live_problems.add(self)
 
     @classmethod
-    def from_tensors(cls, n, m, nb, c, rowptr, colidx, vals, h, bl, bu, cones, ctx: Context | None = None):
-        """A problem from GPU tensors (pdcs_problem_create_device): c (n), CSR
-        rowptr (m+1) / colidx (nnz) int32, vals (nnz), h (m), bl / bu (nb),
-        float64, contiguous; cones from make_cones."""
+    def from_tensors(cls, n, m, nb, c, rowptr, colidx, vals, h, bl, bu, cones, ctx: Context | None = None,
+                     xcones=None):
+        """A problem from GPU tensors (pdcs_problem_create_device[_x]): c (n),
+        CSR rowptr (m+1) / colidx (nnz) int32, vals (nnz), h (m), bl / bu (nb),
+        float64, contiguous; cones from make_cones, xcones (primal-side cone
+        blocks) from make_primal_cones or None."""
         ctx = ctx or default_context()
         self = cls.__new__(cls)
         self.ctx = ctx
         self.n, self.m, self.nb = int(n), int(m), int(nb)
         self.nnz = int(vals.numel())
         self.cones = cones
+        self.xcones = xcones
         self.h = C.c_void_p()
         ptrs = [_device_ptr(c, "c", self.n, "f64"), _device_ptr(rowptr, "rowptr", self.m + 1, "i32"),
                 _device_ptr(colidx, "colidx", self.nnz, "i32"), _device_ptr(vals, "vals", self.nnz, "f64"),
@@ -164,8 +167,9 @@ class Problem:
                 _device_ptr(bu, "bu", self.nb, "f64")]
         _sync_producers([c, rowptr, colidx, vals, h, bl, bu])
         self._arrs = [None] * 7
-        _lib.check(ctx.L.pdcs_problem_create_device(ctx.h, self.n, self.m, self.nb, *ptrs, C.byref(cones),
-                                                    C.byref(self.h)), ctx.h)
+        _lib.check(ctx.L.pdcs_problem_create_device_x(ctx.h, self.n, self.m, self.nb, *ptrs, C.byref(cones),
+                                                      C.byref(xcones) if xcones is not None else None,
+                                                      C.byref(self.h)), ctx.h)
         _live_problems.add(self)
         return self
 

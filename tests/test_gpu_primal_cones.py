@@ -144,6 +144,14 @@ def test_primal_cones_equal_their_row_form(ctx):
         px.update(c=c)
     with pytest.raises(_lib.PdcsError):
         px.adjoint(np.ones(n))
+    # the same problem from GPU tensors (pdcs_problem_create_device_x)
+    torch = pytest.importorskip("torch")
+    dev = lambda a, dt=torch.float64: torch.as_tensor(np.asarray(a), dtype=dt).to("cuda").contiguous()
+    pt = Problem.from_tensors(n, m, NB, dev(c), dev(rp, torch.int32), dev(ci, torch.int32), dev(va), dev(h), dev(bl),
+                              dev(bu), make_cones(MZ, MN, SOC, URSOC, EXPG, 0), ctx, xcones=xk)
+    rt = pt.solve(precision=1, **KW)
+    assert _fields(rt) == _fields(rx)
+    assert all(np.array_equal(a, b) for a, b in zip(pt.solution(), px.solution()))
 
 
 def _one_block(G, h, c, **xc):
