@@ -1723,6 +1723,10 @@ int pdcs_problem_update_device(pdcs_problem *P, const double *c, const double *v
                                const double *bl, const double *bu) {
     if (!P) return fail(nullptr, PDCS_ERR_ARG, "problem is NULL");
     pdcs_ctx *ctx = P->ctx;
+    // rejected before staging: the caller's arrays have the caller's sizes
+    if (!P->rsoc_rows.empty())
+        return fail(ctx, PDCS_ERR_UNSUPPORTED, "data update of a problem with rotated-SOC blocks");
+    if (P->user_m >= 0) return fail(ctx, PDCS_ERR_UNSUPPORTED, "data update of a problem with primal cone blocks");
     std::vector<double> hc, hv, hh, hbl, hbu;
     auto stage = [&](std::vector<double> &dst, const double *src, int64_t len) -> int {
         if (!src) return PDCS_OK;
