@@ -152,6 +152,8 @@ def test_primal_cones_equal_their_row_form(ctx):
     rt = pt.solve(precision=1, **KW)
     assert _fields(rt) == _fields(rx)
     assert all(np.array_equal(a, b) for a, b in zip(pt.solution(), px.solution()))
+    with pytest.raises(_lib.PdcsError):
+        pt.update_tensors(h=dev(h))
 
 
 def _one_block(G, h, c, **xc):
