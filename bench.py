@@ -82,10 +82,11 @@ def batch_section(a, np, cpu):
     B.close()
     sample = [probs[i] for i in range(0, len(probs), max(1, len(probs) // 16))][:16]
     ts = []
+    graph = 0 if a.no_graph else 1
     for P in sample:
         prob = Problem.from_instance(P)
-        prob.solve(time_limit=60.0)
-        ts.append(prob.solve(time_limit=60.0)["time_solve"])
+        prob.solve(time_limit=60.0, use_graph=graph)
+        ts.append(prob.solve(time_limit=60.0, use_graph=graph)["time_solve"])
         prob.close()
     out["single_instance_path_s_mean"] = round(float(np.mean(ts)), 5)
     out["single_instance_path_est_total_s"] = round(float(np.mean(ts)) * len(probs), 2)
@@ -117,8 +118,8 @@ def large_section(a, np):
     out = {"workload": "large SOC robust regression 1600000x400000 density 1.25e-05", "n": P.n, "m": P.m,
            "nnz": P.nnz, "setup_s": round(time.time() - t0, 2)}
     for prec, tag, kw in ((1, "mixed", {}), (0, "fp64", {}), (0, "fp64_check200", {"check_terminate_freq": 200})):
-        prob.solve(precision=prec, time_limit=600.0, **kw)               # warm-up
-        r = prob.solve(precision=prec, time_limit=600.0, **kw)
+        prob.solve(precision=prec, time_limit=600.0, use_graph=0 if a.no_graph else 1, **kw)   # warm-up
+        r = prob.solve(precision=prec, time_limit=600.0, use_graph=0 if a.no_graph else 1, **kw)
         out[tag] = {"time_to_tol_s": round(r["time_solve"], 4), "iters": r["iter"], "iter_fp32": r["iter_fp32"],
                     "exit_code": r["exit_code"], "pobj": r["pobj"],
                     "readme_residuals": [r["rel_primal_res"], r["rel_dual_res"], r["rel_gap"]]}
