@@ -421,7 +421,7 @@ def rpdhg_gpu_solve(*, n, m, nb, c, G, h, mGzero, mGnonnegative, socG, rsocG=(),
                     eps_dual_infeasible_high_acc=1e-16, print_freq=2000, check_terminate_freq=2000,
                     verbose=1, time_limit=float("inf"), method="average", logfile_name=None,
                     use_kkt_restart=False, kkt_restart_freq=2000, use_duality_gap_restart=True,
-                    duality_gap_restart_freq=2000, precision="fp64", ctx: Context | None = None):
+                    duality_gap_restart_freq=2000, precision="fp64", ctx: Context | None = None, _prob=None):
     """Mirror of ``PDCS_GPU.rpdhg_gpu_solve`` (rpdhg_alg_gpu_gen.jl:1219).
 
     ``G`` is a ``(rowptr, colidx, vals)`` 0-based CSR triple or a scipy sparse
@@ -436,17 +436,20 @@ def rpdhg_gpu_solve(*, n, m, nb, c, G, h, mGzero, mGnonnegative, socG, rsocG=(),
     if not (use_adaptive_restart and use_adaptive_step_size_weight and use_resolving and use_aggressive
             and not use_accelerated and method == "average"):
         raise ValueError("only the aggressive restarted adaptive main loop (the reference GPU default) is supported")
-    if hasattr(G, "tocsr"):
-        Gc = G.tocsr()
-        Gc.sort_indices()
-        rowptr, colidx, vals = Gc.indptr, Gc.indices, Gc.data
+    if _prob is not None:
+        prob = _prob
     else:
-        rowptr, colidx, vals = G
-    ctx = ctx or default_context()
-    cones = make_cones(mGzero, mGnonnegative, socG, rsocG, expG, dual_expG)
-    has_x = len(soc_x) or len(rsoc_x) or exp_x or dual_exp_x
-    xcones = make_primal_cones(soc_x, rsoc_x, exp_x, dual_exp_x) if has_x else None
-    prob = Problem(ctx, n, m, nb, c, rowptr, colidx, vals, h, bl, bu, cones, xcones)
+        if hasattr(G, "tocsr"):
+            Gc = G.tocsr()
+            Gc.sort_indices()
+            rowptr, colidx, vals = Gc.indptr, Gc.indices, Gc.data
+        else:
+            rowptr, colidx, vals = G
+        ctx = ctx or default_context()
+        cones = make_cones(mGzero, mGnonnegative, socG, rsocG, expG, dual_expG)
+        has_x = len(soc_x) or len(rsoc_x) or exp_x or dual_exp_x
+        xcones = make_primal_cones(soc_x, rsoc_x, exp_x, dual_exp_x) if has_x else None
+        prob = Problem(ctx, n, m, nb, c, rowptr, colidx, vals, h, bl, bu, cones, xcones)
     prm = _lib.default_params()
     prm.max_outer_iter = max_outer_iter
     prm.max_inner_iter = max_inner_iter
@@ -473,3 +476,22 @@ def rpdhg_gpu_solve(*, n, m, nb, c, G, h, mGzero, mGnonnegative, socG, rsocG=(),
     x, y, s = prob.solution()
     prob.close()
     return _solution_struct(x, y, s, res)
+
+
+def rpdhg_gpu_solve_input_gpu_data(*, n, m, nb, c, G, h, mGzero, mGnonnegative, socG, rsocG=(), expG=0,
+                                   dual_expG=0, bl, bu, soc_x=(), rsoc_x=(), exp_x=0, dual_exp_x=0,
+                                   ctx: Context | None = None, **kw):
+    """Mirror of ``PDCS_GPU.rpdhg_gpu_solve_input_gpu_data``
+    (rpdhg_alg_gpu_gen.jl:387): the data are GPU tensors (c, h, bl, bu
+    float64; G a (rowptr, colidx, vals) CSR triple of int32, int32, float64
+    tensors) that go in through pdcs_problem_create_device[_x]; the other
+    keywords and the result are those of :func:`rpdhg_gpu_solve`."""
+    rowptr, colidx, vals = G
+    ctx = ctx or default_context()
+    cones = make_cones(mGzero, mGnonnegative, socG, rsocG, expG, dual_expG)
+    has_x = len(soc_x) or len(rsoc_x) or exp_x or dual_exp_x
+    xcones = make_primal_cones(soc_x, rsoc_x, exp_x, dual_exp_x) if has_x else None
+    prob = Problem.from_tensors(n, m, nb, c, rowptr, colidx, vals, h, bl, bu, cones, ctx, xcones=xcones)
+    return rpdhg_gpu_solve(n=n, m=m, nb=nb, c=None, G=None, h=None, mGzero=mGzero, mGnonnegative=mGnonnegative,
+                           socG=socG, rsocG=rsocG, expG=expG, dual_expG=dual_expG, bl=None, bu=None, soc_x=soc_x,
+                           rsoc_x=rsoc_x, exp_x=exp_x, dual_exp_x=dual_exp_x, ctx=ctx, _prob=prob, **kw)

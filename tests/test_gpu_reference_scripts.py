@@ -16,7 +16,7 @@ import pytest
 from oracle import oracle
 from pdcs_amd import instances
 from pdcs_amd.model import Model
-from pdcs_amd.solver import Problem, make_cones, rpdhg_gpu_solve
+from pdcs_amd.solver import Problem, make_cones, rpdhg_gpu_solve, rpdhg_gpu_solve_input_gpu_data
 
 pytestmark = pytest.mark.gpu
 
@@ -81,6 +81,13 @@ def test_reference_example_script(ctx, cone):
     xt, _, _ = pt.solution()
     assert rt["exit_code"] == 0 and rt["pobj"] == mdl.objective_value() and rt["iter"] == mdl.iterations()
     assert np.array_equal(xt, mdl.value())
+    # the reference's second entry point on the same tensors
+    sg = rpdhg_gpu_solve_input_gpu_data(n=N, m=M, nb=N, c=dev(c), G=(dev(rowptr, torch.int32),
+                                        dev(colidx, torch.int32), dev(vals)), h=dev(b), mGzero=MZ, mGnonnegative=MN,
+                                        socG=socG, expG=expG, bl=dev(np.zeros(N)), bu=dev(np.full(N, np.inf)),
+                                        verbose=0, ctx=ctx, **KW)
+    assert sg.info.exit_code == 0
+    assert abs(sg.info.pObj - sol.info.pObj) <= 2e-6 * (1 + abs(sol.info.pObj))
     # the oracle's optimum
     P = instances.ConicProblem(n=N, m=M, nb=N, c=c, rowptr=rowptr, colidx=colidx, vals=vals, h=b,
                                bl=np.zeros(N), bu=np.full(N, np.inf), mzero=MZ, mnonneg=MN, soc=socG, nexp=expG,
